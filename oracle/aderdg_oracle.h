@@ -1,0 +1,103 @@
+/*
+ * ADER-DG oracle: a plain-C FP64 restatement of the reference's generic
+ * ("generic::c") ADER-DG kernels, used ONLY as test infrastructure.
+ *
+ * Who may use this: the tests under tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py -- and only as the checker or the CPU
+ * baseline, never as the product path.  The product (the HIP library behind
+ * include/exahype_amd.h) never links, loads or calls anything in oracle/.
+ *
+ * Every function cites the reference file:line it restates.  Paths are
+ * relative to the reference root (ExaHyPE-Engine).
+ *
+ * Parity pins (what fixes each function independently of this code):
+ *   tables          legendre_tables.json (values printed by the reference
+ *                   generator, GaussLegendreBasis.csnippet)
+ *   euler flux      golden testPDEFluxes (2D/3D)
+ *   stp_nonlinear   golden testSpaceTimePredictorNonlinear (2D/3D, uniform
+ *                   state: layout + extrapolation) + brute-force collocation
+ *                   solve for a linear flux + fixed-point residual identity
+ *   volume integral golden testVolumeIntegralNonlinear is disabled in the
+ *                   reference; pinned by the exact-derivative identity
+ *                   (polynomial flux of degree <= N) and conservation
+ *   riemann         golden testRiemannSolverNonlinear (2D/3D)
+ *   face/surface    golden testSurfaceIntegralNonlinear (2D/3D)
+ *   solution update golden testSolutionUpdate (2D/3D)
+ *   stable dt       closed form CFL*PNPM/sum_d((|u_d|+c)/dx_d), uniform state
+ *   linear CK path  closed-form plane wave (design-order convergence)
+ *   amr projections constant preservation + polynomial reproduction
+ */
+#ifndef EXAHYPE_AMD_ORACLE_H
+#define EXAHYPE_AMD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAXB 16
+
+/* Operator tables for one order N (B = N+1 nodes), laid out as the
+ * reference's kernels::legendre::*[order] arrays (row-major [i][j]). */
+typedef struct orc_basis {
+  int order;
+  double w[ORC_MAXB];                 /* weights on [0,1] */
+  double x[ORC_MAXB];                 /* Gauss-Legendre nodes on [0,1] */
+  double Kxi[ORC_MAXB * ORC_MAXB];    /* Kxi[i][j] */
+  double iK1[ORC_MAXB * ORC_MAXB];    /* iK1[i][j] */
+  double dudx[ORC_MAXB * ORC_MAXB];   /* dudx[i][j] */
+  double F0[ORC_MAXB];                /* FCoeff[order][0] = phi(0) */
+  double F1[ORC_MAXB];                /* FCoeff[order][1] = phi(1) */
+  double fgp[3 * ORC_MAXB * ORC_MAXB];/* fineGridProjector[order][sub][i][j] */
+} orc_basis;
+
+enum { ORC_PDE_EULER = 0, ORC_PDE_ACOUSTIC = 1, ORC_PDE_ADVECTION = 2 };
+
+typedef struct orc_solver {
+  int dim;            /* 2 or 3 */
+  int nvar;           /* NumberOfVariables */
+  int npar;           /* NumberOfParameters */
+  int pde;            /* ORC_PDE_* */
+  int use_max_picard; /* UseMaxPicardIterations */
+  int max_picard;     /* MaxPicardIterations */
+  double cfl;         /* CFL */
+  double pnpm;        /* PNPM */
+  double adv[3];      /* advection velocity (ORC_PDE_ADVECTION only) */
+  orc_basis b;
+} orc_solver;
+
+/* ---- PDE hooks ---------------------------------------------------------- */
+void orc_flux(const orc_solver* s, const double* Q, double* F /* [dim][nvar] */);
+void orc_eigenvalues(const orc_solver* s, const double* Q, int d, double* lambda);
+
+/* ---- nonlinear cell kernels (reference generic c/{2d,3d}) --------------- */
+int orc_stp_nonlinear(const orc_solver* s, const double* luh, const double* invDx,
+                      double dt, double* lQi, double* rhs, double* lFi,
+                      double* lQhi, double* lFhi, double* lQhbnd, double* lFhbnd);
+void orc_volume_integral_nonlinear(const orc_solver* s, double* lduh,
+                                   const double* lFhi, const double* dx);
+void orc_riemann_nonlinear(const orc_solver* s, double* FL, double* FR,
+                           const double* QL, const double* QR, int direction);
+void orc_face_integral_nonlinear(const orc_solver* s, double* lduh, const double* lFhbnd,
+                                 int direction, int orientation, const double* dx);
+void orc_surface_integral_nonlinear(const orc_solver* s, double* lduh,
+                                    const double* lFhbnd, const double* dx);
+void orc_solution_update(const orc_solver* s, double* luh, const double* luhOld,
+                         const double* lduh, double dt);
+double orc_stable_dt(const orc_solver* s, const double* luh, const double* dx);
+
+/* Glue: fusedSpaceTimePredictorVolumeIntegral with addToUpdate == true. */
+int orc_fused_stp_vi(const orc_solver* s, double* lduh, double* lQhbnd, double* lFhbnd,
+                     const double* luh, const double* dx, double dt);
+
+/* ---- uniform periodic mesh driver (time loop semantics) ----------------- */
+/* luh: [ncells][B^D][nData], cells ordered x fastest.  Runs `steps` global
+ * time steps in place.  dt_io: in = first dt (<=0: compute from luh),
+ * out = dt of the next step.  picard_out (nullable): per-cell Picard
+ * counts of the last step.  threads: OpenMP threads (<=0: default). */
+int orc_run_uniform(const orc_solver* s, double* luh, int n_per_dim, double h,
+                    int steps, double* dt_io, int* picard_out, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,250 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py import this module, and only as the checker / CPU baseline.  The
+operator tables come from ``tests/golden/legendre_tables.json`` -- the values
+the reference generator printed (ExaHyPE/kernels/generated/GaussLegendreBasis.csnippet).
+"""
+import ctypes
+import json
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "liborc.so")
+TABLES_PATH = os.path.join(ROOT, "tests", "golden", "legendre_tables.json")
+
+MAXB = 16
+PDE_EULER, PDE_ACOUSTIC, PDE_ADVECTION = 0, 1, 2
+
+# PNPM per order (Toolkit/exahype/toolkit/solverController.py:17-23).
+PNPM = [1.0, 0.33, 0.17, 0.1, 0.069, 0.045, 0.038, 0.03, 0.02, 0.015]
+
+
+class Basis(ctypes.Structure):
+    _fields_ = [
+        ("order", ctypes.c_int),
+        ("w", ctypes.c_double * MAXB),
+        ("x", ctypes.c_double * MAXB),
+        ("Kxi", ctypes.c_double * (MAXB * MAXB)),
+        ("iK1", ctypes.c_double * (MAXB * MAXB)),
+        ("dudx", ctypes.c_double * (MAXB * MAXB)),
+        ("F0", ctypes.c_double * MAXB),
+        ("F1", ctypes.c_double * MAXB),
+        ("fgp", ctypes.c_double * (3 * MAXB * MAXB)),
+    ]
+
+
+class Solver(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("nvar", ctypes.c_int),
+        ("npar", ctypes.c_int),
+        ("pde", ctypes.c_int),
+        ("use_max_picard", ctypes.c_int),
+        ("max_picard", ctypes.c_int),
+        ("cfl", ctypes.c_double),
+        ("pnpm", ctypes.c_double),
+        ("adv", ctypes.c_double * 3),
+        ("b", Basis),
+    ]
+
+
+_tables = None
+
+
+def tables(order):
+    """Reference operator tables for ``order`` as numpy arrays."""
+    global _tables
+    if _tables is None:
+        with open(TABLES_PATH) as f:
+            _tables = json.load(f)["tables"]
+    t = _tables[str(order)]
+    return {
+        "w": np.array(t["weights"]),
+        "x": np.array(t["nodes"]),
+        "Kxi": np.array(t["Kxi"]),
+        "iK1": np.array(t["iK1"]),
+        "dudx": np.array(t["dudx"]),
+        "F0": np.array(t["FCoeff"][0]),
+        "F1": np.array(t["FCoeff"][1]),
+        "fgp": np.array(t["fineGridProjector"]),
+        "equidistant": np.array(t["equidistantGridProjector"]),
+    }
+
+
+def _lib():
+    if not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", HERE])
+    lib = ctypes.CDLL(LIB_PATH)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    sp = ctypes.POINTER(Solver)
+    lib.orc_flux.argtypes = [sp, dp, dp]
+    lib.orc_eigenvalues.argtypes = [sp, dp, ctypes.c_int, dp]
+    lib.orc_stp_nonlinear.argtypes = [sp, dp, dp, ctypes.c_double] + [dp] * 7
+    lib.orc_stp_nonlinear.restype = ctypes.c_int
+    lib.orc_volume_integral_nonlinear.argtypes = [sp, dp, dp, dp]
+    lib.orc_riemann_nonlinear.argtypes = [sp, dp, dp, dp, dp, ctypes.c_int]
+    lib.orc_face_integral_nonlinear.argtypes = [sp, dp, dp, ctypes.c_int, ctypes.c_int, dp]
+    lib.orc_surface_integral_nonlinear.argtypes = [sp, dp, dp, dp]
+    lib.orc_solution_update.argtypes = [sp, dp, dp, dp, ctypes.c_double]
+    lib.orc_stable_dt.argtypes = [sp, dp, dp]
+    lib.orc_stable_dt.restype = ctypes.c_double
+    lib.orc_fused_stp_vi.argtypes = [sp, dp, dp, dp, dp, dp, ctypes.c_double]
+    lib.orc_fused_stp_vi.restype = ctypes.c_int
+    lib.orc_run_uniform.argtypes = [sp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, ip, ctypes.c_int]
+    lib.orc_run_uniform.restype = ctypes.c_int
+    return lib
+
+
+LIB = None
+
+
+def lib():
+    global LIB
+    if LIB is None:
+        LIB = _lib()
+    return LIB
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Oracle:
+    """One solver configuration (dimension, order, variables, PDE)."""
+
+    def __init__(self, dim, order, nvar=5, npar=0, pde=PDE_EULER, cfl=0.9, pnpm=None,
+                 max_picard=None, adv=(1.0, 0.5, 0.25), table_override=None):
+        self.dim, self.order, self.nvar, self.npar = dim, order, nvar, npar
+        self.B = order + 1
+        s = Solver()
+        s.dim, s.nvar, s.npar, s.pde = dim, nvar, npar, pde
+        s.use_max_picard = 1 if max_picard is not None else 0
+        s.max_picard = max_picard if max_picard is not None else order + 1
+        s.cfl = cfl
+        s.pnpm = PNPM[order] if pnpm is None else pnpm
+        for i in range(3):
+            s.adv[i] = adv[i]
+        t = dict(tables(order))
+        if table_override:
+            t.update(table_override)
+        self.t = t
+        b = s.b
+        b.order = order
+        B = self.B
+        for i in range(B):
+            b.w[i] = t["w"][i]
+            b.x[i] = t["x"][i]
+            b.F0[i] = t["F0"][i]
+            b.F1[i] = t["F1"][i]
+            for j in range(B):
+                b.Kxi[i * B + j] = t["Kxi"][i][j]
+                b.iK1[i * B + j] = t["iK1"][i][j]
+                b.dudx[i * B + j] = t["dudx"][i][j]
+                for sub in range(3):
+                    b.fgp[(sub * B + i) * B + j] = t["fgp"][sub][i][j]
+        self.s = s
+        self.ps = ctypes.pointer(s)
+
+    # sizes -----------------------------------------------------------------
+    @property
+    def nd(self):
+        return self.nvar + self.npar
+
+    @property
+    def nspace(self):
+        return self.B ** self.dim
+
+    @property
+    def nface(self):
+        return self.B ** (self.dim - 1)
+
+    # kernels ---------------------------------------------------------------
+    def flux(self, Q):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        F = np.zeros(self.dim * self.nvar)
+        lib().orc_flux(self.ps, _p(Q), _p(F))
+        return F.reshape(self.dim, self.nvar)
+
+    def stp(self, luh, dx, dt):
+        B, D, V, ND = self.B, self.dim, self.nvar, self.nd
+        st = self.nspace * B
+        luh = np.ascontiguousarray(luh, dtype=np.float64)
+        lQi = np.zeros(st * ND)
+        rhs = np.zeros(st * ND)
+        lFi = np.zeros(st * (D + 1) * V)
+        lQhi = np.zeros(self.nspace * ND)
+        lFhi = np.zeros(self.nspace * (D + 1) * V)
+        lQhbnd = np.zeros(2 * D * self.nface * ND)
+        lFhbnd = np.zeros(2 * D * self.nface * V)
+        invdx = np.array([1.0 / d for d in dx], dtype=np.float64)
+        it = lib().orc_stp_nonlinear(self.ps, _p(luh), _p(invdx), dt, _p(lQi), _p(rhs), _p(lFi),
+                                     _p(lQhi), _p(lFhi), _p(lQhbnd), _p(lFhbnd))
+        return dict(iters=it, lQi=lQi, lFi=lFi, lQhi=lQhi, lFhi=lFhi, lQhbnd=lQhbnd, lFhbnd=lFhbnd)
+
+    def volume_integral(self, lFhi, dx):
+        lFhi = np.ascontiguousarray(lFhi, dtype=np.float64)
+        lduh = np.zeros(self.nspace * self.nvar)
+        lib().orc_volume_integral_nonlinear(self.ps, _p(lduh), _p(lFhi), _p(np.array(dx, dtype=np.float64)))
+        return lduh
+
+    def fused(self, luh, dx, dt):
+        D, V, ND = self.dim, self.nvar, self.nd
+        luh = np.ascontiguousarray(luh, dtype=np.float64)
+        lduh = np.zeros(self.nspace * V)
+        lQhbnd = np.zeros(2 * D * self.nface * ND)
+        lFhbnd = np.zeros(2 * D * self.nface * V)
+        it = lib().orc_fused_stp_vi(self.ps, _p(lduh), _p(lQhbnd), _p(lFhbnd), _p(luh),
+                                    _p(np.array(dx, dtype=np.float64)), dt)
+        return dict(iters=it, lduh=lduh, lQhbnd=lQhbnd, lFhbnd=lFhbnd)
+
+    def riemann(self, FL, FR, QL, QR, direction):
+        FL = np.array(FL, dtype=np.float64)
+        FR = np.array(FR, dtype=np.float64)
+        QL = np.ascontiguousarray(QL, dtype=np.float64)
+        QR = np.ascontiguousarray(QR, dtype=np.float64)
+        lib().orc_riemann_nonlinear(self.ps, _p(FL), _p(FR), _p(QL), _p(QR), direction)
+        return FL, FR
+
+    def face_integral(self, lduh, lFhbnd_face, direction, orientation, dx):
+        lduh = np.array(lduh, dtype=np.float64)
+        lib().orc_face_integral_nonlinear(self.ps, _p(lduh), _p(np.ascontiguousarray(lFhbnd_face, dtype=np.float64)),
+                                          direction, orientation, _p(np.array(dx, dtype=np.float64)))
+        return lduh
+
+    def surface_integral(self, lduh, lFhbnd, dx):
+        lduh = np.array(lduh, dtype=np.float64)
+        lib().orc_surface_integral_nonlinear(self.ps, _p(lduh), _p(np.ascontiguousarray(lFhbnd, dtype=np.float64)),
+                                             _p(np.array(dx, dtype=np.float64)))
+        return lduh
+
+    def solution_update(self, luh, lduh, dt):
+        luh = np.array(luh, dtype=np.float64)
+        lib().orc_solution_update(self.ps, _p(luh), _p(luh), _p(np.ascontiguousarray(lduh, dtype=np.float64)), dt)
+        return luh
+
+    def stable_dt(self, luh, dx):
+        return lib().orc_stable_dt(self.ps, _p(np.ascontiguousarray(luh, dtype=np.float64)),
+                                   _p(np.array(dx, dtype=np.float64)))
+
+    def run_uniform(self, luh, n, h, steps, dt=0.0, threads=0, want_picard=False):
+        """Advance a uniform periodic mesh in place; returns (dt_next, picard)."""
+        assert luh.dtype == np.float64 and luh.flags["C_CONTIGUOUS"]
+        dt_io = np.array([dt], dtype=np.float64)
+        pic = np.zeros(luh.size // (self.nspace * self.nd), dtype=np.int32) if want_picard else None
+        rc = lib().orc_run_uniform(self.ps, _p(luh), n, h, steps, _p(dt_io),
+                                   _ip(pic) if pic is not None else None, threads)
+        if rc != 0:
+            raise MemoryError("oracle run_uniform failed to allocate")
+        return float(dt_io[0]), pic
