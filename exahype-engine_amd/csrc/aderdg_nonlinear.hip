@@ -1,0 +1,708 @@
+// HIP kernels of the nonlinear ADER-DG cell update for CDNA4 (gfx950).
+//
+// Reference path (ExaHyPE/kernels/aderdg/generic/c/{2d,3d}):
+//   predictor_kernel  : spaceTimePredictorNonlinear.cpph (Picard loop :39-433,
+//                       time averaging :441-515, extrapolation :517-687) fused with
+//                       volumeIntegralNonlinear.cpph (time-averaged branch :133-223)
+//   riemann_kernel    : riemannSolverNonlinear.cpph:37-128 + RiemannSolverUtils.h:152-177
+//   corrector_kernel  : faceIntegralNonlinear.cpph:21-103 (faces 0..2D-1 in order,
+//                       ADERDGSolver.cpp:1207-1221) + solutionUpdate.cpph:28-67 +
+//                       stableTimeStepSize.cpph:14-62 with the global min reduction
+//
+// Work decomposition (MI355X-first, not the reference's loop nests):
+//   * predictor: one workgroup holds CPB whole cells; one thread owns one
+//     spatial node s of one cell and keeps that node's whole space-time
+//     column lQi[s][0..B)[0..nData) plus its Picard accumulators in VGPRs.
+//     Per time slice the node's flux is evaluated in registers and published
+//     to a double-buffered LDS slab; each thread then reads the B-point lines
+//     through its node along x/y/z (stiffness contraction), so the only LDS
+//     traffic is the flux slab.  lQi/rhs/lFi never touch HBM: per cell-step
+//     the kernel reads luh once and writes lduh, lQhbnd, lFhbnd once.
+//   * Riemann: one wavefront per face, lanes over face nodes; the face
+//     averages are wave reductions (DPP shuffles).
+//   * corrector: thread per (cell, node); the per-cell CFL minimum is reduced
+//     in LDS and merged into the global dt with one 64-bit atomicMin per
+//     workgroup on the IEEE bit pattern (exact and order independent for
+//     positive doubles).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "engine.h"
+
+namespace exa {
+namespace {
+
+constexpr int ceil64(int n) { return (n + 63) / 64 * 64; }
+constexpr int next_pow2(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+
+template <int D, int B, int V, int NP>
+struct Shape {
+  static constexpr int ND = V + NP;
+  static constexpr int NS = ipow(B, D);      // spatial nodes per cell
+  static constexpr int NF = ipow(B, D - 1);  // nodes per face
+  static constexpr int CPB = NS >= 128 ? 1 : 256 / NS;
+  static constexpr int THREADS = ceil64(CPB * NS);
+  static constexpr int NSP = next_pow2(NS);
+};
+
+template <typename R>
+__device__ __forceinline__ R tabK(const DevTables* T, int i);
+template <>
+__device__ __forceinline__ double tabK<double>(const DevTables* T, int i) { return T->Kxi[i]; }
+template <>
+__device__ __forceinline__ float tabK<float>(const DevTables* T, int i) { return T->Kxif[i]; }
+template <typename R>
+__device__ __forceinline__ R tabI(const DevTables* T, int i);
+template <>
+__device__ __forceinline__ double tabI<double>(const DevTables* T, int i) { return T->iK1[i]; }
+template <>
+__device__ __forceinline__ float tabI<float>(const DevTables* T, int i) { return T->iK1f[i]; }
+
+// Coordinates (z, y, x) of spatial node s; z = 0 in 2D.
+template <int D, int B>
+__device__ __forceinline__ void node_zyx(int s, int& z, int& y, int& x) {
+  x = s % B;
+  y = (s / B) % B;
+  z = (D == 3) ? s / (B * B) : 0;
+}
+
+// Product of the quadrature weights transverse to direction d, in the order
+// the reference multiplies them (x: w_z*w_y, y: w_z*w_x, z: w_y*w_x).
+template <int D>
+__device__ __forceinline__ double transverse_weight(const double* w, int d, int z, int y, int x) {
+  if (D == 3) {
+    if (d == 0) return w[z] * w[y];
+    if (d == 1) return w[z] * w[x];
+    return w[y] * w[x];
+  }
+  return d == 0 ? w[y] : w[x];
+}
+
+// Node index along direction d of node s, and the index of s's projection
+// on the face orthogonal to d (x-faces (z,y), y-faces (z,x), z-faces (y,x)).
+template <int D, int B>
+__device__ __forceinline__ int coord(int d, int z, int y, int x) {
+  return d == 0 ? x : (d == 1 ? y : z);
+}
+template <int D, int B>
+__device__ __forceinline__ int face_node(int d, int z, int y, int x) {
+  if (D == 3) return d == 0 ? z * B + y : (d == 1 ? z * B + x : y * B + x);
+  return d == 0 ? y : x;
+}
+template <int D, int B>
+__device__ __forceinline__ int stride_of(int d) {
+  return d == 0 ? 1 : (d == 1 ? B : B * B);
+}
+
+// ---------------------------------------------------------------------------
+// Space-time predictor + volume integral
+// ---------------------------------------------------------------------------
+template <int D, int B, int V, int NP, typename R>
+__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_kernel(
+    long ncells, const double* __restrict__ luh, const double* __restrict__ hcell,
+    const double* __restrict__ dtp, double* __restrict__ lduh, double* __restrict__ lQhbnd,
+    double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, PdeParams P,
+    int maxIter, int adaptive) {
+  using S = Shape<D, B, V, NP>;
+  constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, NSP = S::NSP;
+  constexpr int SLAB = NS * D * V;                       // one flux slab, per cell
+  constexpr int SLAB_BYTES = 2 * CPB * SLAB * int(sizeof(R));
+  constexpr int END_BYTES = CPB * NS * (ND + D * V) * 8;  // time averages, FP64
+  constexpr int BUF_BYTES = SLAB_BYTES > END_BYTES ? SLAB_BYTES : END_BYTES;
+  __shared__ __align__(16) unsigned char buf[BUF_BYTES];
+  __shared__ double red[CPB * NSP];
+
+  const int tid = threadIdx.x;
+  const int cib = tid / NS;
+  const int s = tid - cib * NS;
+  const bool inslot = cib < CPB;            // thread maps to a cell slot of this block
+  const long cell = (long)blockIdx.x * CPB + cib;
+  const bool valid = inslot && (cell < ncells);
+  const int cslot = inslot ? cib : 0;       // only dereferenced by in-slot threads
+
+  int z, y, x;
+  node_zyx<D, B>(s, z, y, x);
+  const double* w = T->w;
+  const double dt = *dtp;
+  const double h = valid ? hcell[cell] : 1.0;
+  const double invDx = 1.0 / h;
+
+  double u[ND];
+  R q[B][ND];
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < ND; v++) u[v] = luh[(cell * NS + s) * ND + v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < ND; v++) u[v] = (v == 0) ? 1.0 : 0.0;
+  }
+#pragma unroll
+  for (int t = 0; t < B; t++)
+#pragma unroll
+    for (int v = 0; v < ND; v++) q[t][v] = static_cast<R>(u[v]);
+
+  // node weight (reference: w[z]*w[y]*w[x], 2D w[y]*w[x]) and the two
+  // transverse weights per direction in the reference's multiplication order
+  const double W = (D == 3) ? w[z] * w[y] * w[x] : w[y] * w[x];
+  const double iweight = 1.0 / W;
+  double ta[D], tb[D];
+  int cc[D], base[D];
+  R Kc[D][B];  // Kxi[n][c_d]: the stiffness column this node needs per direction
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    if (D == 3) {
+      ta[d] = d == 2 ? w[y] : w[z];
+      tb[d] = d == 0 ? w[y] : w[x];
+    } else {
+      ta[d] = d == 0 ? w[y] : w[x];
+      tb[d] = 1.0;
+    }
+    cc[d] = coord<D, B>(d, z, y, x);
+    base[d] = s - cc[d] * stride_of<D, B>(d);
+#pragma unroll
+    for (int n = 0; n < B; n++) Kc[d][n] = tabK<R>(T, n * B + cc[d]);
+  }
+
+  R fh[D][V];  // time-averaged flux of the last executed iteration
+#pragma unroll
+  for (int d = 0; d < D; d++)
+#pragma unroll
+    for (int v = 0; v < V; v++) fh[d][v] = R(0);
+  bool active = valid;
+  int iters = maxIter;
+  for (int iter = 0; iter < maxIter; iter++) {
+    R acc[B][V];
+#pragma unroll
+    for (int t = 0; t < B; t++)
+#pragma unroll
+      for (int v = 0; v < V; v++) acc[t][v] = R(0);
+    if (active) {
+#pragma unroll
+      for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int v = 0; v < V; v++) fh[d][v] = R(0);
+    }
+
+#pragma unroll
+    for (int t = 0; t < B; t++) {
+      R* slab = reinterpret_cast<R*>(buf) + ((t & 1) * CPB + cslot) * SLAB;
+      if (active) {
+        R F[D][V];
+        pde_flux<D, R>(P, q[t], F);
+        const R wt = static_cast<R>(w[t]);
+#pragma unroll
+        for (int d = 0; d < D; d++)
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            fh[d][v] += F[d][v] * wt;
+            slab[(s * D + d) * V + v] = F[d][v];
+          }
+      }
+      __syncthreads();
+      if (active) {
+        // rhs0 = weight * F0[t] * luh, then minus the x, y, z stiffness terms
+        const R rw = static_cast<R>(W * T->F0[t]);
+        R r[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) r[v] = rw * static_cast<R>(u[v]);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          const R upd = static_cast<R>(w[t] * ta[d] * tb[d] * dt * invDx);
+#pragma unroll
+          for (int v = 0; v < V; v++)
+#pragma unroll
+            for (int n = 0; n < B; n++) {
+              const R f = slab[((base[d] + n * stride_of<D, B>(d)) * D + d) * V + v];
+              r[v] -= upd * f * Kc[d][n];
+            }
+        }
+        // (K1)^-1 in time: lQi_new[t'] += iweight * rhs[t] * iK1[t'][t]
+        const R iw = static_cast<R>(iweight);
+#pragma unroll
+        for (int tp = 0; tp < B; tp++)
+#pragma unroll
+          for (int v = 0; v < V; v++) acc[tp][v] += iw * r[v] * tabI<R>(T, tp * B + t);
+      }
+    }
+    // residual and update of lQi (variables only; parameters are carried)
+    double sq = 0.0;
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < B; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          const double dq = static_cast<double>(acc[t][v]) - static_cast<double>(q[t][v]);
+          sq += dq * dq;
+          q[t][v] = acc[t][v];
+        }
+    }
+    if (adaptive) {
+      // deterministic per-cell tree reduction of the squared update
+      if (inslot) {
+        red[cib * NSP + s] = sq;
+        if (s < NSP - NS) red[cib * NSP + NS + s] = 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int stride = NSP / 2; stride > 0; stride >>= 1) {
+        if (inslot && s < stride) red[cib * NSP + s] += red[cib * NSP + s + stride];
+        __syncthreads();
+      }
+      const double res = inslot ? red[cib * NSP] : 0.0;
+      const double tol = 1e-7;
+      if (active && res < tol * tol) {
+        active = false;
+        iters = (D == 3) ? iter : maxIter + 2;  // reference return conventions
+      }
+      if (!__syncthreads_or(active)) break;
+    } else {
+      __syncthreads();
+    }
+  }
+
+  // ---- time averages -> LDS (FP64) ----
+  double* Qh = reinterpret_cast<double*>(buf) + cslot * NS * (ND + D * V);
+  double* Fh = Qh + NS * ND;  // [d][node][v]
+  __syncthreads();  // slabs no longer read
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < ND; v++) {
+      double a = 0.0;
+#pragma unroll
+      for (int t = 0; t < B; t++) a += static_cast<double>(q[t][v]) * w[t];
+      Qh[s * ND + v] = a;
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int v = 0; v < V; v++) Fh[(d * NS + s) * V + v] = static_cast<double>(fh[d][v]);
+  }
+  __syncthreads();
+  if (!valid) return;
+
+  // ---- volume integral (lduh zeroed, then x, y, z contributions) ----
+  {
+    double du[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) du[v] = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const double upd = (D == 3 ? ta[d] * tb[d] : ta[d]) * invDx;
+      const int c = cc[d];
+#pragma unroll
+      for (int v = 0; v < V; v++)
+#pragma unroll
+        for (int m = 0; m < B; m++)
+          du[v] += T->Kxi[c * B + m] * Fh[(d * NS + base[d] + m * stride_of<D, B>(d)) * V + v] * upd;
+    }
+#pragma unroll
+    for (int v = 0; v < V; v++) lduh[(cell * NS + s) * V + v] = du[v];
+  }
+
+  // ---- extrapolation to the 2D faces (faces zeroed, then FCoeff sums) ----
+  {
+    constexpr int NQ = 2 * D * NF * ND;
+    for (int o = s; o < NQ; o += NS) {
+      const int v = o % ND;
+      const int fn = (o / ND) % NF;
+      const int f = o / (ND * NF);
+      const int d = f >> 1;
+      const double* FC = (f & 1) ? T->F1 : T->F0;
+      // node of the face line at parameter l
+      int a = fn / B, b = fn % B;
+      if (D == 2) { a = 0; b = fn; }
+      double acc = 0.0;
+      const bool skip = (D == 2) && (d == 0) && (v >= V);  // 2D x-faces carry no parameters
+      if (!skip) {
+#pragma unroll
+        for (int l = 0; l < B; l++) {
+          int nz, ny, nx;
+          if (D == 3) {
+            if (d == 0) { nz = a; ny = b; nx = l; }
+            else if (d == 1) { nz = a; ny = l; nx = b; }
+            else { nz = l; ny = a; nx = b; }
+          } else {
+            nz = 0;
+            if (d == 0) { ny = b; nx = l; }
+            else { ny = l; nx = b; }
+          }
+          const int node = (D == 3) ? (nz * B + ny) * B + nx : ny * B + nx;
+          acc += Qh[node * ND + v] * FC[l];
+        }
+      }
+      lQhbnd[cell * (2 * D * NF * ND) + o] = acc;
+    }
+    constexpr int NFL = 2 * D * NF * V;
+    for (int o = s; o < NFL; o += NS) {
+      const int v = o % V;
+      const int fn = (o / V) % NF;
+      const int f = o / (V * NF);
+      const int d = f >> 1;
+      const double* FC = (f & 1) ? T->F1 : T->F0;
+      int a = fn / B, b = fn % B;
+      if (D == 2) { a = 0; b = fn; }
+      double acc = 0.0;
+#pragma unroll
+      for (int l = 0; l < B; l++) {
+        int nz, ny, nx;
+        if (D == 3) {
+          if (d == 0) { nz = a; ny = b; nx = l; }
+          else if (d == 1) { nz = a; ny = l; nx = b; }
+          else { nz = l; ny = a; nx = b; }
+        } else {
+          nz = 0;
+          if (d == 0) { ny = b; nx = l; }
+          else { ny = l; nx = b; }
+        }
+        const int node = (D == 3) ? (nz * B + ny) * B + nx : ny * B + nx;
+        acc += Fh[(d * NS + node) * V + v] * FC[l];
+      }
+      lFhbnd[cell * (2 * D * NF * V) + o] = acc;
+    }
+  }
+  if (s == 0 && picard) picard[cell] = iters;
+}
+
+// ---------------------------------------------------------------------------
+// Rusanov flux, one wavefront per face
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+template <int D, int B, int V, int NP>
+__global__ __launch_bounds__(256) void riemann_kernel(long nfaces, const int4* __restrict__ faces,
+                                                      const double* __restrict__ lQhbnd,
+                                                      double* __restrict__ lFhbnd,
+                                                      const DevTables* __restrict__ T, PdeParams P) {
+  using S = Shape<D, B, V, NP>;
+  constexpr int ND = S::ND, NF = S::NF;
+  const int lane = threadIdx.x & 63;
+  const long f = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (f >= nfaces) return;  // uniform per wavefront; no block barriers below
+  const int4 fc = faces[f];
+  const int d = fc.z;
+  const double* QL = lQhbnd + ((long)fc.x * 2 * D + 2 * d + 1) * NF * ND;
+  const double* QR = lQhbnd + ((long)fc.y * 2 * D + 2 * d) * NF * ND;
+  double* FL = lFhbnd + ((long)fc.x * 2 * D + 2 * d + 1) * NF * V;
+  double* FR = lFhbnd + ((long)fc.y * 2 * D + 2 * d) * NF * V;
+  const double* w = T->w;
+  double aL[ND], aR[ND];
+#pragma unroll
+  for (int k = 0; k < ND; k++) aL[k] = aR[k] = 0.0;
+  for (int j = lane; j < NF; j += 64) {
+    const double wj = (D == 3) ? w[j / B] * w[j % B] : w[j];
+#pragma unroll
+    for (int k = 0; k < ND; k++) {
+      aL[k] += wj * QL[j * ND + k];
+      aR[k] += wj * QR[j * ND + k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ND; k++) {
+    aL[k] = wave_sum(aL[k]);
+    aR[k] = wave_sum(aR[k]);
+  }
+  const double smax = fmax(pde_max_abs_eig<D>(P, aL, d), pde_max_abs_eig<D>(P, aR, d));
+  for (int j = lane; j < NF; j += 64) {
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+      const double fl = FL[j * V + k], fr = FR[j * V + k];
+      const double ql = QL[j * ND + k], qr = QR[j * ND + k];
+      const double flux = 0.5 * (fr + fl) - 0.5 * smax * (qr - ql);
+      FL[j * V + k] = flux;
+      FR[j * V + k] = flux;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Face integrals + solution update + CFL
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void atomic_min_pos_double(double* addr, double v) {
+  // positive IEEE doubles order like their bit patterns
+  atomicMin(reinterpret_cast<unsigned long long*>(addr), static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+template <int D, int B, int V, int NP>
+__device__ __forceinline__ double node_cfl_dt(const double* __restrict__ Q, const PdeParams& P, double invDx,
+                                              double cflpnpm) {
+  double den = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; d++) den += pde_max_abs_eig<D>(P, Q, d) * invDx;
+  return cflpnpm / den;
+}
+
+template <int THREADS>
+__device__ __forceinline__ double block_min(double v, double* sred) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sred[wid] = v;
+  __syncthreads();
+  double r = sred[0];
+#pragma unroll
+  for (int i = 1; i < THREADS / 64; i++) r = fmin(r, sred[i]);
+  return r;
+}
+
+template <int D, int B, int V, int NP>
+__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_kernel(
+    long ncells, double* __restrict__ luh, const double* __restrict__ lduh, const double* __restrict__ lFhbnd,
+    const double* __restrict__ hcell, const double* __restrict__ dtp, double* __restrict__ dt_next,
+    const DevTables* __restrict__ T, PdeParams P, double cflpnpm, int do_update) {
+  using S = Shape<D, B, V, NP>;
+  constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, THREADS = S::THREADS;
+  __shared__ double sred[THREADS / 64];
+  const int tid = threadIdx.x;
+  const int cib = tid / NS;
+  const int s = tid - cib * NS;
+  const long cell = (long)blockIdx.x * CPB + cib;
+  const bool valid = (cib < CPB) && (cell < ncells);
+  double mydt = DBL_MAX;
+  if (valid) {
+    int z, y, x;
+    node_zyx<D, B>(s, z, y, x);
+    const double* w = T->w;
+    const double h = hcell[cell];
+    double Q[ND];
+#pragma unroll
+    for (int v = 0; v < ND; v++) Q[v] = luh[(cell * NS + s) * ND + v];
+    if (do_update) {
+      const double dt = *dtp;
+      double du[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) du[v] = lduh[(cell * NS + s) * V + v];
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        const int c = coord<D, B>(d, z, y, x);
+        const int fn = face_node<D, B>(d, z, y, x);
+        const double tw = transverse_weight<D>(w, d, z, y, x);
+#pragma unroll
+        for (int o = 0; o < 2; o++) {
+          const double scaling = 1.0 / h * (2.0 * o - 1.0);
+          const double sw = tw * scaling;
+          const double fcoef = o ? T->F1[c] : T->F0[c];
+          const double* F = lFhbnd + ((cell * 2 * D + 2 * d + o) * NF + fn) * V;
+#pragma unroll
+          for (int v = 0; v < V; v++) du[v] -= sw * fcoef * F[v];
+        }
+      }
+      const double Wn = (D == 3) ? w[z] * w[y] * w[x] : w[y] * w[x];
+      const double upd = dt / Wn;
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        Q[v] = Q[v] + du[v] * upd;
+        luh[(cell * NS + s) * ND + v] = Q[v];
+      }
+    }
+    mydt = node_cfl_dt<D, B, V, NP>(Q, P, 1.0 / h, cflpnpm);
+  }
+  const double bmin = block_min<THREADS>(mydt, sred);
+  if (tid == 0 && bmin < DBL_MAX) atomic_min_pos_double(dt_next, bmin);
+}
+
+// Single-cell helpers used by the per-cell compatibility entry points.
+template <int D, int B, int V, int NP>
+__global__ void face_integral_kernel(double* __restrict__ lduh, const double* __restrict__ F, int d, int o,
+                                     double invdx, const DevTables* __restrict__ T) {
+  using S = Shape<D, B, V, NP>;
+  constexpr int NS = S::NS;
+  const int s = threadIdx.x;
+  if (s >= NS) return;
+  int z, y, x;
+  node_zyx<D, B>(s, z, y, x);
+  const int c = coord<D, B>(d, z, y, x);
+  const int fn = face_node<D, B>(d, z, y, x);
+  const double scaling = invdx * (2.0 * o - 1.0);
+  const double sw = transverse_weight<D>(T->w, d, z, y, x) * scaling;
+  const double fcoef = o ? T->F1[c] : T->F0[c];
+#pragma unroll
+  for (int v = 0; v < V; v++) lduh[s * V + v] -= sw * fcoef * F[fn * V + v];
+}
+
+template <int D, int B, int V, int NP>
+__global__ void solution_update_kernel(double* __restrict__ luh, const double* __restrict__ luhOld,
+                                       const double* __restrict__ lduh, const double* __restrict__ dtp,
+                                       const DevTables* __restrict__ T) {
+  using S = Shape<D, B, V, NP>;
+  constexpr int NS = S::NS, ND = S::ND;
+  const int s = threadIdx.x;
+  if (s >= NS) return;
+  int z, y, x;
+  node_zyx<D, B>(s, z, y, x);
+  const double* w = T->w;
+  const double Wn = (D == 3) ? w[z] * w[y] * w[x] : w[y] * w[x];
+  const double upd = *dtp / Wn;
+#pragma unroll
+  for (int v = 0; v < V; v++) luh[s * ND + v] = luhOld[s * ND + v] + lduh[s * V + v] * upd;
+}
+
+__global__ void advance_time_kernel(double* state) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    state[0] += state[1];
+    state[1] = state[2];
+    state[2] = DBL_MAX;
+  }
+}
+
+__global__ void fill_kernel(double* p, long n, double value) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch over the compiled (D, B, V, NP) shapes
+// ---------------------------------------------------------------------------
+template <int D, int B, int V, int NP>
+struct Launch {
+  using S = Shape<D, B, V, NP>;
+  static int predictor(const KernelCfg& k, hipStream_t st, long n, const double* luh, const double* h,
+                       const double* dt, double* lduh, double* qb, double* fb, int* pic) {
+    const long blocks = (n + S::CPB - 1) / S::CPB;
+    const int adaptive = k.picard_max <= 0;
+    const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
+    if (k.precision == 0) {
+      hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, double>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh,
+                         h, dt, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+    } else {
+      hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, float>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh,
+                         h, dt, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+    }
+    return static_cast<int>(hipGetLastError());
+  }
+  static int riemann(const KernelCfg& k, hipStream_t st, long nf, const int* faces, const double* qb,
+                     double* fb) {
+    const long blocks = (nf + 3) / 4;
+    hipLaunchKernelGGL((riemann_kernel<D, B, V, NP>), dim3(blocks), dim3(256), 0, st, nf,
+                       reinterpret_cast<const int4*>(faces), qb, fb, k.tab, k.pde);
+    return static_cast<int>(hipGetLastError());
+  }
+  static int corrector(const KernelCfg& k, hipStream_t st, long n, double* luh, double* lduh, const double* fb,
+                       const double* h, const double* dt, double* dt_next, int do_update) {
+    const long blocks = (n + S::CPB - 1) / S::CPB;
+    hipLaunchKernelGGL((corrector_kernel<D, B, V, NP>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, lduh, fb,
+                       h, dt, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update);
+    return static_cast<int>(hipGetLastError());
+  }
+  static int face_integral(const KernelCfg& k, hipStream_t st, double* lduh, const double* F, int d, int o,
+                           double invdx) {
+    hipLaunchKernelGGL((face_integral_kernel<D, B, V, NP>), dim3(1), dim3(ceil64(S::NS)), 0, st, lduh, F, d, o,
+                       invdx, k.tab);
+    return static_cast<int>(hipGetLastError());
+  }
+  static int solution_update(const KernelCfg& k, hipStream_t st, double* luh, const double* luhOld,
+                             const double* lduh, const double* dt) {
+    hipLaunchKernelGGL((solution_update_kernel<D, B, V, NP>), dim3(1), dim3(ceil64(S::NS)), 0, st, luh, luhOld,
+                       lduh, dt, k.tab);
+    return static_cast<int>(hipGetLastError());
+  }
+};
+
+struct Entry {
+  int D, B, V, NP;
+  int (*predictor)(const KernelCfg&, hipStream_t, long, const double*, const double*, const double*, double*,
+                   double*, double*, int*);
+  int (*riemann)(const KernelCfg&, hipStream_t, long, const int*, const double*, double*);
+  int (*corrector)(const KernelCfg&, hipStream_t, long, double*, double*, const double*, const double*,
+                   const double*, double*, int);
+  int (*face_integral)(const KernelCfg&, hipStream_t, double*, const double*, int, int, double);
+  int (*solution_update)(const KernelCfg&, hipStream_t, double*, const double*, const double*, const double*);
+};
+
+#define EXA_ENTRY(D, B, V, NP)                                                                      \
+  Entry {                                                                                           \
+    D, B, V, NP, &Launch<D, B, V, NP>::predictor, &Launch<D, B, V, NP>::riemann,                    \
+        &Launch<D, B, V, NP>::corrector, &Launch<D, B, V, NP>::face_integral,                       \
+        &Launch<D, B, V, NP>::solution_update                                                       \
+  }
+
+// Compiled shapes: Euler (5 variables) for orders 1..6 in 2D and 1..5 in 3D,
+// plus the reference unit-test shape (Order 3, one parameter).
+const Entry kEntries[] = {
+    EXA_ENTRY(2, 2, 5, 0), EXA_ENTRY(2, 3, 5, 0), EXA_ENTRY(2, 4, 5, 0), EXA_ENTRY(2, 5, 5, 0),
+    EXA_ENTRY(2, 6, 5, 0), EXA_ENTRY(2, 7, 5, 0), EXA_ENTRY(3, 2, 5, 0), EXA_ENTRY(3, 3, 5, 0),
+    EXA_ENTRY(3, 4, 5, 0), EXA_ENTRY(3, 5, 5, 0), EXA_ENTRY(3, 6, 5, 0), EXA_ENTRY(2, 4, 5, 1),
+    EXA_ENTRY(3, 4, 5, 1),
+};
+
+const Entry* find_entry(const KernelCfg& k) {
+  for (const Entry& e : kEntries)
+    if (e.D == k.dim && e.B == k.order + 1 && e.V == k.nvar && e.NP == k.npar) return &e;
+  return nullptr;
+}
+
+}  // namespace
+
+bool shape_supported(int dim, int order, int nvar, int npar) {
+  for (const Entry& e : kEntries)
+    if (e.D == dim && e.B == order + 1 && e.V == nvar && e.NP == npar) return true;
+  return false;
+}
+
+int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* luh, const double* h,
+                     const double* dt, double* lduh, double* qb, double* fb, int* pic) {
+  const Entry* e = find_entry(k);
+  if (!e) return -1;
+  if (n <= 0) return 0;
+  return e->predictor(k, s, n, luh, h, dt, lduh, qb, fb, pic);
+}
+
+int launch_riemann(const KernelCfg& k, hipStream_t s, long nf, const int* faces, const double* qb, double* fb) {
+  const Entry* e = find_entry(k);
+  if (!e) return -1;
+  if (nf <= 0) return 0;
+  return e->riemann(k, s, nf, faces, qb, fb);
+}
+
+int launch_corrector(const KernelCfg& k, hipStream_t s, long n, double* luh, double* lduh, const double* fb,
+                     const double* h, const double* dt, double* dt_next) {
+  const Entry* e = find_entry(k);
+  if (!e) return -1;
+  if (n <= 0) return 0;
+  return e->corrector(k, s, n, luh, lduh, fb, h, dt, dt_next, 1);
+}
+
+int launch_stable_dt(const KernelCfg& k, hipStream_t s, long n, const double* luh, const double* h,
+                     double* dt_out) {
+  const Entry* e = find_entry(k);
+  if (!e) return -1;
+  if (n <= 0) return 0;
+  return e->corrector(k, s, n, const_cast<double*>(luh), nullptr, nullptr, h, nullptr, dt_out, 0);
+}
+
+int launch_face_integral(const KernelCfg& k, hipStream_t s, double* lduh, const double* F, int d, int o,
+                         double invdx) {
+  const Entry* e = find_entry(k);
+  if (!e) return -1;
+  return e->face_integral(k, s, lduh, F, d, o, invdx);
+}
+
+int launch_solution_update(const KernelCfg& k, hipStream_t s, double* luh, const double* luhOld,
+                           const double* lduh, const double* dt) {
+  const Entry* e = find_entry(k);
+  if (!e) return -1;
+  return e->solution_update(k, s, luh, luhOld, lduh, dt);
+}
+
+int launch_advance_time(hipStream_t s, double* state) {
+  hipLaunchKernelGGL(advance_time_kernel, dim3(1), dim3(64), 0, s, state);
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_fill(hipStream_t s, double* p, long n, double value) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(fill_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, value);
+  return static_cast<int>(hipGetLastError());
+}
+
+}  // namespace exa

@@ -1,0 +1,486 @@
+// C ABI of the engine: contexts, per-cell compatibility entry points,
+// batched device entry points.  See include/exahype_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/exahype_amd.h"
+#include "engine.h"
+
+namespace exa {
+
+bool shape_supported(int dim, int order, int nvar, int npar);
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+}  // namespace exa
+
+using exa::set_error;
+
+struct exa_ctx {
+  exa::KernelCfg k{};
+  exa::Tables tab{};
+  exa::DevTables* d_tab = nullptr;
+  hipStream_t stream = nullptr;
+  // scratch for the per-cell compatibility entry points
+  double* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+namespace {
+
+int hip_fail(hipError_t e, const char* what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return EXA_ERR_HIP;
+}
+
+#define HIP_OK(expr, what)                   \
+  do {                                       \
+    hipError_t _e = (expr);                  \
+    if (_e != hipSuccess) return hip_fail(_e, what); \
+  } while (0)
+
+int launch_status(int rc, const char* what) {
+  if (rc == -1) {
+    set_error(std::string(what) + ": shape not compiled into this library");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  if (rc != 0) return hip_fail(static_cast<hipError_t>(rc), what);
+  return EXA_OK;
+}
+
+int ensure_scratch(exa_ctx* c, size_t bytes) {
+  if (c->scratch_bytes >= bytes) return EXA_OK;
+  if (c->d_scratch) (void)hipFree(c->d_scratch);
+  c->d_scratch = nullptr;
+  c->scratch_bytes = 0;
+  HIP_OK(hipMalloc(&c->d_scratch, bytes), "hipMalloc(scratch)");
+  c->scratch_bytes = bytes;
+  return EXA_OK;
+}
+
+int nspace(const exa_ctx* c) {
+  const int B = c->k.order + 1;
+  return c->k.dim == 3 ? B * B * B : B * B;
+}
+int nface(const exa_ctx* c) {
+  const int B = c->k.order + 1;
+  return c->k.dim == 3 ? B * B : B;
+}
+int ndata(const exa_ctx* c) { return c->k.nvar + c->k.npar; }
+
+}  // namespace
+
+extern "C" {
+
+const char* exa_last_error(void) { return exa::g_error.c_str(); }
+
+int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pde, int precision,
+                   int picard_max, double cfl) {
+  if (!out) {
+    set_error("exa_ctx_create: out is NULL");
+    return EXA_ERR_ARG;
+  }
+  *out = nullptr;
+  if (dim != 2 && dim != 3) {
+    set_error("exa_ctx_create: dim must be 2 or 3");
+    return EXA_ERR_ARG;
+  }
+  if (pde != EXA_PDE_EULER && pde != EXA_PDE_ADVECTION) {
+    set_error("exa_ctx_create: unknown pde");
+    return EXA_ERR_ARG;
+  }
+  if (precision < EXA_PREC_FP64 || precision > EXA_PREC_BF16) {
+    set_error("exa_ctx_create: unknown precision");
+    return EXA_ERR_ARG;
+  }
+  if (precision == EXA_PREC_BF16) {
+    set_error("exa_ctx_create: BF16 predictor not compiled into this library");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  if (!(cfl > 0)) {
+    set_error("exa_ctx_create: cfl must be positive");
+    return EXA_ERR_ARG;
+  }
+  if (!exa::shape_supported(dim, order, nvar, npar)) {
+    set_error("exa_ctx_create: (dim, order, nvar, npar) not compiled into this library");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  exa_ctx* c = new exa_ctx;
+  if (!exa::build_tables(order, &c->tab)) {
+    delete c;
+    set_error("exa_ctx_create: cannot build tables for this order");
+    return EXA_ERR_ARG;
+  }
+  c->k.dim = dim;
+  c->k.order = order;
+  c->k.nvar = nvar;
+  c->k.npar = npar;
+  c->k.precision = precision;
+  c->k.picard_max = picard_max;
+  c->k.cfl = cfl;
+  c->k.pnpm = exa::pnpm_for_order(order);
+  c->k.pde.id = pde;
+  c->k.pde.a[0] = 1.0;
+  c->k.pde.a[1] = 0.5;
+  c->k.pde.a[2] = 0.25;
+  exa::DevTables h{};
+  const int B = order + 1;
+  for (int i = 0; i < B; i++) {
+    h.w[i] = c->tab.w[i];
+    h.x[i] = c->tab.x[i];
+    h.F0[i] = c->tab.F0[i];
+    h.F1[i] = c->tab.F1[i];
+    h.wf[i] = static_cast<float>(c->tab.w[i]);
+    h.F0f[i] = static_cast<float>(c->tab.F0[i]);
+  }
+  for (int i = 0; i < B * B; i++) {
+    h.Kxi[i] = c->tab.Kxi[i];
+    h.iK1[i] = c->tab.iK1[i];
+    h.dudx[i] = c->tab.dudx[i];
+    h.Kxif[i] = static_cast<float>(c->tab.Kxi[i]);
+    h.iK1f[i] = static_cast<float>(c->tab.iK1[i]);
+  }
+  hipError_t e = hipMalloc(&c->d_tab, sizeof(exa::DevTables));
+  if (e == hipSuccess) e = hipMemcpy(c->d_tab, &h, sizeof(h), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    if (c->d_tab) (void)hipFree(c->d_tab);
+    delete c;
+    return hip_fail(e, "exa_ctx_create");
+  }
+  c->k.tab = c->d_tab;
+  *out = c;
+  return EXA_OK;
+}
+
+int exa_ctx_destroy(exa_ctx* c) {
+  if (!c) return EXA_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->d_scratch) (void)hipFree(c->d_scratch);
+  if (c->d_tab) (void)hipFree(c->d_tab);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return EXA_OK;
+}
+
+int exa_ctx_set_advection(exa_ctx* c, const double* a) {
+  if (!c || !a) {
+    set_error("exa_ctx_set_advection: NULL argument");
+    return EXA_ERR_ARG;
+  }
+  for (int d = 0; d < c->k.dim; d++) c->k.pde.a[d] = a[d];
+  return EXA_OK;
+}
+
+int exa_tables(int order, double* w, double* x, double* Kxi, double* iK1, double* dudx, double* F0, double* F1,
+               double* fgp, double* egp) {
+  exa::Tables t;
+  if (!exa::build_tables(order, &t)) {
+    set_error("exa_tables: unsupported order");
+    return EXA_ERR_ARG;
+  }
+  const int B = order + 1;
+  if (w) std::memcpy(w, t.w, sizeof(double) * B);
+  if (x) std::memcpy(x, t.x, sizeof(double) * B);
+  if (Kxi) std::memcpy(Kxi, t.Kxi, sizeof(double) * B * B);
+  if (iK1) std::memcpy(iK1, t.iK1, sizeof(double) * B * B);
+  if (dudx) std::memcpy(dudx, t.dudx, sizeof(double) * B * B);
+  if (F0) std::memcpy(F0, t.F0, sizeof(double) * B);
+  if (F1) std::memcpy(F1, t.F1, sizeof(double) * B);
+  if (fgp) std::memcpy(fgp, t.fgp, sizeof(double) * 3 * B * B);
+  if (egp) std::memcpy(egp, t.egp, sizeof(double) * B * B);
+  return EXA_OK;
+}
+
+int exa_ctx_tables(const exa_ctx* c, double* w, double* x, double* Kxi, double* iK1, double* dudx, double* F0,
+                   double* F1, double* fgp) {
+  if (!c) {
+    set_error("exa_ctx_tables: NULL context");
+    return EXA_ERR_ARG;
+  }
+  const int B = c->k.order + 1;
+  if (w) std::memcpy(w, c->tab.w, sizeof(double) * B);
+  if (x) std::memcpy(x, c->tab.x, sizeof(double) * B);
+  if (Kxi) std::memcpy(Kxi, c->tab.Kxi, sizeof(double) * B * B);
+  if (iK1) std::memcpy(iK1, c->tab.iK1, sizeof(double) * B * B);
+  if (dudx) std::memcpy(dudx, c->tab.dudx, sizeof(double) * B * B);
+  if (F0) std::memcpy(F0, c->tab.F0, sizeof(double) * B);
+  if (F1) std::memcpy(F1, c->tab.F1, sizeof(double) * B);
+  if (fgp) std::memcpy(fgp, c->tab.fgp, sizeof(double) * 3 * B * B);
+  return EXA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Per-cell compatibility entry points
+// ---------------------------------------------------------------------------
+
+int exa_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* c, double* lduh, double* lQhbnd, double* lGradQhbnd,
+                                              double* lFhbnd, double* luh, const double* centre,
+                                              const double* dx, double t, double dt,
+                                              int addVolumeIntegralResultToUpdate, int* picard_iterations) {
+  (void)lGradQhbnd;
+  (void)centre;
+  (void)t;
+  if (!c || !lduh || !lQhbnd || !lFhbnd || !luh || !dx) {
+    set_error("exa_fusedSpaceTimePredictorVolumeIntegral: NULL argument");
+    return EXA_ERR_ARG;
+  }
+  for (int d = 1; d < c->k.dim; d++)
+    if (dx[d] != dx[0]) {
+      set_error("exa_fusedSpaceTimePredictorVolumeIntegral: cells must be cubic (dx[d] equal)");
+      return EXA_ERR_ARG;
+    }
+  const int NS = nspace(c), NF = nface(c), ND = ndata(c), V = c->k.nvar, D = c->k.dim;
+  const size_t nLuh = size_t(NS) * ND, nLduh = size_t(NS) * V;
+  const size_t nQb = size_t(2 * D) * NF * ND, nFb = size_t(2 * D) * NF * V;
+  const size_t total = nLuh + nLduh + nQb + nFb + 2 + 2;
+  int rc = ensure_scratch(c, total * sizeof(double));
+  if (rc) return rc;
+  double* d_luh = c->d_scratch;
+  double* d_lduh = d_luh + nLuh;
+  double* d_qb = d_lduh + nLduh;
+  double* d_fb = d_qb + nQb;
+  double* d_h = d_fb + nFb;
+  double* d_dt = d_h + 2;
+  int* d_pic = reinterpret_cast<int*>(d_dt + 1);
+  double hd[3] = {dx[0], dt, 0.0};
+  HIP_OK(hipMemcpyAsync(d_luh, luh, nLuh * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy luh");
+  HIP_OK(hipMemcpyAsync(d_h, &hd[0], sizeof(double), hipMemcpyHostToDevice, c->stream), "copy dx");
+  HIP_OK(hipMemcpyAsync(d_dt, &hd[1], sizeof(double), hipMemcpyHostToDevice, c->stream), "copy dt");
+  rc = launch_status(exa::launch_predictor(c->k, c->stream, 1, d_luh, d_h, d_dt, d_lduh, d_qb, d_fb, d_pic),
+                     "predictor");
+  if (rc) return rc;
+  if (!addVolumeIntegralResultToUpdate) {
+    // luh += dt/w * volume integral (AbstractGenericADERDGSolverImplementation.template:206-210)
+    rc = launch_status(exa::launch_solution_update(c->k, c->stream, d_luh, d_luh, d_lduh, d_dt), "update");
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(luh, d_luh, nLuh * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy luh");
+  } else {
+    HIP_OK(hipMemcpyAsync(lduh, d_lduh, nLduh * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy lduh");
+  }
+  HIP_OK(hipMemcpyAsync(lQhbnd, d_qb, nQb * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy lQhbnd");
+  HIP_OK(hipMemcpyAsync(lFhbnd, d_fb, nFb * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy lFhbnd");
+  int pic = 0;
+  HIP_OK(hipMemcpyAsync(&pic, d_pic, sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy picard");
+  HIP_OK(hipStreamSynchronize(c->stream), "sync");
+  if (picard_iterations) *picard_iterations = pic;
+  return EXA_OK;
+}
+
+int exa_riemannSolver(exa_ctx* c, double* FL, double* FR, const double* QL, const double* QR, double t, double dt,
+                      const double* dx, int direction, int isBoundaryFace, int faceIndex) {
+  (void)t;
+  (void)dt;
+  (void)dx;
+  (void)isBoundaryFace;
+  (void)faceIndex;
+  if (!c || !FL || !FR || !QL || !QR) {
+    set_error("exa_riemannSolver: NULL argument");
+    return EXA_ERR_ARG;
+  }
+  if (direction < 0 || direction >= c->k.dim) {
+    set_error("exa_riemannSolver: direction out of range");
+    return EXA_ERR_ARG;
+  }
+  const int NF = nface(c), ND = ndata(c), V = c->k.nvar, D = c->k.dim;
+  // two pseudo cells: cell 0 holds the left state on face 2d+1, cell 1 the right state on face 2d
+  const size_t qcell = size_t(2 * D) * NF * ND, fcell = size_t(2 * D) * NF * V;
+  const size_t total = 2 * qcell + 2 * fcell + 2;
+  int rc = ensure_scratch(c, total * sizeof(double));
+  if (rc) return rc;
+  double* d_qb = c->d_scratch;
+  double* d_fb = d_qb + 2 * qcell;
+  int* d_face = reinterpret_cast<int*>(d_fb + 2 * fcell);
+  const int d = direction;
+  const size_t ql = (size_t(2 * d + 1)) * NF * ND, qr = qcell + size_t(2 * d) * NF * ND;
+  const size_t fl = (size_t(2 * d + 1)) * NF * V, fr = fcell + size_t(2 * d) * NF * V;
+  const int face[4] = {0, 1, d, 0};
+  HIP_OK(hipMemcpyAsync(d_qb + ql, QL, NF * ND * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy QL");
+  HIP_OK(hipMemcpyAsync(d_qb + qr, QR, NF * ND * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy QR");
+  HIP_OK(hipMemcpyAsync(d_fb + fl, FL, NF * V * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy FL");
+  HIP_OK(hipMemcpyAsync(d_fb + fr, FR, NF * V * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy FR");
+  HIP_OK(hipMemcpyAsync(d_face, face, sizeof(face), hipMemcpyHostToDevice, c->stream), "copy face");
+  rc = launch_status(exa::launch_riemann(c->k, c->stream, 1, d_face, d_qb, d_fb), "riemann");
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(FL, d_fb + fl, NF * V * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy FL");
+  HIP_OK(hipMemcpyAsync(FR, d_fb + fr, NF * V * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy FR");
+  HIP_OK(hipStreamSynchronize(c->stream), "sync");
+  return EXA_OK;
+}
+
+int exa_faceIntegral(exa_ctx* c, double* out, const double* lFhbnd, int direction, int orientation,
+                     int levelDelta, const double* dx, double dt, int addToUpdate) {
+  if (!c || !out || !lFhbnd || !dx) {
+    set_error("exa_faceIntegral: NULL argument");
+    return EXA_ERR_ARG;
+  }
+  if (direction < 0 || direction >= c->k.dim || orientation < 0 || orientation > 1) {
+    set_error("exa_faceIntegral: direction/orientation out of range");
+    return EXA_ERR_ARG;
+  }
+  if (levelDelta != 0) {
+    set_error("exa_faceIntegral: levelDelta > 0 (coarse-fine faces) is not supported by this entry point");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  const int NS = nspace(c), NF = nface(c), ND = ndata(c), V = c->k.nvar;
+  const size_t total = size_t(NS) * ND + size_t(NS) * V + size_t(NF) * V + 1;
+  int rc = ensure_scratch(c, total * sizeof(double));
+  if (rc) return rc;
+  double* d_luh = c->d_scratch;
+  double* d_lduh = d_luh + size_t(NS) * ND;
+  double* d_F = d_lduh + size_t(NS) * V;
+  double* d_dt = d_F + size_t(NF) * V;
+  HIP_OK(hipMemcpyAsync(d_F, lFhbnd, NF * V * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy F");
+  if (addToUpdate) {
+    HIP_OK(hipMemcpyAsync(d_lduh, out, NS * V * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy lduh");
+  } else {
+    HIP_OK(hipMemsetAsync(d_lduh, 0, NS * V * sizeof(double), c->stream), "zero lduh");
+    HIP_OK(hipMemcpyAsync(d_luh, out, NS * ND * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy luh");
+    HIP_OK(hipMemcpyAsync(d_dt, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream), "copy dt");
+  }
+  rc = launch_status(
+      exa::launch_face_integral(c->k, c->stream, d_lduh, d_F, direction, orientation, 1.0 / dx[direction]),
+      "face integral");
+  if (rc) return rc;
+  if (addToUpdate) {
+    HIP_OK(hipMemcpyAsync(out, d_lduh, NS * V * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy lduh");
+  } else {
+    rc = launch_status(exa::launch_solution_update(c->k, c->stream, d_luh, d_luh, d_lduh, d_dt), "update");
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(out, d_luh, NS * ND * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy luh");
+  }
+  HIP_OK(hipStreamSynchronize(c->stream), "sync");
+  return EXA_OK;
+}
+
+int exa_surfaceIntegral(exa_ctx* c, double* lduh, const double* lFhbnd, const double* dx) {
+  if (!c || !lduh || !lFhbnd || !dx) {
+    set_error("exa_surfaceIntegral: NULL argument");
+    return EXA_ERR_ARG;
+  }
+  const int NF = nface(c), V = c->k.nvar, D = c->k.dim;
+  for (int d = 0; d < D; d++)
+    for (int o = 0; o < 2; o++) {
+      const int rc = exa_faceIntegral(c, lduh, lFhbnd + size_t(2 * d + o) * NF * V, d, o, 0, dx, 0.0, 1);
+      if (rc) return rc;
+    }
+  return EXA_OK;
+}
+
+int exa_addUpdateToSolution(exa_ctx* c, double* luh, const double* luhOld, const double* lduh, double dt) {
+  if (!c || !luh || !luhOld || !lduh) {
+    set_error("exa_addUpdateToSolution: NULL argument");
+    return EXA_ERR_ARG;
+  }
+  const int NS = nspace(c), ND = ndata(c), V = c->k.nvar;
+  const size_t total = 2 * size_t(NS) * ND + size_t(NS) * V + 1;
+  int rc = ensure_scratch(c, total * sizeof(double));
+  if (rc) return rc;
+  double* d_luh = c->d_scratch;
+  double* d_old = d_luh + size_t(NS) * ND;
+  double* d_lduh = d_old + size_t(NS) * ND;
+  double* d_dt = d_lduh + size_t(NS) * V;
+  HIP_OK(hipMemcpyAsync(d_old, luhOld, NS * ND * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+  HIP_OK(hipMemcpyAsync(d_luh, luhOld, NS * ND * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+  HIP_OK(hipMemcpyAsync(d_lduh, lduh, NS * V * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+  HIP_OK(hipMemcpyAsync(d_dt, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+  rc = launch_status(exa::launch_solution_update(c->k, c->stream, d_luh, d_old, d_lduh, d_dt), "update");
+  if (rc) return rc;
+  // parameters of luh are left untouched (solutionUpdate.cpph:55-60 updates variables only)
+  std::vector<double> tmp(size_t(NS) * ND);
+  HIP_OK(hipMemcpyAsync(tmp.data(), d_luh, NS * ND * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+  HIP_OK(hipStreamSynchronize(c->stream), "sync");
+  for (int s = 0; s < NS; s++)
+    for (int v = 0; v < V; v++) luh[s * ND + v] = tmp[size_t(s) * ND + v];
+  return EXA_OK;
+}
+
+int exa_stableTimeStepSize(exa_ctx* c, const double* luh, const double* dx, double* dt_out) {
+  if (!c || !luh || !dx || !dt_out) {
+    set_error("exa_stableTimeStepSize: NULL argument");
+    return EXA_ERR_ARG;
+  }
+  for (int d = 1; d < c->k.dim; d++)
+    if (dx[d] != dx[0]) {
+      set_error("exa_stableTimeStepSize: cells must be cubic (dx[d] equal)");
+      return EXA_ERR_ARG;
+    }
+  const int NS = nspace(c), ND = ndata(c);
+  const size_t total = size_t(NS) * ND + 2;
+  int rc = ensure_scratch(c, total * sizeof(double));
+  if (rc) return rc;
+  double* d_luh = c->d_scratch;
+  double* d_h = d_luh + size_t(NS) * ND;
+  double* d_out = d_h + 1;
+  const double init[2] = {dx[0], DBL_MAX};
+  HIP_OK(hipMemcpyAsync(d_luh, luh, NS * ND * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+  HIP_OK(hipMemcpyAsync(d_h, init, 2 * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+  rc = launch_status(exa::launch_stable_dt(c->k, c->stream, 1, d_luh, d_h, d_out), "stable dt");
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(dt_out, d_out, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+  HIP_OK(hipStreamSynchronize(c->stream), "sync");
+  return EXA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched device entry points
+// ---------------------------------------------------------------------------
+
+int exa_batch_predictor(exa_ctx* c, void* stream, long ncells, const double* d_luh, const double* d_h,
+                        const double* d_dt, double* d_lduh, double* d_lQhbnd, double* d_lFhbnd, int* d_picard) {
+  if (!c || ncells < 0 || (ncells > 0 && (!d_luh || !d_h || !d_dt || !d_lduh || !d_lQhbnd || !d_lFhbnd))) {
+    set_error("exa_batch_predictor: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  return launch_status(exa::launch_predictor(c->k, static_cast<hipStream_t>(stream), ncells, d_luh, d_h, d_dt,
+                                             d_lduh, d_lQhbnd, d_lFhbnd, d_picard),
+                       "exa_batch_predictor");
+}
+
+int exa_batch_riemann(exa_ctx* c, void* stream, long nfaces, const int* d_faces, const double* d_lQhbnd,
+                      double* d_lFhbnd) {
+  if (!c || nfaces < 0 || (nfaces > 0 && (!d_faces || !d_lQhbnd || !d_lFhbnd))) {
+    set_error("exa_batch_riemann: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  return launch_status(exa::launch_riemann(c->k, static_cast<hipStream_t>(stream), nfaces, d_faces, d_lQhbnd,
+                                           d_lFhbnd),
+                       "exa_batch_riemann");
+}
+
+int exa_batch_corrector(exa_ctx* c, void* stream, long ncells, double* d_luh, double* d_lduh,
+                        const double* d_lFhbnd, const double* d_h, const double* d_dt, double* d_dt_next) {
+  if (!c || ncells < 0 || (ncells > 0 && (!d_luh || !d_lduh || !d_lFhbnd || !d_h || !d_dt || !d_dt_next))) {
+    set_error("exa_batch_corrector: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  return launch_status(exa::launch_corrector(c->k, static_cast<hipStream_t>(stream), ncells, d_luh, d_lduh,
+                                             d_lFhbnd, d_h, d_dt, d_dt_next),
+                       "exa_batch_corrector");
+}
+
+int exa_batch_stable_dt(exa_ctx* c, void* stream, long ncells, const double* d_luh, const double* d_h,
+                        double* d_dt_out) {
+  if (!c || ncells < 0 || (ncells > 0 && (!d_luh || !d_h || !d_dt_out))) {
+    set_error("exa_batch_stable_dt: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  return launch_status(exa::launch_stable_dt(c->k, static_cast<hipStream_t>(stream), ncells, d_luh, d_h,
+                                             d_dt_out),
+                       "exa_batch_stable_dt");
+}
+
+}  // extern "C"
+
+// Accessors for the mesh driver (same library).
+namespace exa {
+const KernelCfg& ctx_cfg(const exa_ctx* c) { return c->k; }
+}  // namespace exa

@@ -1,0 +1,60 @@
+// Internal declarations shared by the C-ABI layer, the mesh driver and the
+// HIP kernels.  Not part of the public interface (include/exahype_amd.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "pde.h"
+#include "tables.h"
+
+namespace exa {
+
+// Operator tables as stored on the device (one block per context).
+struct DevTables {
+  double w[kMaxBasis];
+  double x[kMaxBasis];
+  double F0[kMaxBasis];
+  double F1[kMaxBasis];
+  double Kxi[kMaxBasis * kMaxBasis];
+  double iK1[kMaxBasis * kMaxBasis];
+  double dudx[kMaxBasis * kMaxBasis];
+  // FP32 copies for the reduced-precision predictor
+  float wf[kMaxBasis];
+  float F0f[kMaxBasis];
+  float Kxif[kMaxBasis * kMaxBasis];
+  float iK1f[kMaxBasis * kMaxBasis];
+};
+
+struct KernelCfg {
+  int dim, order, nvar, npar;
+  int precision;   // exa_precision
+  int picard_max;  // <= 0 adaptive
+  double cfl, pnpm;
+  PdeParams pde;
+  const DevTables* tab;  // device pointer
+};
+
+// Launchers (return hipError_t as int; 0 on success, -1 for an unsupported
+// (dim, order, nvar, npar, precision) combination).
+int launch_predictor(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
+                     const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard);
+int launch_riemann(const KernelCfg& k, hipStream_t s, long nfaces, const int* faces, const double* lQhbnd,
+                   double* lFhbnd);
+int launch_corrector(const KernelCfg& k, hipStream_t s, long ncells, double* luh, double* lduh,
+                     const double* lFhbnd, const double* h, const double* dt, double* dt_next);
+int launch_stable_dt(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
+                     double* dt_out);
+// Single-cell helper kernels for the compatibility entry points.
+int launch_face_integral(const KernelCfg& k, hipStream_t s, double* lduh, const double* lFface, int direction,
+                         int orientation, double invdx_scaled);
+int launch_solution_update(const KernelCfg& k, hipStream_t s, double* luh, const double* luhOld,
+                           const double* lduh, const double* dt);
+// Time bookkeeping of the device step: t += dt; dt = dt_next; dt_next = DBL_MAX.
+int launch_advance_time(hipStream_t s, double* state);
+int launch_fill(hipStream_t s, double* p, long n, double value);
+
+void set_error(const std::string& msg);
+
+}  // namespace exa

@@ -1,0 +1,307 @@
+// Host mesh driver: the part of the reference's Peano traversal + solver
+// orchestration that the hot path needs, re-implemented for device-resident
+// patch arrays.  One global time step (non-fused global scheme of
+// ExaHyPE/exahype/solvers/ADERDGSolver.cpp):
+//   1. predictor + volume integral for all cells   (predictionAndVolumeIntegralBody, :887-947)
+//   2. Riemann solve on all faces                   (solveRiemannProblemAtInterface, :1764-1823)
+//   3. face integrals into the update, update       (surfaceIntegral :1190-1239, addUpdateToSolution :1241-1256)
+//   4. dt_{n+1} = min over cells stableTimeStepSize (startNewTimeStep :1031-1049, reduce :458-462)
+// The first dt comes from the initial solution (updateTimeStepSize, :1051-1060).
+// Periodic neighbours are this driver's own rule: the reference ADER-DG path
+// has no periodic boundary.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <string>
+#include <vector>
+
+#include "../../include/exahype_amd.h"
+#include "engine.h"
+
+namespace exa {
+const KernelCfg& ctx_cfg(const exa_ctx* c);
+}
+
+struct exa_mesh {
+  exa_ctx* ctx = nullptr;
+  exa::KernelCfg k{};
+  int n = 0;
+  long ncells = 0, nfaces = 0;
+  int NS = 0, NF = 0, ND = 0;
+  double h = 0;
+  hipStream_t stream = nullptr;
+  double *d_luh = nullptr, *d_lduh = nullptr, *d_qb = nullptr, *d_fb = nullptr, *d_h = nullptr;
+  double* d_state = nullptr;  // {t, dt, dt_next}
+  int* d_faces = nullptr;
+  int* d_picard = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double ms_pred = 0, ms_riem = 0, ms_corr = 0;
+  int timed_steps = 0;
+};
+
+namespace {
+
+int fail(hipError_t e, const char* what) {
+  exa::set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return EXA_ERR_HIP;
+}
+
+#define HIP_OK(expr, what)                          \
+  do {                                              \
+    hipError_t _e = (expr);                         \
+    if (_e != hipSuccess) return fail(_e, what);    \
+  } while (0)
+
+int status(int rc, const char* what) {
+  if (rc == -1) {
+    exa::set_error(std::string(what) + ": shape not compiled into this library");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  if (rc != 0) return fail(static_cast<hipError_t>(rc), what);
+  return EXA_OK;
+}
+
+// Enqueue one global time step on m->stream.
+int enqueue_step(exa_mesh* m, bool with_events) {
+  double* st = m->d_state;
+  if (with_events) HIP_OK(hipEventRecord(m->ev[0], m->stream), "event");
+  int rc = status(exa::launch_predictor(m->k, m->stream, m->ncells, m->d_luh, m->d_h, st + 1, m->d_lduh, m->d_qb,
+                                        m->d_fb, m->d_picard),
+                  "predictor");
+  if (rc) return rc;
+  if (with_events) HIP_OK(hipEventRecord(m->ev[1], m->stream), "event");
+  rc = status(exa::launch_riemann(m->k, m->stream, m->nfaces, m->d_faces, m->d_qb, m->d_fb), "riemann");
+  if (rc) return rc;
+  if (with_events) HIP_OK(hipEventRecord(m->ev[2], m->stream), "event");
+  rc = status(exa::launch_corrector(m->k, m->stream, m->ncells, m->d_luh, m->d_lduh, m->d_fb, m->d_h, st + 1,
+                                    st + 2),
+              "corrector");
+  if (rc) return rc;
+  rc = status(exa::launch_advance_time(m->stream, st), "advance time");
+  if (rc) return rc;
+  if (with_events) HIP_OK(hipEventRecord(m->ev[3], m->stream), "event");
+  return EXA_OK;
+}
+
+void free_mesh(exa_mesh* m) {
+  if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  for (auto& e : m->ev)
+    if (e) (void)hipEventDestroy(e);
+  double* bufs[] = {m->d_luh, m->d_lduh, m->d_qb, m->d_fb, m->d_h, m->d_state};
+  for (double* b : bufs)
+    if (b) (void)hipFree(b);
+  if (m->d_faces) (void)hipFree(m->d_faces);
+  if (m->d_picard) (void)hipFree(m->d_picard);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int exa_mesh_create_uniform(exa_mesh** out, exa_ctx* ctx, int n, double h) {
+  if (!out || !ctx || n < 1 || !(h > 0)) {
+    exa::set_error("exa_mesh_create_uniform: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  *out = nullptr;
+  exa_mesh* m = new exa_mesh;
+  m->ctx = ctx;
+  m->k = exa::ctx_cfg(ctx);
+  const int D = m->k.dim, B = m->k.order + 1;
+  m->n = n;
+  m->h = h;
+  m->ncells = (D == 3) ? long(n) * n * n : long(n) * n;
+  m->nfaces = D * m->ncells;
+  m->NS = (D == 3) ? B * B * B : B * B;
+  m->NF = (D == 3) ? B * B : B;
+  m->ND = m->k.nvar + m->k.npar;
+  const int V = m->k.nvar;
+  hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&m->d_luh, sizeof(double) * m->ncells * m->NS * m->ND);
+  if (e == hipSuccess) e = hipMalloc(&m->d_lduh, sizeof(double) * m->ncells * m->NS * V);
+  if (e == hipSuccess) e = hipMalloc(&m->d_qb, sizeof(double) * m->ncells * 2 * D * m->NF * m->ND);
+  if (e == hipSuccess) e = hipMalloc(&m->d_fb, sizeof(double) * m->ncells * 2 * D * m->NF * V);
+  if (e == hipSuccess) e = hipMalloc(&m->d_h, sizeof(double) * m->ncells);
+  if (e == hipSuccess) e = hipMalloc(&m->d_state, sizeof(double) * 4);
+  if (e == hipSuccess) e = hipMalloc(&m->d_faces, sizeof(int) * 4 * m->nfaces);
+  if (e == hipSuccess) e = hipMalloc(&m->d_picard, sizeof(int) * m->ncells);
+  for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&m->ev[i]);
+  if (e != hipSuccess) {
+    free_mesh(m);
+    return fail(e, "exa_mesh_create_uniform");
+  }
+  // faces ordered by direction, then by left cell (x fastest)
+  std::vector<int> faces(4 * m->nfaces);
+  long f = 0;
+  for (int d = 0; d < D; d++)
+    for (long c = 0; c < m->ncells; c++) {
+      long ix = c % n, iy = (c / n) % n, iz = (D == 3) ? c / (long(n) * n) : 0;
+      if (d == 0) ix = (ix + 1) % n;
+      if (d == 1) iy = (iy + 1) % n;
+      if (d == 2) iz = (iz + 1) % n;
+      const long r = (D == 3) ? (iz * n + iy) * n + ix : iy * n + ix;
+      faces[4 * f + 0] = static_cast<int>(c);
+      faces[4 * f + 1] = static_cast<int>(r);
+      faces[4 * f + 2] = d;
+      faces[4 * f + 3] = 0;
+      f++;
+    }
+  std::vector<double> hs(m->ncells, h);
+  const double st[4] = {0.0, 0.0, DBL_MAX, 0.0};
+  e = hipMemcpy(m->d_faces, faces.data(), sizeof(int) * faces.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->d_h, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->d_state, st, sizeof(st), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(m->d_picard, 0, sizeof(int) * m->ncells);
+  if (e != hipSuccess) {
+    free_mesh(m);
+    return fail(e, "exa_mesh_create_uniform(upload)");
+  }
+  *out = m;
+  return EXA_OK;
+}
+
+int exa_mesh_destroy(exa_mesh* m) {
+  if (!m) return EXA_OK;
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  free_mesh(m);
+  return EXA_OK;
+}
+
+long exa_mesh_ncells(const exa_mesh* m) { return m ? m->ncells : 0; }
+
+int exa_mesh_set_solution(exa_mesh* m, const double* luh) {
+  if (!m || !luh) {
+    exa::set_error("exa_mesh_set_solution: NULL argument");
+    return EXA_ERR_ARG;
+  }
+  HIP_OK(hipMemcpyAsync(m->d_luh, luh, sizeof(double) * m->ncells * m->NS * m->ND, hipMemcpyHostToDevice,
+                        m->stream),
+         "set_solution");
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
+  return EXA_OK;
+}
+
+int exa_mesh_get_solution(exa_mesh* m, double* luh) {
+  if (!m || !luh) {
+    exa::set_error("exa_mesh_get_solution: NULL argument");
+    return EXA_ERR_ARG;
+  }
+  HIP_OK(hipMemcpyAsync(luh, m->d_luh, sizeof(double) * m->ncells * m->NS * m->ND, hipMemcpyDeviceToHost,
+                        m->stream),
+         "get_solution");
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
+  return EXA_OK;
+}
+
+double* exa_mesh_device_solution(exa_mesh* m) { return m ? m->d_luh : nullptr; }
+
+int exa_mesh_init_dt(exa_mesh* m, double dt) {
+  if (!m) {
+    exa::set_error("exa_mesh_init_dt: NULL mesh");
+    return EXA_ERR_ARG;
+  }
+  const double st[4] = {0.0, dt > 0 ? dt : DBL_MAX, DBL_MAX, 0.0};
+  HIP_OK(hipMemcpyAsync(m->d_state, st, sizeof(st), hipMemcpyHostToDevice, m->stream), "init_dt");
+  if (!(dt > 0)) {
+    const int rc = status(exa::launch_stable_dt(m->k, m->stream, m->ncells, m->d_luh, m->d_h, m->d_state + 1),
+                          "initial dt");
+    if (rc) return rc;
+  }
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
+  return EXA_OK;
+}
+
+int exa_mesh_run(exa_mesh* m, int nsteps, int use_graph) {
+  if (!m || nsteps < 0) {
+    exa::set_error("exa_mesh_run: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  if (m->timing) {
+    m->ms_pred = m->ms_riem = m->ms_corr = 0;
+    m->timed_steps = 0;
+    for (int i = 0; i < nsteps; i++) {
+      int rc = enqueue_step(m, true);
+      if (rc) return rc;
+      HIP_OK(hipEventSynchronize(m->ev[3]), "event sync");
+      float a = 0, b = 0, c = 0;
+      HIP_OK(hipEventElapsedTime(&a, m->ev[0], m->ev[1]), "elapsed");
+      HIP_OK(hipEventElapsedTime(&b, m->ev[1], m->ev[2]), "elapsed");
+      HIP_OK(hipEventElapsedTime(&c, m->ev[2], m->ev[3]), "elapsed");
+      m->ms_pred += a;
+      m->ms_riem += b;
+      m->ms_corr += c;
+      m->timed_steps++;
+    }
+    return EXA_OK;
+  }
+  if (use_graph) {
+    if (!m->graph_exec) {
+      HIP_OK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+      const int rc = enqueue_step(m, false);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(m->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(e, "end capture");
+      m->graph = g;
+      HIP_OK(hipGraphInstantiate(&m->graph_exec, m->graph, nullptr, nullptr, 0), "instantiate");
+    }
+    for (int i = 0; i < nsteps; i++) HIP_OK(hipGraphLaunch(m->graph_exec, m->stream), "graph launch");
+    return EXA_OK;
+  }
+  for (int i = 0; i < nsteps; i++) {
+    const int rc = enqueue_step(m, false);
+    if (rc) return rc;
+  }
+  return EXA_OK;
+}
+
+int exa_mesh_sync(exa_mesh* m) {
+  if (!m) return EXA_ERR_ARG;
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
+  return EXA_OK;
+}
+
+int exa_mesh_time(exa_mesh* m, double* t, double* dt) {
+  if (!m) return EXA_ERR_ARG;
+  double st[4];
+  HIP_OK(hipMemcpyAsync(st, m->d_state, sizeof(st), hipMemcpyDeviceToHost, m->stream), "time");
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
+  if (t) *t = st[0];
+  if (dt) *dt = st[1];
+  return EXA_OK;
+}
+
+int exa_mesh_picard(exa_mesh* m, int* counts) {
+  if (!m || !counts) return EXA_ERR_ARG;
+  HIP_OK(hipMemcpyAsync(counts, m->d_picard, sizeof(int) * m->ncells, hipMemcpyDeviceToHost, m->stream), "picard");
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
+  return EXA_OK;
+}
+
+void* exa_mesh_stream(exa_mesh* m) { return m ? static_cast<void*>(m->stream) : nullptr; }
+
+int exa_mesh_set_timing(exa_mesh* m, int enabled) {
+  if (!m) return EXA_ERR_ARG;
+  m->timing = enabled != 0;
+  return EXA_OK;
+}
+
+int exa_mesh_phase_times(exa_mesh* m, double* a, double* b, double* c, int* n) {
+  if (!m) return EXA_ERR_ARG;
+  if (a) *a = m->ms_pred;
+  if (b) *b = m->ms_riem;
+  if (c) *c = m->ms_corr;
+  if (n) *n = m->timed_steps;
+  return EXA_OK;
+}
+
+}  // extern "C"

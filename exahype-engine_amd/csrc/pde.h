@@ -1,0 +1,69 @@
+// Device PDE functors for the nonlinear ADER-DG path.
+//
+// Euler (gamma = 1.4), ApplicationExamples/Euler/Euler_ADERDG/EulerSolver_ADERDG.cpp:
+//   flux        :61-115  (2D keeps 5 variables, |j|^2 drops Q[3] in 2D, :81-85)
+//   eigenvalues :117-147 (u_n -+ c, u_n x3)
+// Linear advection (test PDE): F_d = a_d Q, eigenvalues a_d.
+//
+// The expression order follows the reference hooks so that FP64 results
+// agree to rounding level.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace exa {
+
+enum PdeId { kPdeEuler = 0, kPdeAdvection = 2 };
+
+struct PdeParams {
+  int id;
+  double a[3];  // advection velocity
+};
+
+template <int D, typename R>
+__device__ __forceinline__ void pde_flux(const PdeParams& P, const R* __restrict__ Q, R (&F)[D][5]) {
+  if (P.id == kPdeAdvection) {
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int v = 0; v < 5; v++) F[d][v] = static_cast<R>(P.a[d]) * Q[v];
+    return;
+  }
+  constexpr R gamma = static_cast<R>(1.4);
+  const R irho = R(1) / Q[0];
+  const R j2 = (D == 3) ? Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3] : Q[1] * Q[1] + Q[2] * Q[2];
+  const R p = (gamma - R(1)) * (Q[4] - R(0.5) * irho * j2);
+  F[0][0] = Q[1];
+  F[0][1] = irho * Q[1] * Q[1] + p;
+  F[0][2] = irho * Q[2] * Q[1];
+  F[0][3] = irho * Q[3] * Q[1];
+  F[0][4] = irho * (Q[4] + p) * Q[1];
+  F[1][0] = Q[2];
+  F[1][1] = irho * Q[1] * Q[2];
+  F[1][2] = irho * Q[2] * Q[2] + p;
+  F[1][3] = irho * Q[3] * Q[2];
+  F[1][4] = irho * (Q[4] + p) * Q[2];
+  if constexpr (D == 3) {
+    F[2][0] = Q[3];
+    F[2][1] = irho * Q[1] * Q[3];
+    F[2][2] = irho * Q[2] * Q[3];
+    F[2][3] = irho * Q[3] * Q[3] + p;
+    F[2][4] = irho * (Q[4] + p) * Q[3];
+  }
+}
+
+// max_v |lambda_v(Q, d)| with the reference's eigenvalue expressions.
+template <int D>
+__device__ __forceinline__ double pde_max_abs_eig(const PdeParams& P, const double* __restrict__ Q, int d) {
+  if (P.id == kPdeAdvection) return fabs(P.a[d]);
+  constexpr double gamma = 1.4;
+  const double irho = 1. / Q[0];
+  const double j2 = (D == 3) ? Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3] : Q[1] * Q[1] + Q[2] * Q[2];
+  const double p = (gamma - 1) * (Q[4] - 0.5 * irho * j2);
+  const double u_n = Q[d + 1] * irho;
+  const double c = sqrt(gamma * p * irho);
+  const double l0 = fabs(u_n - c), l1 = fabs(u_n), l4 = fabs(u_n + c);
+  return fmax(fmax(l0, l1), l4);
+}
+
+}  // namespace exa

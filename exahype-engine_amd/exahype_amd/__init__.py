@@ -1,0 +1,276 @@
+"""ctypes binding of libexahype_amd.so (the C ABI in include/exahype_amd.h).
+
+The product path: every call goes to the HIP library.  There is no CPU
+fallback -- importing this module raises if the library is missing, and every
+entry point raises ``ExaError`` with the library's message on failure.
+
+Method names mirror the reference's ADER-DG kernel API
+(exahype::solvers::ADERDGSolver, ExaHyPE/exahype/solvers/ADERDGSolver.h:2392-2660).
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+ENGINE_DIR = os.path.dirname(PKG_DIR)
+REPO_DIR = os.path.dirname(ENGINE_DIR)
+LIB_PATH = os.path.join(ENGINE_DIR, "libexahype_amd.so")
+HEADER_PATH = os.path.join(REPO_DIR, "include", "exahype_amd.h")
+
+PDE_EULER, PDE_ADVECTION = 0, 2
+PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+EXA_OK, EXA_ERR_ARG, EXA_ERR_HIP, EXA_ERR_UNSUPPORTED, EXA_ERR_NOMEM = 0, 1, 2, 3, 4
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"exahype_amd: {LIB_PATH} is missing -- build it (python -c 'import __graft_entry__ as g; g.build()')")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_cp = ctypes.c_void_p  # opaque handles
+
+
+class ExaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"exahype_amd error {code}: {msg}")
+        self.code = code
+
+
+def _sig(name, args, res=ctypes.c_int):
+    f = getattr(_lib, name)
+    f.argtypes = args
+    f.restype = res
+    return f
+
+
+_sig("exa_last_error", [], ctypes.c_char_p)
+_sig("exa_ctx_create", [ctypes.POINTER(_cp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                        ctypes.c_int, ctypes.c_int, ctypes.c_double])
+_sig("exa_ctx_destroy", [_cp])
+_sig("exa_ctx_set_advection", [_cp, _dp])
+_sig("exa_tables", [ctypes.c_int] + [_dp] * 9)
+_sig("exa_ctx_tables", [_cp] + [_dp] * 8)
+_sig("exa_fusedSpaceTimePredictorVolumeIntegral",
+     [_cp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ip])
+_sig("exa_riemannSolver", [_cp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int,
+                           ctypes.c_int, ctypes.c_int])
+_sig("exa_faceIntegral", [_cp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
+                          ctypes.c_int])
+_sig("exa_surfaceIntegral", [_cp, _dp, _dp, _dp])
+_sig("exa_addUpdateToSolution", [_cp, _dp, _dp, _dp, ctypes.c_double])
+_sig("exa_stableTimeStepSize", [_cp, _dp, _dp, _dp])
+_sig("exa_batch_predictor", [_cp, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp, _vp])
+_sig("exa_batch_riemann", [_cp, _vp, ctypes.c_long, _vp, _vp, _vp])
+_sig("exa_batch_corrector", [_cp, _vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp])
+_sig("exa_batch_stable_dt", [_cp, _vp, ctypes.c_long, _vp, _vp, _vp])
+_sig("exa_mesh_create_uniform", [ctypes.POINTER(_cp), _cp, ctypes.c_int, ctypes.c_double])
+_sig("exa_mesh_destroy", [_cp])
+_sig("exa_mesh_ncells", [_cp], ctypes.c_long)
+_sig("exa_mesh_set_solution", [_cp, _dp])
+_sig("exa_mesh_get_solution", [_cp, _dp])
+_sig("exa_mesh_device_solution", [_cp], ctypes.c_void_p)
+_sig("exa_mesh_init_dt", [_cp, ctypes.c_double])
+_sig("exa_mesh_run", [_cp, ctypes.c_int, ctypes.c_int])
+_sig("exa_mesh_sync", [_cp])
+_sig("exa_mesh_time", [_cp, _dp, _dp])
+_sig("exa_mesh_picard", [_cp, _ip])
+_sig("exa_mesh_stream", [_cp], ctypes.c_void_p)
+_sig("exa_mesh_set_timing", [_cp, ctypes.c_int])
+_sig("exa_mesh_phase_times", [_cp, _dp, _dp, _dp, _ip])
+
+
+def declared_symbols():
+    """Names of all functions declared in include/exahype_amd.h."""
+    text = open(HEADER_PATH).read()
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(exa_\w+)\s*\(", text)))
+
+
+def _check(rc):
+    if rc != EXA_OK:
+        msg = _lib.exa_last_error()
+        raise ExaError(rc, msg.decode() if msg else "")
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "expected a C-contiguous float64 array"
+    return a.ctypes.data_as(_dp)
+
+
+def tables(order):
+    """Operator tables derived by the library (host only, no device needed)."""
+    B = order + 1
+    out = {k: np.zeros(n) for k, n in (("w", B), ("x", B), ("Kxi", B * B), ("iK1", B * B), ("dudx", B * B),
+                                       ("F0", B), ("F1", B), ("fgp", 3 * B * B), ("egp", B * B))}
+    _check(_lib.exa_tables(order, *[_p(out[k]) for k in ("w", "x", "Kxi", "iK1", "dudx", "F0", "F1", "fgp", "egp")]))
+    for k in ("Kxi", "iK1", "dudx", "egp"):
+        out[k] = out[k].reshape(B, B)
+    out["fgp"] = out["fgp"].reshape(3, B, B)
+    return out
+
+
+class Context:
+    """One solver configuration; methods follow the reference kernel API."""
+
+    def __init__(self, dim, order, nvar=5, npar=0, pde=PDE_EULER, precision=PREC_FP64, picard_max=0, cfl=0.9):
+        h = _cp()
+        _check(_lib.exa_ctx_create(ctypes.byref(h), dim, order, nvar, npar, pde, precision, picard_max, cfl))
+        self.h = h
+        self.dim, self.order, self.nvar, self.npar = dim, order, nvar, npar
+        self.B = order + 1
+
+    def close(self):
+        if self.h:
+            _lib.exa_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nd(self):
+        return self.nvar + self.npar
+
+    @property
+    def nspace(self):
+        return self.B ** self.dim
+
+    @property
+    def nface(self):
+        return self.B ** (self.dim - 1)
+
+    def set_advection(self, a):
+        _check(_lib.exa_ctx_set_advection(self.h, _p(np.ascontiguousarray(a, dtype=np.float64))))
+
+    # -- per-cell compatibility API ---------------------------------------
+    def fusedSpaceTimePredictorVolumeIntegral(self, luh, dx, t, dt, addVolumeIntegralResultToUpdate=True):
+        D, V, ND = self.dim, self.nvar, self.nd
+        luh = np.array(luh, dtype=np.float64)
+        lduh = np.zeros(self.nspace * V)
+        lQhbnd = np.zeros(2 * D * self.nface * ND)
+        lFhbnd = np.zeros(2 * D * self.nface * V)
+        centre = np.zeros(D)
+        pic = ctypes.c_int(0)
+        _check(_lib.exa_fusedSpaceTimePredictorVolumeIntegral(
+            self.h, _p(lduh), _p(lQhbnd), None, _p(lFhbnd), _p(luh), _p(centre),
+            _p(np.array(dx, dtype=np.float64)), t, dt, int(bool(addVolumeIntegralResultToUpdate)), ctypes.byref(pic)))
+        return dict(iters=pic.value, lduh=lduh, lQhbnd=lQhbnd, lFhbnd=lFhbnd, luh=luh)
+
+    def riemannSolver(self, FL, FR, QL, QR, direction, dx=None, t=0.0, dt=0.0):
+        FL = np.array(FL, dtype=np.float64)
+        FR = np.array(FR, dtype=np.float64)
+        dxa = np.array(dx if dx is not None else [1.0] * self.dim, dtype=np.float64)
+        _check(_lib.exa_riemannSolver(self.h, _p(FL), _p(FR), _p(np.ascontiguousarray(QL, dtype=np.float64)),
+                                      _p(np.ascontiguousarray(QR, dtype=np.float64)), t, dt, _p(dxa), direction, 0, 0))
+        return FL, FR
+
+    def faceIntegral(self, out, lFhbnd_face, direction, orientation, dx, dt=0.0, addToUpdate=True):
+        out = np.array(out, dtype=np.float64)
+        _check(_lib.exa_faceIntegral(self.h, _p(out), _p(np.ascontiguousarray(lFhbnd_face, dtype=np.float64)),
+                                     direction, orientation, 0, _p(np.array(dx, dtype=np.float64)), dt,
+                                     int(bool(addToUpdate))))
+        return out
+
+    def surfaceIntegral(self, lduh, lFhbnd, dx):
+        lduh = np.array(lduh, dtype=np.float64)
+        _check(_lib.exa_surfaceIntegral(self.h, _p(lduh), _p(np.ascontiguousarray(lFhbnd, dtype=np.float64)),
+                                        _p(np.array(dx, dtype=np.float64))))
+        return lduh
+
+    def addUpdateToSolution(self, luh, lduh, dt):
+        luh = np.array(luh, dtype=np.float64)
+        _check(_lib.exa_addUpdateToSolution(self.h, _p(luh), _p(luh.copy()),
+                                            _p(np.ascontiguousarray(lduh, dtype=np.float64)), dt))
+        return luh
+
+    def stableTimeStepSize(self, luh, dx):
+        out = ctypes.c_double(0)
+        _check(_lib.exa_stableTimeStepSize(self.h, _p(np.ascontiguousarray(luh, dtype=np.float64)),
+                                           _p(np.array(dx, dtype=np.float64)), ctypes.byref(out)))
+        return out.value
+
+    # -- batched device API (device pointers as ints, e.g. torch data_ptr()) --
+    def batch_predictor(self, stream, ncells, luh, h, dt, lduh, lQhbnd, lFhbnd, picard=None):
+        _check(_lib.exa_batch_predictor(self.h, stream, ncells, luh, h, dt, lduh, lQhbnd, lFhbnd, picard))
+
+    def batch_riemann(self, stream, nfaces, faces, lQhbnd, lFhbnd):
+        _check(_lib.exa_batch_riemann(self.h, stream, nfaces, faces, lQhbnd, lFhbnd))
+
+    def batch_corrector(self, stream, ncells, luh, lduh, lFhbnd, h, dt, dt_next):
+        _check(_lib.exa_batch_corrector(self.h, stream, ncells, luh, lduh, lFhbnd, h, dt, dt_next))
+
+    def batch_stable_dt(self, stream, ncells, luh, h, dt_out):
+        _check(_lib.exa_batch_stable_dt(self.h, stream, ncells, luh, h, dt_out))
+
+
+class Mesh:
+    """Uniform periodic n^D mesh with device-resident patch arrays."""
+
+    def __init__(self, ctx, n, h):
+        m = _cp()
+        _check(_lib.exa_mesh_create_uniform(ctypes.byref(m), ctx.h, n, h))
+        self.h = m
+        self.ctx = ctx
+        self.n = n
+        self.ncells = _lib.exa_mesh_ncells(m)
+
+    def close(self):
+        if self.h:
+            _lib.exa_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_solution(self, luh):
+        luh = np.ascontiguousarray(luh, dtype=np.float64)
+        assert luh.size == self.ncells * self.ctx.nspace * self.ctx.nd
+        _check(_lib.exa_mesh_set_solution(self.h, _p(luh)))
+
+    def get_solution(self):
+        out = np.zeros(self.ncells * self.ctx.nspace * self.ctx.nd)
+        _check(_lib.exa_mesh_get_solution(self.h, _p(out)))
+        return out
+
+    def init_dt(self, dt=0.0):
+        _check(_lib.exa_mesh_init_dt(self.h, dt))
+
+    def run(self, nsteps, use_graph=False):
+        _check(_lib.exa_mesh_run(self.h, nsteps, int(bool(use_graph))))
+
+    def sync(self):
+        _check(_lib.exa_mesh_sync(self.h))
+
+    def time(self):
+        t, dt = ctypes.c_double(0), ctypes.c_double(0)
+        _check(_lib.exa_mesh_time(self.h, ctypes.byref(t), ctypes.byref(dt)))
+        return t.value, dt.value
+
+    def picard(self):
+        out = np.zeros(self.ncells, dtype=np.int32)
+        _check(_lib.exa_mesh_picard(self.h, out.ctypes.data_as(_ip)))
+        return out
+
+    @property
+    def stream(self):
+        return _lib.exa_mesh_stream(self.h)
+
+    def set_timing(self, enabled):
+        _check(_lib.exa_mesh_set_timing(self.h, int(bool(enabled))))
+
+    def phase_times(self):
+        a, b, c, n = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
+        _check(_lib.exa_mesh_phase_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(n)))
+        return dict(predictor_ms=a.value, riemann_ms=b.value, corrector_ms=c.value, steps=n.value)

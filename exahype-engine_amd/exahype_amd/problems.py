@@ -1,0 +1,98 @@
+"""Initial states of the benchmark/parity workloads (synthetic, seeded).
+
+Nodal initialisation at the Gauss-Legendre nodes of each cell, as the
+reference's solutionAdjustment does (ExaHyPE/kernels/aderdg/generic/c/3d/
+solutionAdjustment.cpph:18-63: x = centre + dx * (xi - 1/2)).  Layout
+luh[cell][z][y][x][var], cells x fastest, unit cube / square, n cells per side.
+
+* density_wave_2d  -- 2D Euler, rho = 1 + 0.2 sin(2 pi (x + y)), u = v = 1, p = 1
+* density_wave_3d  -- 3D Euler, rho = 1 + 0.2 sin(2 pi (x + y + z)) + sum of
+                      M seeded Fourier modes of amplitude 0.01/M (|k_i| <= 4) in
+                      rho and in the x-velocity u; v = w = 1, p = 1
+"""
+import numpy as np
+
+GAMMA = 1.4
+
+
+def _nodes(order):
+    x, _ = np.polynomial.legendre.leggauss(order + 1)
+    return (x + 1.0) / 2.0
+
+
+def fourier_modes(seed=0, modes=8, kmax=4):
+    """(k[M,3], phase[M]) for rho and for u, drawn from default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(2):
+        ks = []
+        while len(ks) < modes:
+            k = rng.integers(-kmax, kmax + 1, size=3)
+            if np.any(k != 0):
+                ks.append(k)
+        ph = rng.uniform(0.0, 2.0 * np.pi, size=modes)
+        out.append((np.array(ks, dtype=np.float64), ph))
+    return out
+
+
+def _coords(n, order, dim, xp, device):
+    h = 1.0 / n
+    if xp is np:
+        xi = _nodes(order)
+        ar = np.arange(n, dtype=np.float64)
+    else:
+        xi = xp.as_tensor(_nodes(order), dtype=xp.float64, device=device)
+        ar = xp.arange(n, dtype=xp.float64, device=device)
+    centres = (ar + 0.5) * h
+    # 1D node coordinates per cell index: [n][B]
+    x1 = centres[:, None] + h * (xi[None, :] - 0.5)
+    return x1, h
+
+
+def density_wave_2d(n, order, xp=np, device=None):
+    x1, h = _coords(n, order, 2, xp, device)
+    B = order + 1
+    # cell (iy, ix), node (jy, jx)
+    X = x1[None, :, None, :]                  # [1, ix, 1, jx]
+    Y = x1[:, None, :, None]                  # [iy, 1, jy, 1]
+    rho = 1.0 + 0.2 * xp.sin(2 * np.pi * (X + Y))
+    rho = rho + 0 * (X + Y)
+    u = v = 1.0
+    p = 1.0
+    Q = xp.zeros((n, n, B, B, 5), dtype=xp.float64, **({} if xp is np else {"device": device}))
+    Q[..., 0] = rho
+    Q[..., 1] = rho * u
+    Q[..., 2] = rho * v
+    Q[..., 4] = p / (GAMMA - 1) + 0.5 * rho * (u * u + v * v)
+    # [iy, ix, jy, jx, v] -> [cell=(iy, ix)][jy][jx][v]
+    return Q.reshape(-1)
+
+
+def density_wave_3d(n, order, seed=0, modes=8, xp=np, device=None):
+    x1, h = _coords(n, order, 3, xp, device)
+    B = order + 1
+    (kr, pr), (ku, pu) = fourier_modes(seed, modes)
+    kw = {} if xp is np else {"device": device}
+    # broadcast shapes: [iz, iy, ix, jz, jy, jx]
+    X = x1.reshape(1, 1, n, 1, 1, B)
+    Y = x1.reshape(1, n, 1, 1, B, 1)
+    Z = x1.reshape(n, 1, 1, B, 1, 1)
+    rho = 1.0 + 0.2 * xp.sin(2 * np.pi * (X + Y + Z))
+    u = 1.0 + 0.0 * (X + Y + Z)
+    amp = 0.01 / modes
+    for m in range(modes):
+        rho = rho + amp * xp.cos(2 * np.pi * (kr[m, 0] * X + kr[m, 1] * Y + kr[m, 2] * Z) + pr[m])
+        u = u + amp * xp.cos(2 * np.pi * (ku[m, 0] * X + ku[m, 1] * Y + ku[m, 2] * Z) + pu[m])
+    v = w = 1.0
+    p = 1.0
+    Q = xp.zeros((n, n, n, B, B, B, 5), dtype=xp.float64, **kw)
+    Q[..., 0] = rho
+    Q[..., 1] = rho * u
+    Q[..., 2] = rho * v
+    Q[..., 3] = rho * w
+    Q[..., 4] = p / (GAMMA - 1) + 0.5 * rho * (u * u + v * v + w * w)
+    return Q.reshape(-1)
+
+
+def dof_per_cell(dim, order, nvar=5):
+    return (order + 1) ** dim * nvar

@@ -1,0 +1,195 @@
+/*
+ * exahype_amd -- MI355X-native ADER-DG cell-update engine, C ABI.
+ *
+ * Drop-in boundary for the reference's ADER-DG kernel plugin API
+ * (exahype::solvers::ADERDGSolver pure virtuals, ExaHyPE/exahype/solvers/ADERDGSolver.h:2392-2660,
+ * implemented per application by the generated Abstract<Solver>,
+ * Toolkit/exahype/toolkit/templates/solvers/AbstractGenericADERDGSolverImplementation.template).
+ *
+ * Conventions (identical to the reference generic kernels):
+ *   - all arrays are FP64, row-major, per cell:
+ *       luh    [B^D][nVar+nPar]            (z,y,x, data fastest)
+ *       lduh   [B^D][nVar]
+ *       lQhbnd [2D][B^(D-1)][nVar+nPar]    face f = 2*dir + orientation,
+ *       lFhbnd [2D][B^(D-1)][nVar]         x-faces (z,y), y-faces (z,x), z-faces (y,x)
+ *   - B = order + 1 Gauss-Legendre nodes on [0,1] per dimension.
+ *   - batched entry points take DEVICE pointers to arrays of cells stored back
+ *     to back in that per-cell layout, and a hipStream_t passed as void*.
+ *   - every function returns EXA_OK (0) or an error code; exa_last_error()
+ *     returns a description of the last error of the calling thread.
+ *     Nothing aborts the process (the reference calls std::abort /
+ *     std::terminate instead, e.g. generic/c/3d/faceIntegralNonlinear.cpph:97-101).
+ *   - a context is not thread-safe; use one context per host thread / stream.
+ */
+#ifndef EXAHYPE_AMD_H
+#define EXAHYPE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum exa_status {
+  EXA_OK = 0,
+  EXA_ERR_ARG = 1,          /* invalid argument / size                         */
+  EXA_ERR_HIP = 2,          /* HIP runtime error (launch, copy, allocation)    */
+  EXA_ERR_UNSUPPORTED = 3,  /* combination not compiled into this library      */
+  EXA_ERR_NOMEM = 4         /* device or host allocation failed                */
+};
+
+/* PDE hooks are compiled device functors (device code cannot call host
+ * virtuals).  Reference hooks: flux / eigenvalues
+ * (AbstractGenericADERDGSolverHeader.template:220,311-313). */
+enum exa_pde {
+  EXA_PDE_EULER = 0,      /* ApplicationExamples/Euler/Euler_ADERDG/EulerSolver_ADERDG.cpp:61-147 */
+  EXA_PDE_ADVECTION = 2   /* linear advection F_d = a_d Q, eigenvalues a_d (test PDE)          */
+};
+
+/* Arithmetic of the space-time predictor (Picard loop, time averages,
+ * extrapolation, volume integral).  Riemann solve, face integrals, solution
+ * update, Picard residual and CFL reduction always run in FP64. */
+enum exa_precision {
+  EXA_PREC_FP64 = 0,
+  EXA_PREC_FP32 = 1,       /* FP32 contractions (MFMA where compiled), FP32 accumulation */
+  EXA_PREC_BF16 = 2        /* experiment: BF16 operands, FP32 accumulation                */
+};
+
+typedef struct exa_ctx exa_ctx;
+
+/* Create a solver context.
+ *   dim          2 or 3
+ *   order        N (B = N+1 nodes per dimension)
+ *   nvar, npar   NumberOfVariables, NumberOfParameters
+ *   pde          exa_pde
+ *   precision    exa_precision
+ *   picard_max   <= 0: adaptive Picard, tol 1e-7 on sqrt(sum of squared
+ *                updates), at most 2(N+1) iterations (UseMaxPicardIterations
+ *                = false, generic/c/3d/spaceTimePredictorNonlinear.cpph:84-86,407-423);
+ *                > 0: exactly picard_max iterations (UseMaxPicardIterations
+ *                = true, MaxPicardIterations = picard_max).
+ *   cfl          CFL factor (reference default 0.9)
+ * The PNPM factor follows the reference defaults per order
+ * (Toolkit/exahype/toolkit/solverController.py:17-23). */
+int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pde, int precision,
+                   int picard_max, double cfl);
+int exa_ctx_destroy(exa_ctx* ctx);
+const char* exa_last_error(void);
+
+/* Advection velocity for EXA_PDE_ADVECTION (a[dim]). */
+int exa_ctx_set_advection(exa_ctx* ctx, const double* a);
+
+/* Operator tables of order N derived on the host (no device needed; each
+ * output may be NULL).  Same layout as exa_ctx_tables plus
+ * egp[B*B] = equidistantGridProjector. */
+int exa_tables(int order, double* w, double* x, double* Kxi, double* iK1, double* dudx, double* F0, double* F1,
+               double* fgp, double* egp);
+
+/* Copy out the derived operator tables (each may be NULL):
+ * w[B], x[B], Kxi[B*B], iK1[B*B], dudx[B*B], F0[B], F1[B], fgp[3*B*B]. */
+int exa_ctx_tables(const exa_ctx* ctx, double* w, double* x, double* Kxi, double* iK1, double* dudx,
+                   double* F0, double* F1, double* fgp);
+
+/* ---------------------------------------------------------------------------
+ * Per-cell compatibility entry points: host pointers, the reference argument
+ * order and semantics.  They copy to the device, run the batched kernels on
+ * one cell and copy back (synchronous).  For drop-in use and parity tests.
+ * ------------------------------------------------------------------------- */
+
+/* fusedSpaceTimePredictorVolumeIntegral
+ * (AbstractGenericADERDGSolverImplementation.template:104-235; API ADERDGSolver.h:2542).
+ * lGradQhbnd is unused (no viscous flux) and may be NULL.  centre is unused
+ * by the compiled PDEs (no algebraic source).  Returns the Picard count in
+ * *picard_iterations with the reference's per-dimension convention
+ * (3D: 0-based index of the converged iteration, 2D: MaxIterations+2 on
+ * convergence; MaxIterations when not converged). */
+int exa_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* ctx, double* lduh, double* lQhbnd, double* lGradQhbnd,
+                                              double* lFhbnd, double* luh, const double* centre,
+                                              const double* dx, double t, double dt,
+                                              int addVolumeIntegralResultToUpdate, int* picard_iterations);
+
+/* riemannSolver (…Implementation.template:287-305 -> generic/c/3d/riemannSolverNonlinear.cpph:37-128).
+ * FL/FR are overwritten in place with the Rusanov flux. */
+int exa_riemannSolver(exa_ctx* ctx, double* FL, double* FR, const double* QL, const double* QR, double t,
+                      double dt, const double* dx, int direction, int isBoundaryFace, int faceIndex);
+
+/* faceIntegral (…Implementation.template:247-285 -> generic/c/3d/faceIntegralNonlinear.cpph:21-103).
+ * Conforming faces only (levelDelta == 0).  addToUpdate != 0: out is lduh
+ * and the face integral is accumulated into it; addToUpdate == 0: out is luh
+ * and out += dt/w * faceIntegral (as the reference glue). */
+int exa_faceIntegral(exa_ctx* ctx, double* out, const double* lFhbnd, int direction, int orientation,
+                     int levelDelta, const double* dx, double dt, int addToUpdate);
+
+/* surfaceIntegralNonlinear (generic/c/3d/surfaceIntegralNonlinear.cpph:21-90): all 2D faces. */
+int exa_surfaceIntegral(exa_ctx* ctx, double* lduh, const double* lFhbnd, const double* dx);
+
+/* addUpdateToSolution -> solutionUpdate (…Implementation.template:237-245,
+ * generic/c/3d/solutionUpdate.cpph:28-67).  luh may alias luhOld. */
+int exa_addUpdateToSolution(exa_ctx* ctx, double* luh, const double* luhOld, const double* lduh, double dt);
+
+/* stableTimeStepSize (…Implementation.template:352-361, generic/c/3d/stableTimeStepSize.cpph:14-62). */
+int exa_stableTimeStepSize(exa_ctx* ctx, const double* luh, const double* dx, double* dt_out);
+
+/* ---------------------------------------------------------------------------
+ * Batched device entry points (device pointers, cells back to back).
+ * h: per-cell edge length (cubic cells), device array [ncells]; dt is read
+ * from the device scalar d_dt so that a captured step needs no host value.
+ * ------------------------------------------------------------------------- */
+
+/* Predictor + volume integral for ncells cells (reference row: a4 + a5).
+ * d_picard (nullable): per-cell Picard count. */
+int exa_batch_predictor(exa_ctx* ctx, void* stream, long ncells, const double* d_luh, const double* d_h,
+                        const double* d_dt, double* d_lduh, double* d_lQhbnd, double* d_lFhbnd, int* d_picard);
+
+/* Rusanov flux on nfaces faces.  d_faces: int[4] per face = {left cell,
+ * right cell, direction, unused}; reads left face 2*dir+1 and right face
+ * 2*dir of lQhbnd/lFhbnd and writes the flux into both lFhbnd faces. */
+int exa_batch_riemann(exa_ctx* ctx, void* stream, long nfaces, const int* d_faces, const double* d_lQhbnd,
+                      double* d_lFhbnd);
+
+/* Face integrals of all 2D faces into lduh, then luh += dt/w * lduh, then the
+ * cell's admissible dt is min-reduced into *d_dt_next (IEEE order-independent:
+ * the result does not depend on the reduction order).  d_dt_next must hold a
+ * value >= every cell's dt (e.g. DBL_MAX) before the call. */
+int exa_batch_corrector(exa_ctx* ctx, void* stream, long ncells, double* d_luh, double* d_lduh,
+                        const double* d_lFhbnd, const double* d_h, const double* d_dt, double* d_dt_next);
+
+/* min over cells of stableTimeStepSize(luh) min-reduced into *d_dt_out. */
+int exa_batch_stable_dt(exa_ctx* ctx, void* stream, long ncells, const double* d_luh, const double* d_h,
+                        double* d_dt_out);
+
+/* ---------------------------------------------------------------------------
+ * Host mesh driver (replaces the Peano traversal for the path): a uniform
+ * Cartesian grid of n^D cubic cells of edge h, periodic, one global time
+ * step per step (ADERDGSolver.cpp step order: predictor, Riemann, surface
+ * integral + update, global dt reduction).  Device-resident patch arrays.
+ * ------------------------------------------------------------------------- */
+typedef struct exa_mesh exa_mesh;
+
+int exa_mesh_create_uniform(exa_mesh** out, exa_ctx* ctx, int n, double h);
+int exa_mesh_destroy(exa_mesh* mesh);
+long exa_mesh_ncells(const exa_mesh* mesh);
+/* Host <-> device copies of the solution [ncells][B^D][nData], cells x fastest. */
+int exa_mesh_set_solution(exa_mesh* mesh, const double* luh);
+int exa_mesh_get_solution(exa_mesh* mesh, double* luh);
+/* Device pointer of the resident solution (for zero-copy consumers). */
+double* exa_mesh_device_solution(exa_mesh* mesh);
+/* dt <= 0: compute the first dt from the current solution (stableTimeStepSize). */
+int exa_mesh_init_dt(exa_mesh* mesh, double dt);
+/* Advance nsteps global time steps on the mesh's stream.  use_graph != 0
+ * replays a captured hipGraph of one step. Asynchronous; see exa_mesh_sync. */
+int exa_mesh_run(exa_mesh* mesh, int nsteps, int use_graph);
+int exa_mesh_sync(exa_mesh* mesh);
+/* Current time, dt of the next step, per-cell Picard counts of the last step. */
+int exa_mesh_time(exa_mesh* mesh, double* t, double* dt);
+int exa_mesh_picard(exa_mesh* mesh, int* counts);
+/* The mesh's HIP stream (hipStream_t as void*). */
+void* exa_mesh_stream(exa_mesh* mesh);
+/* Per-phase timing of the last exa_mesh_run with timing enabled: milliseconds
+ * summed over the run's steps for predictor, riemann, corrector; *nsteps. */
+int exa_mesh_set_timing(exa_mesh* mesh, int enabled);
+int exa_mesh_phase_times(exa_mesh* mesh, double* ms_predictor, double* ms_riemann, double* ms_corrector,
+                         int* nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
