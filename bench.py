@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark: DoF-updates/s of the ADER-DG cell update, 3D Euler order 5.
+
+Workload (one GPU): 3D compressible Euler ADER-DG, order N=5 (6^3 nodes per
+cell), 5 variables, 81^3 periodic cells on the unit cube, smooth density wave
+plus 8 seeded random-phase Fourier modes (amplitude 1e-2, |k_i| <= 4) in rho
+and u, global time stepping (CFL 0.9), K time steps.
+
+Two forms run in the same process on the same initial state:
+  * FP64  -- every kernel in FP64, adaptive Picard (tol 1e-7), parity-tested
+             against the CPU oracle at <= 1e-12;
+  * FP32  -- space-time predictor in FP32, fixed Picard count (the FP32
+             residual cannot reach the absolute 1e-7 tolerance), Riemann,
+             corrector, solution update and CFL reduction in FP64.
+`value` is the FP32 form's DoF-updates/s; the FP64 form, the per-variable
+relative L2 error of FP32 vs FP64 and the speed-up are reported beside it.
+
+A DoF-update is one (node, variable) advanced by one time step:
+cells * (N+1)^3 * nVar * steps / wall time.  The timed region starts with the
+state resident in HBM and brackets exactly K steps with device syncs.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "exahype-engine_amd"), os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def predictor_flops_per_cell(dim, B, V, ND, iters):
+    """Algorithmic contraction flops of one predictor+volume-integral call.
+
+    Per Picard iteration: D stiffness contractions 2*V*B^(D+2) each, the iK1
+    time solve 2*V*B^(D+2), rhs0 + residual ~5*V*B^(D+1).  Once per call: time
+    averages of lQi and the D fluxes 2*B^(D+1)*(ND + D*V), extrapolation to
+    2D faces 2*2D*B^(D-1)*B*(ND + V), volume integral D*2*V*B^(D+1).
+    Flux evaluations are not counted (the reference's counting convention,
+    Miscellaneous/KernelsPerfMeasurementsTests/report/report.tex:395-408)."""
+    per_iter = dim * 2 * V * B ** (dim + 2) + 2 * V * B ** (dim + 2) + 5 * V * B ** (dim + 1)
+    once = 2 * B ** (dim + 1) * (ND + dim * V) + 2 * 2 * dim * B ** (dim - 1) * B * (ND + V) + dim * 2 * V * B ** (dim + 1)
+    return per_iter * iters + once
+
+
+def executed_iterations(counts, dim, max_iter, adaptive):
+    counts = np.asarray(counts)
+    if not adaptive:
+        return np.full(counts.shape, max_iter)
+    if dim == 3:  # 0-based index of the converged iteration, max_iter if not converged
+        return np.where(counts < max_iter, counts + 1, max_iter)
+    return np.where(counts == max_iter + 2, counts, max_iter)  # 2D convention loses the index
+
+
+def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes):
+    dim, V = 3, 5
+    ctx = E.Context(dim, order, V, 0, precision=precision, picard_max=picard)
+    mesh = E.Mesh(ctx, n, 1.0 / n)
+    prm = P.density_wave_params(dim, seed, modes)
+    # warm-up steps on the same state, then restart from the initial condition
+    mesh.adjust_solution(prm)
+    mesh.init_dt(0.0)
+    if warmup:
+        mesh.run(warmup)
+        mesh.sync()
+    mesh.adjust_solution(prm)
+    mesh.init_dt(0.0)
+    mesh.set_timing(True)  # HIP events around each kernel on the launch stream, no syncs
+    mesh.sync()
+    t0 = time.perf_counter()
+    mesh.run(steps)
+    mesh.sync()
+    wall = time.perf_counter() - t0
+    ph = mesh.phase_times()
+    pic = mesh.picard()
+    t_end, dt_next = mesh.time()
+    sol = mesh.get_solution()
+    mesh.close()
+    ctx.close()
+    B = order + 1
+    cells = n ** dim
+    adaptive = picard <= 0
+    max_iter = 2 * (order + 1) if adaptive else picard
+    iters = executed_iterations(pic, dim, max_iter, adaptive)
+    flops = float(np.sum(predictor_flops_per_cell(dim, B, V, V, iters)))
+    pred_ms = ph["predictor_ms"] / steps
+    kind = "fp64" if precision == E.PREC_FP64 else "fp32"
+    achieved = flops / (pred_ms * 1e-3) / 1e12
+    dofs = cells * B ** dim * V * steps
+    return dict(
+        kind=kind, wall_s=wall, ms_per_step=wall * 1e3 / steps, value=dofs / wall,
+        predictor_ms=pred_ms, riemann_ms=ph["riemann_ms"] / steps, corrector_ms=ph["corrector_ms"] / steps,
+        picard_hist={int(k): int(v) for k, v in zip(*np.unique(iters, return_counts=True))},
+        picard_max_executed=int(iters.max()), predictor_flops=flops,
+        roofline=dict(bound="mfma" if kind == "fp32" else "fp64-valu", achieved=achieved, peak=PEAK_TFLOPS[kind],
+                      unit="TFLOP/s", frac=achieved / PEAK_TFLOPS[kind], traffic=None),
+        t_end=t_end, dt_next=dt_next, solution=sol)
+
+
+def cpu_baseline(O, P, order, n_cpu, steps_cpu, threads):
+    luh = P.density_wave_3d(n_cpu, order)
+    orc = O.Oracle(3, order)
+    t0 = time.perf_counter()
+    orc.run_uniform(luh, n_cpu, 1.0 / n_cpu, steps_cpu, 0.0, threads=threads)
+    t = time.perf_counter() - t0
+    dofs = n_cpu ** 3 * (order + 1) ** 3 * 5 * steps_cpu
+    return dict(value=dofs / t, unit="DoF-updates/s", cores=threads, kind="port",
+                sample=f"oracle (FP64 C restatement of the reference generic kernels, OpenMP over cells) on "
+                       f"{n_cpu}^3 periodic cells, order {order}, {steps_cpu} global steps of the same workload "
+                       f"({t:.2f} s wall, {threads} threads)")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=81, help="cells per dimension")
+    ap.add_argument("--order", type=int, default=5)
+    ap.add_argument("--picard-fp32", type=int, default=0,
+                    help="fixed Picard count of the FP32 form (0: the largest count the FP64 form executed)")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--modes", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-n", type=int, default=27)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding of the path is a design note, not a runnable mode.", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: launched with WORLD_SIZE > 1; one GPU only", file=sys.stderr)
+        sys.exit(2)
+
+    import exahype_amd as E
+    from exahype_amd import problems as P
+
+    log(f"[bench] 3D Euler N={args.order} {args.n}^3 cells, steps={args.steps} warmup={args.warmup}")
+    r64 = run_form(E, P, args.n, args.order, args.steps, args.warmup, E.PREC_FP64, 0, args.seed, args.modes)
+    log(f"[bench] fp64: {r64['value']:.4e} DoF/s, {r64['ms_per_step']:.2f} ms/step "
+        f"(pred {r64['predictor_ms']:.2f} riem {r64['riemann_ms']:.2f} corr {r64['corrector_ms']:.2f}), "
+        f"picard {r64['picard_hist']}, {r64['roofline']['achieved']:.2f} TFLOP/s")
+    k32 = args.picard_fp32 if args.picard_fp32 > 0 else r64["picard_max_executed"]
+    r32 = run_form(E, P, args.n, args.order, args.steps, args.warmup, E.PREC_FP32, k32, args.seed, args.modes)
+    log(f"[bench] fp32: {r32['value']:.4e} DoF/s, {r32['ms_per_step']:.2f} ms/step "
+        f"(pred {r32['predictor_ms']:.2f} riem {r32['riemann_ms']:.2f} corr {r32['corrector_ms']:.2f}), "
+        f"picard {k32}, {r32['roofline']['achieved']:.2f} TFLOP/s")
+
+    a = r32.pop("solution").reshape(-1, 5)
+    b = r64.pop("solution").reshape(-1, 5)
+    err = [float(np.linalg.norm(a[:, v] - b[:, v]) / np.linalg.norm(b[:, v])) for v in range(5)]
+    del a, b
+    log(f"[bench] rel L2 error fp32 vs fp64 per variable: {['%.2e' % e for e in err]}")
+
+    cpu = None
+    if not args.no_cpu:
+        O = __import__("oracle")
+        try:
+            threads = len(os.sched_getaffinity(0))
+        except AttributeError:
+            threads = os.cpu_count() or 1
+        threads = max(1, min(16, threads))
+        cpu = cpu_baseline(O, P, args.order, args.cpu_n, args.cpu_steps, threads)
+        log(f"[bench] cpu baseline: {cpu['value']:.4e} DoF/s on {threads} threads")
+
+    out = {
+        "metric": "DoF-updates/s 3D Euler order 5: FP32-MFMA vs FP64 at <=1e-6 rel L2 error",
+        "value": r32["value"],
+        "unit": "DoF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D compressible Euler ADER-DG order {args.order} ({args.order + 1}^3 nodes), 5 vars, "
+                        f"{args.n}^3 periodic cells, density wave + {args.modes} seeded Fourier modes "
+                        f"(amp 1e-2, |k|<=4) in rho and u, CFL 0.9, {args.steps} global steps; "
+                        f"FP32 predictor (fixed {k32} Picard iterations), FP64 Riemann/corrector/update/CFL",
+            "cells": args.n ** 3, "order": args.order, "picard_fp32": k32, "seed": args.seed,
+        },
+        "roofline": r32["roofline"],
+        "cpu_baseline": cpu,
+        "fp64": {k: r64[k] for k in ("value", "ms_per_step", "predictor_ms", "riemann_ms", "corrector_ms",
+                                     "picard_hist", "roofline")},
+        "fp32_phases_ms": {k: r32[k] for k in ("predictor_ms", "riemann_ms", "corrector_ms")},
+        "speedup_fp32_vs_fp64": r32["value"] / r64["value"],
+        "rel_l2_err_fp32_vs_fp64": err,
+        "max_rel_l2_err": max(err),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -53,6 +53,9 @@ int launch_solution_update(const KernelCfg& k, hipStream_t s, double* luh, const
                            const double* lduh, const double* dt);
 // Time bookkeeping of the device step: t += dt; dt = dt_next; dt_next = DBL_MAX.
 int launch_advance_time(hipStream_t s, double* state);
+// Initial condition EXA_IC_DENSITY_WAVE on a uniform n^D mesh (initial.hip).
+int launch_density_wave(hipStream_t s, long ncells, int n, int D, int B, int ND, double h, const double* d_xi,
+                        const double* d_prm, double* luh);
 int launch_fill(hipStream_t s, double* p, long n, double value);
 
 void set_error(const std::string& msg);

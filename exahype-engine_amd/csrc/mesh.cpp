@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <cstddef>
 #include <string>
 #include <vector>
 
@@ -37,8 +38,7 @@ struct exa_mesh {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool timing = false;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  double ms_pred = 0, ms_riem = 0, ms_corr = 0;
+  std::vector<hipEvent_t> ev;  // 4 events per timed step: before/after each phase
   int timed_steps = 0;
 };
 
@@ -65,32 +65,32 @@ int status(int rc, const char* what) {
 }
 
 // Enqueue one global time step on m->stream.
-int enqueue_step(exa_mesh* m, bool with_events) {
+int enqueue_step(exa_mesh* m, hipEvent_t* ev) {
   double* st = m->d_state;
-  if (with_events) HIP_OK(hipEventRecord(m->ev[0], m->stream), "event");
+  const bool with_events = ev != nullptr;
+  if (with_events) HIP_OK(hipEventRecord(ev[0], m->stream), "event");
   int rc = status(exa::launch_predictor(m->k, m->stream, m->ncells, m->d_luh, m->d_h, st + 1, m->d_lduh, m->d_qb,
                                         m->d_fb, m->d_picard),
                   "predictor");
   if (rc) return rc;
-  if (with_events) HIP_OK(hipEventRecord(m->ev[1], m->stream), "event");
+  if (with_events) HIP_OK(hipEventRecord(ev[1], m->stream), "event");
   rc = status(exa::launch_riemann(m->k, m->stream, m->nfaces, m->d_faces, m->d_qb, m->d_fb), "riemann");
   if (rc) return rc;
-  if (with_events) HIP_OK(hipEventRecord(m->ev[2], m->stream), "event");
+  if (with_events) HIP_OK(hipEventRecord(ev[2], m->stream), "event");
   rc = status(exa::launch_corrector(m->k, m->stream, m->ncells, m->d_luh, m->d_lduh, m->d_fb, m->d_h, st + 1,
                                     st + 2),
               "corrector");
   if (rc) return rc;
   rc = status(exa::launch_advance_time(m->stream, st), "advance time");
   if (rc) return rc;
-  if (with_events) HIP_OK(hipEventRecord(m->ev[3], m->stream), "event");
+  if (with_events) HIP_OK(hipEventRecord(ev[3], m->stream), "event");
   return EXA_OK;
 }
 
 void free_mesh(exa_mesh* m) {
   if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
   if (m->graph) (void)hipGraphDestroy(m->graph);
-  for (auto& e : m->ev)
-    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
   double* bufs[] = {m->d_luh, m->d_lduh, m->d_qb, m->d_fb, m->d_h, m->d_state};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
@@ -131,7 +131,6 @@ int exa_mesh_create_uniform(exa_mesh** out, exa_ctx* ctx, int n, double h) {
   if (e == hipSuccess) e = hipMalloc(&m->d_state, sizeof(double) * 4);
   if (e == hipSuccess) e = hipMalloc(&m->d_faces, sizeof(int) * 4 * m->nfaces);
   if (e == hipSuccess) e = hipMalloc(&m->d_picard, sizeof(int) * m->ncells);
-  for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&m->ev[i]);
   if (e != hipSuccess) {
     free_mesh(m);
     return fail(e, "exa_mesh_create_uniform");
@@ -201,6 +200,38 @@ int exa_mesh_get_solution(exa_mesh* m, double* luh) {
 
 double* exa_mesh_device_solution(exa_mesh* m) { return m ? m->d_luh : nullptr; }
 
+int exa_mesh_adjust_solution(exa_mesh* m, int ic, const double* params, int nparams) {
+  if (!m || !params || nparams < 7) {
+    exa::set_error("exa_mesh_adjust_solution: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  if (ic != EXA_IC_DENSITY_WAVE || m->k.pde.id != exa::kPdeEuler || m->k.nvar != 5) {
+    exa::set_error("exa_mesh_adjust_solution: unknown initial condition for this PDE");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  const int M = static_cast<int>(params[5]);
+  if (M < 0 || nparams < 7 + 8 * M) {
+    exa::set_error("exa_mesh_adjust_solution: params too short for the number of modes");
+    return EXA_ERR_ARG;
+  }
+  double* d_prm = nullptr;
+  HIP_OK(hipMalloc(&d_prm, sizeof(double) * nparams), "hipMalloc(params)");
+  hipError_t e = hipMemcpyAsync(d_prm, params, sizeof(double) * nparams, hipMemcpyHostToDevice, m->stream);
+  const double* d_xi = reinterpret_cast<const double*>(reinterpret_cast<const char*>(m->k.tab) +
+                                                        offsetof(exa::DevTables, x));
+  int rc = EXA_OK;
+  if (e == hipSuccess) {
+    rc = status(exa::launch_density_wave(m->stream, m->ncells, m->n, m->k.dim, m->k.order + 1, m->ND, m->h, d_xi,
+                                         d_prm, m->d_luh),
+                "density wave");
+    e = hipStreamSynchronize(m->stream);
+  }
+  (void)hipFree(d_prm);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(e, "exa_mesh_adjust_solution");
+  return EXA_OK;
+}
+
 int exa_mesh_init_dt(exa_mesh* m, double dt) {
   if (!m) {
     exa::set_error("exa_mesh_init_dt: NULL mesh");
@@ -223,27 +254,23 @@ int exa_mesh_run(exa_mesh* m, int nsteps, int use_graph) {
     return EXA_ERR_ARG;
   }
   if (m->timing) {
-    m->ms_pred = m->ms_riem = m->ms_corr = 0;
-    m->timed_steps = 0;
+    // events are recorded on the launch stream; nothing synchronises here
+    while (m->ev.size() < size_t(4) * nsteps) {
+      hipEvent_t e = nullptr;
+      HIP_OK(hipEventCreate(&e), "event create");
+      m->ev.push_back(e);
+    }
+    m->timed_steps = nsteps;
     for (int i = 0; i < nsteps; i++) {
-      int rc = enqueue_step(m, true);
+      const int rc = enqueue_step(m, &m->ev[size_t(4) * i]);
       if (rc) return rc;
-      HIP_OK(hipEventSynchronize(m->ev[3]), "event sync");
-      float a = 0, b = 0, c = 0;
-      HIP_OK(hipEventElapsedTime(&a, m->ev[0], m->ev[1]), "elapsed");
-      HIP_OK(hipEventElapsedTime(&b, m->ev[1], m->ev[2]), "elapsed");
-      HIP_OK(hipEventElapsedTime(&c, m->ev[2], m->ev[3]), "elapsed");
-      m->ms_pred += a;
-      m->ms_riem += b;
-      m->ms_corr += c;
-      m->timed_steps++;
     }
     return EXA_OK;
   }
   if (use_graph) {
     if (!m->graph_exec) {
       HIP_OK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-      const int rc = enqueue_step(m, false);
+      const int rc = enqueue_step(m, nullptr);
       hipGraph_t g = nullptr;
       const hipError_t e = hipStreamEndCapture(m->stream, &g);
       if (rc) {
@@ -258,7 +285,7 @@ int exa_mesh_run(exa_mesh* m, int nsteps, int use_graph) {
     return EXA_OK;
   }
   for (int i = 0; i < nsteps; i++) {
-    const int rc = enqueue_step(m, false);
+    const int rc = enqueue_step(m, nullptr);
     if (rc) return rc;
   }
   return EXA_OK;
@@ -297,9 +324,18 @@ int exa_mesh_set_timing(exa_mesh* m, int enabled) {
 
 int exa_mesh_phase_times(exa_mesh* m, double* a, double* b, double* c, int* n) {
   if (!m) return EXA_ERR_ARG;
-  if (a) *a = m->ms_pred;
-  if (b) *b = m->ms_riem;
-  if (c) *c = m->ms_corr;
+  double ms[3] = {0, 0, 0};
+  if (m->timed_steps > 0) HIP_OK(hipEventSynchronize(m->ev[size_t(4) * m->timed_steps - 1]), "event sync");
+  for (int i = 0; i < m->timed_steps; i++) {
+    for (int p = 0; p < 3; p++) {
+      float x = 0;
+      HIP_OK(hipEventElapsedTime(&x, m->ev[size_t(4) * i + p], m->ev[size_t(4) * i + p + 1]), "elapsed");
+      ms[p] += x;
+    }
+  }
+  if (a) *a = ms[0];
+  if (b) *b = ms[1];
+  if (c) *c = ms[2];
   if (n) *n = m->timed_steps;
   return EXA_OK;
 }

@@ -73,6 +73,8 @@ _sig("exa_mesh_ncells", [_cp], ctypes.c_long)
 _sig("exa_mesh_set_solution", [_cp, _dp])
 _sig("exa_mesh_get_solution", [_cp, _dp])
 _sig("exa_mesh_device_solution", [_cp], ctypes.c_void_p)
+_sig("exa_mesh_adjust_solution", [_cp, ctypes.c_int, _dp, ctypes.c_int])
+IC_DENSITY_WAVE = 1
 _sig("exa_mesh_init_dt", [_cp, ctypes.c_double])
 _sig("exa_mesh_run", [_cp, ctypes.c_int, ctypes.c_int])
 _sig("exa_mesh_sync", [_cp])
@@ -243,6 +245,11 @@ class Mesh:
         out = np.zeros(self.ncells * self.ctx.nspace * self.ctx.nd)
         _check(_lib.exa_mesh_get_solution(self.h, _p(out)))
         return out
+
+    def adjust_solution(self, params, ic=IC_DENSITY_WAVE):
+        """Evaluate a compiled initial condition on the device (adjustSolution at t = 0)."""
+        prm = np.ascontiguousarray(params, dtype=np.float64)
+        _check(_lib.exa_mesh_adjust_solution(self.h, ic, _p(prm), prm.size))
 
     def init_dt(self, dt=0.0):
         _check(_lib.exa_mesh_init_dt(self.h, dt))

@@ -94,5 +94,18 @@ def density_wave_3d(n, order, seed=0, modes=8, xp=np, device=None):
     return Q.reshape(-1)
 
 
+def density_wave_params(dim, seed=0, modes=8):
+    """Parameter vector of the device initial condition EXA_IC_DENSITY_WAVE that
+    reproduces density_wave_2d (dim 2) / density_wave_3d (dim 3)."""
+    if dim == 2:
+        return np.array([0.2, 1.0, 1.0, 0.0, 1.0, 0, 0.0])
+    (kr, pr), (ku, pu) = fourier_modes(seed, modes)
+    prm = [0.2, 1.0, 1.0, 1.0, 1.0, modes, 0.01 / modes]
+    for k, ph in ((kr, pr), (ku, pu)):
+        for m in range(modes):
+            prm += [k[m, 0], k[m, 1], k[m, 2], ph[m]]
+    return np.array(prm, dtype=np.float64)
+
+
 def dof_per_cell(dim, order, nvar=5):
     return (order + 1) ** dim * nvar

@@ -172,6 +172,15 @@ int exa_mesh_set_solution(exa_mesh* mesh, const double* luh);
 int exa_mesh_get_solution(exa_mesh* mesh, double* luh);
 /* Device pointer of the resident solution (for zero-copy consumers). */
 double* exa_mesh_device_solution(exa_mesh* mesh);
+
+/* Initial conditions evaluated on the device at every cell's nodes, the
+ * reference's adjustSolution at t = 0 (generic/c/3d/solutionAdjustment.cpph:18-63).
+ * EXA_IC_DENSITY_WAVE (Euler): params = {a0, u0, v0, w0, p0, M, amp,
+ *   M x (kx, ky, kz, phase) for rho, M x (kx, ky, kz, phase) for u};
+ *   rho = 1 + a0 sin(2 pi (x+y[+z])) + amp sum cos(2 pi k.x + phase),
+ *   u = u0 + amp sum cos(...), v = v0, w = w0 (0 in 2D), p = p0. */
+enum exa_initial_condition { EXA_IC_DENSITY_WAVE = 1 };
+int exa_mesh_adjust_solution(exa_mesh* mesh, int ic, const double* params, int nparams);
 /* dt <= 0: compute the first dt from the current solution (stableTimeStepSize). */
 int exa_mesh_init_dt(exa_mesh* mesh, double dt);
 /* Advance nsteps global time steps on the mesh's stream.  use_graph != 0

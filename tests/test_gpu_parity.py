@@ -301,3 +301,16 @@ def test_conservation_and_determinism_full_size():
     assert abs(mass(results[0]) / m0 - 1) < 1e-12
     np.testing.assert_array_equal(results[0], results[1])
     ctx.close()
+
+
+@pytest.mark.parametrize("dim,n,order", [(3, 3, 5), (3, 4, 3), (2, 9, 3)])
+def test_device_initial_condition_matches_host(dim, n, order):
+    """exa_mesh_adjust_solution (device) reproduces the host generator."""
+    ctx = E.Context(dim, order)
+    mesh = E.Mesh(ctx, n, 1.0 / n)
+    mesh.adjust_solution(P.density_wave_params(dim))
+    got = mesh.get_solution()
+    ref = P.density_wave_3d(n, order) if dim == 3 else P.density_wave_2d(n, order)
+    mesh.close()
+    ctx.close()
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-14)
