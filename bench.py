@@ -63,9 +63,10 @@ def executed_iterations(counts, dim, max_iter, adaptive):
     return np.where(counts == max_iter + 2, counts, max_iter)  # 2D convention loses the index
 
 
-def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes):
+def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, variant=0):
     dim, V = 3, 5
     ctx = E.Context(dim, order, V, 0, precision=precision, picard_max=picard)
+    ctx.set_predictor_variant(variant)
     mesh = E.Mesh(ctx, n, 1.0 / n)
     prm = P.density_wave_params(dim, seed, modes)
     # warm-up steps on the same state, then restart from the initial condition
@@ -135,6 +136,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-n", type=int, default=27)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--sweep-variants", action="store_true",
+                    help="only time the predictor kernel variants (node-column, line, line+MFMA) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -149,6 +152,15 @@ def main():
     from exahype_amd import problems as P
 
     log(f"[bench] 3D Euler N={args.order} {args.n}^3 cells, steps={args.steps} warmup={args.warmup}")
+    if args.sweep_variants:
+        for prec, name, pic in ((E.PREC_FP64, "fp64", 0), (E.PREC_FP32, "fp32", 3)):
+            for variant, vname in ((1, "node-column"), (2, "line"), (3, "line+mfma")):
+                if prec == E.PREC_FP64 and variant == 3:
+                    continue
+                r = run_form(E, P, args.n, args.order, args.steps, args.warmup, prec, pic, args.seed, args.modes, variant)
+                log(f"[sweep] {name} {vname:12s} predictor {r['predictor_ms']:.2f} ms/step "
+                    f"({r['roofline']['achieved']:.2f} TFLOP/s)")
+        return
     r64 = run_form(E, P, args.n, args.order, args.steps, args.warmup, E.PREC_FP64, 0, args.seed, args.modes)
     log(f"[bench] fp64: {r64['value']:.4e} DoF/s, {r64['ms_per_step']:.2f} ms/step "
         f"(pred {r64['predictor_ms']:.2f} riem {r64['riemann_ms']:.2f} corr {r64['corrector_ms']:.2f}), "

@@ -654,6 +654,13 @@ int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* lu
   const Entry* e = find_entry(k);
   if (!e) return -1;
   if (n <= 0) return 0;
+  // auto: FP32 -> line kernel on the matrix cores; FP64 -> node-column kernel
+  if (lines_supported(k)) {
+    const int v = k.pred_variant;
+    const bool fp32 = k.precision != 0;
+    if ((v == 0 && fp32) || v == 3) return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, fp32);
+    if (v == 2) return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, false);
+  }
   return e->predictor(k, s, n, luh, h, dt, lduh, qb, fb, pic);
 }
 

@@ -147,6 +147,9 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
     h.dudx[i] = c->tab.dudx[i];
     h.Kxif[i] = static_cast<float>(c->tab.Kxi[i]);
     h.iK1f[i] = static_cast<float>(c->tab.iK1[i]);
+    h.dudxf[i] = static_cast<float>(c->tab.dudx[i]);
+    h.Pt[i] = c->tab.iK1[i] * c->tab.w[i % B];
+    h.Ptf[i] = static_cast<float>(h.Pt[i]);
   }
   hipError_t e = hipMalloc(&c->d_tab, sizeof(exa::DevTables));
   if (e == hipSuccess) e = hipMemcpy(c->d_tab, &h, sizeof(h), hipMemcpyHostToDevice);
@@ -177,6 +180,15 @@ int exa_ctx_set_advection(exa_ctx* c, const double* a) {
     return EXA_ERR_ARG;
   }
   for (int d = 0; d < c->k.dim; d++) c->k.pde.a[d] = a[d];
+  return EXA_OK;
+}
+
+int exa_ctx_set_predictor_variant(exa_ctx* c, int variant) {
+  if (!c || variant < 0 || variant > 3) {
+    set_error("exa_ctx_set_predictor_variant: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  c->k.pred_variant = variant;
   return EXA_OK;
 }
 

@@ -25,6 +25,10 @@ struct DevTables {
   float F0f[kMaxBasis];
   float Kxif[kMaxBasis * kMaxBasis];
   float iK1f[kMaxBasis * kMaxBasis];
+  // derived operators of the line-decomposed predictor
+  float dudxf[kMaxBasis * kMaxBasis];
+  double Pt[kMaxBasis * kMaxBasis];  // Pt[i][j] = iK1[i][j] * w[j]
+  float Ptf[kMaxBasis * kMaxBasis];
 };
 
 struct KernelCfg {
@@ -34,6 +38,7 @@ struct KernelCfg {
   double cfl, pnpm;
   PdeParams pde;
   const DevTables* tab;  // device pointer
+  int pred_variant;      // 0 auto, 1 node-column, 2 line (VALU), 3 line (FP32 matrix cores)
 };
 
 // Launchers (return hipError_t as int; 0 on success, -1 for an unsupported
@@ -51,6 +56,11 @@ int launch_face_integral(const KernelCfg& k, hipStream_t s, double* lduh, const 
                          int orientation, double invdx_scaled);
 int launch_solution_update(const KernelCfg& k, hipStream_t s, double* luh, const double* luhOld,
                            const double* lduh, const double* dt);
+// Line-decomposed 3D predictor (predictor3d_lines.hip) for the shapes it covers.
+bool lines_supported(const KernelCfg& k);
+int launch_predictor3d_lines(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
+                             const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard,
+                             bool mfma);
 // Time bookkeeping of the device step: t += dt; dt = dt_next; dt_next = DBL_MAX.
 int launch_advance_time(hipStream_t s, double* state);
 // Initial condition EXA_IC_DENSITY_WAVE on a uniform n^D mesh (initial.hip).

@@ -1,0 +1,476 @@
+// 3D nonlinear space-time predictor + volume integral, "line" decomposition.
+//
+// Same mathematics as the reference Picard loop
+// (ExaHyPE/kernels/aderdg/generic/c/3d/spaceTimePredictorNonlinear.cpph:39-433), written as
+//   G[t][s]    = sum_d sum_n dudx[s_d][n] F_d(lQi[t][s|s_d=n])        (flux divergence)
+//   lQi'[t'][s]= luh[s] - dt/h * sum_t P[t'][t] G[t][s],  P = iK1 * diag(w)
+// which is the reference update lQi' = W^-1 iK1 (W F0 luh - stiffness terms):
+// Kxi[n][l] / w_l = dudx[l][n] and iK1 * F0 = 1 (rows of iK1 sum the F0
+// weights to one), so the two agree up to rounding.
+//
+// Decomposition (one workgroup = CPB cells, B^3 lanes per cell):
+//   lane (t, z, y) owns the x-line {(t, z, y, x) : x < B}: lQi of its B
+//   points (B*V values) lives in registers.  Per Picard iteration:
+//     x  : flux + x contraction entirely in registers;
+//     y  : F_y is written to the cell's LDS tile, each lane reads one y-line,
+//          contracts it in registers and writes the result back onto the
+//          same line (lines are disjoint: no barrier between read and write),
+//          the owners read their x-lines back;
+//     z  : as y with z-lines;
+//     t  : G is written to LDS, each lane reads one time line (all t of one
+//          point), applies P and writes lQi' back onto the same line.
+//   LDS traffic: 12 values per space-time point and variable per iteration,
+//   no re-reads of whole lines per output point.
+//   On convergence the time averages lQhi = sum_t w_t lQi and
+//   lFhi_d = sum_t w_t F_d(lQi of the previous iterate) (the reference
+//   averages the flux of the last iteration, :441-515) are reduced over the
+//   B time lanes through LDS; extrapolation (:517-687) and the volume integral
+//   (volumeIntegralNonlinear.cpph:133-223) follow as in the reference.
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+
+namespace exa {
+namespace {
+
+constexpr int lceil64(int n) { return (n + 63) / 64 * 64; }
+
+template <int B>
+struct LShape {
+  static constexpr int V = 5;
+  static constexpr int NS = B * B * B;   // spatial nodes = lanes per cell
+  static constexpr int NST = NS * B;     // space-time points per cell
+  static constexpr int NF = B * B;
+  static constexpr int CPB = NS >= 128 ? 1 : 256 / NS;
+  static constexpr int THREADS = lceil64(CPB * NS);
+  static constexpr int TILE = NST * V;   // LDS values per cell
+};
+
+// Hides a uniform pointer from loop-invariant code motion so that operator
+// entries are re-read through the scalar cache in each phase instead of being
+// hoisted into (vector) registers for the whole kernel.
+template <typename X>
+__device__ __forceinline__ const X* opaque(const X* p) {
+  unsigned long long a = reinterpret_cast<unsigned long long>(p);
+  asm volatile("" : "+s"(a));
+  // re-establish uniformity so that the loads stay scalar (s_load)
+  const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(a));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(a >> 32));
+  return reinterpret_cast<const X*>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// Makes register values opaque to the optimiser at a phase boundary.  Without
+// it the flux products irho*Q[k] of the x, y, z and averaging phases are
+// common subexpressions and stay live across the whole Picard iteration
+// (~90 extra registers); recomputing them costs a few multiplies.
+template <typename R, int N, int M>
+__device__ __forceinline__ void launder(R (&a)[N][M]) {
+#pragma unroll
+  for (int i = 0; i < N; i++)
+#pragma unroll
+    for (int j = 0; j < M; j++) asm volatile("" : "+v"(a[i][j]));
+}
+
+typedef float exa_f4 __attribute__((ext_vector_type(4)));
+
+// out[i] = sum_n op[i][n] * L[n], i < B <= 8, on the matrix cores with
+// v_mfma_f32_4x4x1_16b_f32: block b = lane / 4 computes a 4x4 outer product
+// A_b (x) B_b with A_b[i] from lane 4b+i and B_b[j] from lane 4b+j and
+// leaves D_b[i][j] in accumulator register i of lane 4b+j.  Every lane puts
+// its own line value L[n] into B (column j = its own line) and the operator
+// entry op[4h + lane%4][n] into A, so after the B rank-one steps lane 4b+j
+// holds rows 4h..4h+3 of its own output line.  aop[h][n] = op[4h+lane%4][n]
+// (zero for rows >= B).  Must be executed by all 64 lanes of the wave.
+template <int B>
+__device__ __forceinline__ void mfma_line(const float (&aop)[2][B], const float (&L)[B], float (&out)[B]) {
+  exa_f4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int n = 0; n < B; n++) {
+    c0 = __builtin_amdgcn_mfma_f32_4x4x1f32(aop[0][n], L[n], c0, 0, 0, 0);
+    if (B > 4) c1 = __builtin_amdgcn_mfma_f32_4x4x1f32(aop[1][n], L[n], c1, 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < B; i++) out[i] = i < 4 ? c0[i] : c1[i - 4];
+}
+
+template <typename R>
+__device__ __forceinline__ const R* tab_dudx(const DevTables* T);
+template <>
+__device__ __forceinline__ const double* tab_dudx<double>(const DevTables* T) { return T->dudx; }
+template <>
+__device__ __forceinline__ const float* tab_dudx<float>(const DevTables* T) { return T->dudxf; }
+template <typename R>
+__device__ __forceinline__ const R* tab_P(const DevTables* T);
+template <>
+__device__ __forceinline__ const double* tab_P<double>(const DevTables* T) { return T->Pt; }
+template <>
+__device__ __forceinline__ const float* tab_P<float>(const DevTables* T) { return T->Ptf; }
+
+// Euler pressure and 1/rho at one point (EulerSolver_ADERDG.cpp:78-86).
+template <typename R>
+__device__ __forceinline__ void euler_prim(const R* Q, R& irho, R& p) {
+  irho = R(1) / Q[0];
+  const R j2 = Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3];
+  p = (R(1.4) - R(1)) * (Q[4] - R(0.5) * irho * j2);
+}
+
+// Flux component d of the Euler flux (EulerSolver_ADERDG.cpp:88-113).
+template <typename R>
+__device__ __forceinline__ void euler_flux_dir(const R* Q, R irho, R p, int d, R* F) {
+  const R qd = Q[d + 1];
+  F[0] = qd;
+  F[1] = irho * Q[1] * qd + (d == 0 ? p : R(0));
+  F[2] = irho * Q[2] * qd + (d == 1 ? p : R(0));
+  F[3] = irho * Q[3] * qd + (d == 2 ? p : R(0));
+  F[4] = irho * (Q[4] + p) * qd;
+}
+
+template <int B, typename R, bool MF>
+__global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel(
+    long ncells, const double* __restrict__ luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
+    double* __restrict__ lduh, double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive) {
+  using S = LShape<B>;
+  constexpr int V = S::V, NS = S::NS, NF = S::NF, TILE = S::TILE;
+  static_assert(S::CPB == 1, "one cell per workgroup");
+  static_assert(!MF || (sizeof(R) == 4 && B <= 8), "matrix-core form: FP32, B <= 8");
+  constexpr int NWAVE = S::THREADS / 64;
+  constexpr int TILE_BYTES = TILE * int(sizeof(R));
+  constexpr int AVG_BYTES = 4 * NS * V * 8;  // lQhi + lFhi_{x,y,z} in FP64
+  constexpr int BUF_BYTES = TILE_BYTES > AVG_BYTES ? TILE_BYTES : AVG_BYTES;
+  __shared__ __align__(16) unsigned char buf[BUF_BYTES];
+  __shared__ double red[NWAVE];
+  R* tl = reinterpret_cast<R*>(buf);
+
+  const int l = threadIdx.x;              // (t, z, y) for l < NS
+  const bool lane_on = l < NS;            // lanes NS..THREADS-1 only join barriers
+  const long cell = blockIdx.x;
+  const int t = l / (B * B), z = (l / B) % B, y = l % B;
+
+  const double dt = *dtp;
+  const double h = hcell[cell];
+  const double* w = T->w;
+  // operators (uniform across lanes; read through the scalar cache at use):
+  //   Dm[i][j] = dudx[i][j] (derivative), Pm[i][j] = iK1[i][j] * w[j]
+  const R* Dm = tab_dudx<R>(T);
+  const R* Pm = tab_P<R>(T);
+  const R cdt = static_cast<R>(dt / h);
+  // matrix-core operand columns: aD[h][n] = dudx[4h + l%4][n], aP[h][n] = P[4h + l%4][n]
+  float aD[2][B], aP[2][B];
+  if constexpr (MF) {
+    const int r = l & 3;
+#pragma unroll
+    for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+      for (int n = 0; n < B; n++) {
+        const int row = 4 * hh + r;
+        aD[hh][n] = row < B ? T->dudxf[row * B + n] : 0.f;
+        aP[hh][n] = row < B ? T->Ptf[row * B + n] : 0.f;
+      }
+  }
+
+  // time-line role: this lane solves in time for point (z, y, x = t)
+  R u[V];
+  R q[B][V];  // lQi at (t, z, y, x), x < B
+  {
+    const long base = (cell * NS + ((lane_on ? z : 0) * B + (lane_on ? y : 0)) * B) * V;
+#pragma unroll
+    for (int x = 0; x < B; x++)
+#pragma unroll
+      for (int v = 0; v < V; v++) q[x][v] = static_cast<R>(luh[base + x * V + v]);
+#pragma unroll
+    for (int v = 0; v < V; v++) u[v] = static_cast<R>(luh[base + (lane_on ? t : 0) * V + v]);
+  }
+  const int own = ((t * B + z) * B + y) * B * V;          // this lane's x-line in the tile
+  const int ybase = (t * B + z) * B * B * V + y * V;      // y-line (t, z, *, x = y)
+  const int zbase = (t * B * B * B + y * B + z) * V;      // z-line (t, *, y, x = z)
+  const int tbase = ((z * B + y) * B + t) * V;            // time line of (z, y, x = t)
+
+  int iters = maxIter;
+  for (int iter = 0; iter < maxIter; iter++) {
+    R G[B][V];
+    Dm = opaque(Dm);
+    launder(q);
+    // ---- x: flux + contraction in registers (streamed over the line) ----
+#pragma unroll
+    for (int i = 0; i < B; i++)
+#pragma unroll
+      for (int v = 0; v < V; v++) G[i][v] = R(0);
+    if constexpr (MF) {
+      float F[B][V];
+#pragma unroll
+      for (int n = 0; n < B; n++) {
+        R ir, pr;
+        euler_prim<R>(q[n], ir, pr);
+        euler_flux_dir<R>(q[n], ir, pr, 0, F[n]);
+      }
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        float L[B], o[B];
+#pragma unroll
+        for (int n = 0; n < B; n++) L[n] = F[n][v];
+        mfma_line<B>(aD, L, o);
+#pragma unroll
+        for (int i = 0; i < B; i++) G[i][v] = o[i];
+      }
+    } else {
+#pragma unroll
+      for (int n = 0; n < B; n++) {
+        R ir, pr, F[V];
+        euler_prim<R>(q[n], ir, pr);
+        euler_flux_dir<R>(q[n], ir, pr, 0, F);
+#pragma unroll
+        for (int i = 0; i < B; i++)
+#pragma unroll
+          for (int v = 0; v < V; v++) G[i][v] += Dm[i * B + n] * F[v];
+      }
+    }
+    // ---- y, z: line transposes through the tile ----
+#pragma unroll
+    for (int d = 1; d <= 2; d++) {
+      Dm = opaque(Dm);
+      launder(q);
+      if (lane_on) {
+#pragma unroll
+        for (int x = 0; x < B; x++) {
+          R ir, pr, F[V];
+          euler_prim<R>(q[x], ir, pr);
+          euler_flux_dir<R>(q[x], ir, pr, d, F);
+#pragma unroll
+          for (int v = 0; v < V; v++) tl[own + x * V + v] = F[v];
+        }
+      }
+      __syncthreads();
+      if constexpr (MF) {
+        const int lb = (d == 1) ? ybase : zbase;
+        const int ls = (d == 1) ? B * V : B * B * V;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          float L[B], o[B];
+#pragma unroll
+          for (int n = 0; n < B; n++) L[n] = lane_on ? tl[lb + n * ls + v] : 0.f;
+          mfma_line<B>(aD, L, o);
+          if (lane_on) {
+#pragma unroll
+            for (int i = 0; i < B; i++) tl[lb + i * ls + v] = o[i];
+          }
+        }
+      } else if (lane_on) {
+        const int lb = (d == 1) ? ybase : zbase;
+        const int ls = (d == 1) ? B * V : B * B * V;
+        // one variable at a time; each lane writes back onto its own line only
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          R L[B];
+#pragma unroll
+          for (int n = 0; n < B; n++) L[n] = tl[lb + n * ls + v];
+#pragma unroll
+          for (int i = 0; i < B; i++) {
+            R a = R(0);
+#pragma unroll
+            for (int n = 0; n < B; n++) a += Dm[i * B + n] * L[n];
+            tl[lb + i * ls + v] = a;
+          }
+        }
+      }
+      __syncthreads();
+      if (lane_on) {
+#pragma unroll
+        for (int x = 0; x < B; x++)
+#pragma unroll
+          for (int v = 0; v < V; v++) G[x][v] += tl[own + x * V + v];
+      }
+      __syncthreads();
+    }
+    // ---- t: time solve through the tile ----
+    Pm = opaque(Pm);
+    if (lane_on) {
+#pragma unroll
+      for (int x = 0; x < B; x++)
+#pragma unroll
+        for (int v = 0; v < V; v++) tl[own + x * V + v] = G[x][v];
+    }
+    __syncthreads();
+    if constexpr (MF) {
+      constexpr int ts = NS * V;
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        float L[B], o[B];
+#pragma unroll
+        for (int n = 0; n < B; n++) L[n] = lane_on ? tl[tbase + n * ts + v] : 0.f;
+        mfma_line<B>(aP, L, o);
+        if (lane_on) {
+#pragma unroll
+          for (int i = 0; i < B; i++) tl[tbase + i * ts + v] = u[v] - cdt * o[i];
+        }
+      }
+    } else if (lane_on) {
+      constexpr int ts = NS * V;
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        R L[B];
+#pragma unroll
+        for (int n = 0; n < B; n++) L[n] = tl[tbase + n * ts + v];
+#pragma unroll
+        for (int i = 0; i < B; i++) {
+          R a = R(0);
+#pragma unroll
+          for (int n = 0; n < B; n++) a += Pm[i * B + n] * L[n];
+          tl[tbase + i * ts + v] = u[v] - cdt * a;
+        }
+      }
+    }
+    __syncthreads();
+    double sq = 0.0;
+    if (lane_on) {
+#pragma unroll
+      for (int x = 0; x < B; x++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          const double dq = static_cast<double>(tl[own + x * V + v]) - static_cast<double>(q[x][v]);
+          sq += dq * dq;
+        }
+    }
+    bool last = (iter == maxIter - 1);
+    if (adaptive) {
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) sq += __shfl_xor(sq, m, 64);
+      if ((l & 63) == 0) red[l >> 6] = sq;
+      __syncthreads();
+      double res = 0.0;
+#pragma unroll
+      for (int i = 0; i < NWAVE; i++) res += red[i];
+      if (res < 1e-7 * 1e-7) {  // block-uniform decision (one cell per block)
+        iters = iter;
+        last = true;
+      }
+    }
+    if (last) break;
+    if (lane_on) {
+#pragma unroll
+      for (int x = 0; x < B; x++)
+#pragma unroll
+        for (int v = 0; v < V; v++) q[x][v] = tl[own + x * V + v];
+    }
+    __syncthreads();
+  }
+
+  // ---- after the Picard loop: q = previous iterate, the tile holds the final one ----
+    // ---- time averages: lQhi = sum_t w_t lQi (final iterate),
+    //      lFhi_d = sum_t w_t F_d(previous iterate) ----
+    // component 0 (lQhi) is reduced straight from the tile, which holds the
+    // final iterate on every x-line; components 1..3 overwrite the tile with
+    // F_d of the previous iterate.
+    R acc[4][V];  // reduced values of point (z, y, x = t)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      launder(q);
+      __syncthreads();
+      if (lane_on && c > 0) {
+#pragma unroll
+        for (int x = 0; x < B; x++) {
+          R F[V];
+          R ir, pr;
+          euler_prim<R>(q[x], ir, pr);
+          euler_flux_dir<R>(q[x], ir, pr, c - 1, F);
+#pragma unroll
+          for (int v = 0; v < V; v++) tl[own + x * V + v] = F[v];
+        }
+      }
+      __syncthreads();
+      if (lane_on) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          R a = R(0);
+#pragma unroll
+          for (int n = 0; n < B; n++) a += tl[tbase + n * NS * V + v] * static_cast<R>(w[n]);
+          acc[c][v] = a;
+        }
+      }
+    }
+    __syncthreads();
+    double* Qh = reinterpret_cast<double*>(buf);  // [NS][V]
+    double* Fh = Qh + NS * V;                     // [3][NS][V]
+    const int s = (z * B + y) * B + t;
+    if (lane_on) {
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        Qh[s * V + v] = static_cast<double>(acc[0][v]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) Fh[(d * NS + s) * V + v] = static_cast<double>(acc[d + 1][v]);
+      }
+    }
+    __syncthreads();
+    if (lane_on) {
+      // volume integral at point s = (z, y, x = t)
+      const int sz = z, sy = y, sx = t;
+      const double invDx = 1.0 / h;
+      double du[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) du[v] = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const double tw = (d == 0) ? w[sz] * w[sy] : (d == 1 ? w[sz] * w[sx] : w[sy] * w[sx]);
+        const double upd = tw * invDx;
+        const int c = (d == 0) ? sx : (d == 1 ? sy : sz);
+        const int stride = (d == 0) ? 1 : (d == 1 ? B : B * B);
+        const int base = s - c * stride;
+#pragma unroll
+        for (int v = 0; v < V; v++)
+#pragma unroll
+          for (int m = 0; m < B; m++)
+            du[v] += T->Kxi[c * B + m] * Fh[(d * NS + base + m * stride) * V + v] * upd;
+      }
+#pragma unroll
+      for (int v = 0; v < V; v++) lduh[(cell * NS + s) * V + v] = du[v];
+      // extrapolation to the six faces; lane l produces outputs l, l + NS, ...
+      constexpr int NQ = 6 * NF * V;
+      for (int o = l; o < NQ; o += NS) {
+        const int v = o % V;
+        const int fn = (o / V) % NF;
+        const int f = o / (V * NF);
+        const int d = f >> 1;
+        const double* FC = (f & 1) ? T->F1 : T->F0;
+        const int a = fn / B, b = fn % B;
+        double qa = 0.0, fa = 0.0;
+#pragma unroll
+        for (int m = 0; m < B; m++) {
+          const int node = (d == 0) ? (a * B + b) * B + m : (d == 1 ? (a * B + m) * B + b : (m * B + a) * B + b);
+          qa += Qh[node * V + v] * FC[m];
+          fa += Fh[(d * NS + node) * V + v] * FC[m];
+        }
+        lQhbnd[cell * NQ + o] = qa;
+        lFhbnd[cell * NQ + o] = fa;
+      }
+      if (l == 0 && picard) picard[cell] = iters;
+    }
+}
+
+}  // namespace
+
+bool lines_supported(const KernelCfg& k) {
+  return k.dim == 3 && k.nvar == 5 && k.npar == 0 && k.order + 1 == 6 && k.pde.id == kPdeEuler;
+}
+
+int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const double* luh, const double* h,
+                             const double* dt, double* lduh, double* qb, double* fb, int* pic, bool mfma) {
+  constexpr int B = 6;
+  using S = LShape<B>;
+  const long blocks = (n + S::CPB - 1) / S::CPB;
+  const int adaptive = k.picard_max <= 0;
+  const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
+  if (k.precision == 0) {
+    hipLaunchKernelGGL((predictor3d_lines_kernel<B, double, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h, dt,
+                       lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+  } else {
+    if (mfma)
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h,
+                         dt, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+    else
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh,
+                         h, dt, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+  }
+  return static_cast<int>(hipGetLastError());
+}
+
+}  // namespace exa

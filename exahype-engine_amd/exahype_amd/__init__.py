@@ -52,6 +52,7 @@ _sig("exa_ctx_create", [ctypes.POINTER(_cp), ctypes.c_int, ctypes.c_int, ctypes.
                         ctypes.c_int, ctypes.c_int, ctypes.c_double])
 _sig("exa_ctx_destroy", [_cp])
 _sig("exa_ctx_set_advection", [_cp, _dp])
+_sig("exa_ctx_set_predictor_variant", [_cp, ctypes.c_int])
 _sig("exa_tables", [ctypes.c_int] + [_dp] * 9)
 _sig("exa_ctx_tables", [_cp] + [_dp] * 8)
 _sig("exa_fusedSpaceTimePredictorVolumeIntegral",
@@ -149,6 +150,10 @@ class Context:
     @property
     def nface(self):
         return self.B ** (self.dim - 1)
+
+    def set_predictor_variant(self, variant):
+        """0: fastest kernel for the shape (default), 1: generic node-column kernel."""
+        _check(_lib.exa_ctx_set_predictor_variant(self.h, variant))
 
     def set_advection(self, a):
         _check(_lib.exa_ctx_set_advection(self.h, _p(np.ascontiguousarray(a, dtype=np.float64))))

@@ -77,6 +77,11 @@ const char* exa_last_error(void);
 /* Advection velocity for EXA_PDE_ADVECTION (a[dim]). */
 int exa_ctx_set_advection(exa_ctx* ctx, const double* a);
 
+/* Predictor kernel selection: 0 (default) picks the fastest kernel compiled
+ * for the shape, 1 forces the generic node-column kernel (every shape).
+ * Meshes copy the context's settings when they are created. */
+int exa_ctx_set_predictor_variant(exa_ctx* ctx, int variant);
+
 /* Operator tables of order N derived on the host (no device needed; each
  * output may be NULL).  Same layout as exa_ctx_tables plus
  * egp[B*B] = equidistantGridProjector. */

@@ -314,3 +314,27 @@ def test_device_initial_condition_matches_host(dim, n, order):
     mesh.close()
     ctx.close()
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("precision,variant,tol", [(E.PREC_FP64, 2, 1e-13), (E.PREC_FP32, 2, 2e-6),
+                                                   (E.PREC_FP32, 3, 2e-6), (E.PREC_FP32, 0, 2e-6)])
+def test_predictor_variants_agree(precision, variant, tol):
+    """The line-decomposed 3D order-5 predictor (VALU and matrix-core forms)
+    agrees with the generic node-column kernel."""
+    n, order, steps = 4, 5, 2
+    luh = P.density_wave_3d(n, order)
+    outs = []
+    for variant in (variant, 1):
+        ctx = E.Context(3, order, 5, 0, precision=precision, picard_max=0 if precision == E.PREC_FP64 else 3)
+        ctx.set_predictor_variant(variant)
+        mesh = E.Mesh(ctx, n, 1.0 / n)
+        mesh.set_solution(luh)
+        mesh.init_dt(0.0)
+        mesh.run(steps)
+        mesh.sync()
+        outs.append((mesh.get_solution(), mesh.picard()))
+        mesh.close()
+        ctx.close()
+    assert rel_l2(outs[0][0] - luh, outs[1][0] - luh) < tol * 1e3
+    assert rel_l2(outs[0][0], outs[1][0]) < tol
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
