@@ -24,7 +24,12 @@
  *   face/surface    golden testSurfaceIntegralNonlinear (2D/3D)
  *   solution update golden testSolutionUpdate (2D/3D)
  *   stable dt       closed form CFL*PNPM/sum_d((|u_d|+c)/dx_d), uniform state
- *   linear CK path  closed-form plane wave (design-order convergence)
+ *   linear kernels  golden 2D elasticity fixtures (testSpaceTimePredictorLinear,
+ *                   testRiemannSolverLinear, testVolumeIntegralLinear,
+ *                   testSurfaceIntegralLinear; disabled in the reference but
+ *                   matching the generic kernels) + closed-form acoustic plane
+ *                   wave (exact time average, exact upwind fluctuations)
+ *   linear adjoint  finite differences + dot-product (transpose) identity
  *   amr projections constant preservation + polynomial reproduction
  */
 #ifndef EXAHYPE_AMD_ORACLE_H
@@ -50,7 +55,7 @@ typedef struct orc_basis {
   double fgp[3 * ORC_MAXB * ORC_MAXB];/* fineGridProjector[order][sub][i][j] */
 } orc_basis;
 
-enum { ORC_PDE_EULER = 0, ORC_PDE_ACOUSTIC = 1, ORC_PDE_ADVECTION = 2 };
+enum { ORC_PDE_EULER = 0, ORC_PDE_ACOUSTIC = 1, ORC_PDE_ADVECTION = 2, ORC_PDE_ELASTIC2D = 3 };
 
 typedef struct orc_solver {
   int dim;            /* 2 or 3 */
@@ -96,6 +101,27 @@ int orc_fused_stp_vi(const orc_solver* s, double* lduh, double* lQhbnd, double* 
  * counts of the last step.  threads: OpenMP threads (<=0: default). */
 int orc_run_uniform(const orc_solver* s, double* luh, int n_per_dim, double h,
                     int steps, double* dt_io, int* picard_out, int threads);
+
+/* ---- linear kernels (aderdg_linear_oracle.c) ----------------------------- */
+/* flags: 1 flux, 2 non-conservative product, 4 material matrix */
+void orc_lin_stp(const orc_solver* s, int flags, const double* luh, const double* dx, double dt,
+                 const double* PSi, double* lQhi, double* lFhi, double* lQbnd, double* lFbnd,
+                 double* levels);
+void orc_lin_volume(const orc_solver* s, double* lduh, const double* lFhi);
+void orc_lin_face(const orc_solver* s, double* lduh, const double* lFhbnd, int direction,
+                  int orientation, const double* dx);
+void orc_lin_surface(const orc_solver* s, double* lduh, const double* lFhbnd, const double* dx);
+void orc_lin_riemann(const orc_solver* s, int flags, double* FL, double* FR, const double* QL,
+                     const double* QR, int direction);
+double orc_ricker(double t, double f0, double t0);
+void orc_lin_psi(const orc_solver* s, const double* centre, double h, double t, double dt,
+                 const double* xs, double amp, double f0, double t0, double* PSi);
+int orc_lin_run(const orc_solver* s, double* luh, int n, double h, double dt, int nsteps,
+                const double* src_x, double amp, double f0, double t0, int nrec, const double* xr,
+                double* rec, double* states, int threads);
+double orc_lin_misfit_grad(const orc_solver* s, const double* u0, int n, double h, double dt,
+                           int nsteps, const double* src_x, double amp, double f0, double t0,
+                           int nrec, const double* xr, const double* obs, double* gc, int threads);
 
 #ifdef __cplusplus
 }

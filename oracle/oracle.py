@@ -18,7 +18,8 @@ LIB_PATH = os.path.join(HERE, "liborc.so")
 TABLES_PATH = os.path.join(ROOT, "tests", "golden", "legendre_tables.json")
 
 MAXB = 16
-PDE_EULER, PDE_ACOUSTIC, PDE_ADVECTION = 0, 1, 2
+PDE_EULER, PDE_ACOUSTIC, PDE_ADVECTION, PDE_ELASTIC2D = 0, 1, 2, 3
+LIN_FLUX, LIN_NCP, LIN_MM = 1, 2, 4
 
 # PNPM per order (Toolkit/exahype/toolkit/solverController.py:17-23).
 PNPM = [1.0, 0.33, 0.17, 0.1, 0.069, 0.045, 0.038, 0.03, 0.02, 0.015]
@@ -98,6 +99,19 @@ def _lib():
     lib.orc_fused_stp_vi.restype = ctypes.c_int
     lib.orc_run_uniform.argtypes = [sp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, ip, ctypes.c_int]
     lib.orc_run_uniform.restype = ctypes.c_int
+    I, D = ctypes.c_int, ctypes.c_double
+    lib.orc_lin_stp.argtypes = [sp, I, dp, dp, D, dp, dp, dp, dp, dp, dp]
+    lib.orc_lin_volume.argtypes = [sp, dp, dp]
+    lib.orc_lin_face.argtypes = [sp, dp, dp, I, I, dp]
+    lib.orc_lin_surface.argtypes = [sp, dp, dp, dp]
+    lib.orc_lin_riemann.argtypes = [sp, I, dp, dp, dp, dp, I]
+    lib.orc_ricker.argtypes = [D, D, D]
+    lib.orc_ricker.restype = D
+    lib.orc_lin_psi.argtypes = [sp, dp, D, D, D, dp, D, D, D, dp]
+    lib.orc_lin_run.argtypes = [sp, dp, I, D, D, I, dp, D, D, D, I, dp, dp, dp, I]
+    lib.orc_lin_run.restype = I
+    lib.orc_lin_misfit_grad.argtypes = [sp, dp, I, D, D, I, dp, D, D, D, I, dp, dp, dp, I]
+    lib.orc_lin_misfit_grad.restype = D
     return lib
 
 
@@ -248,3 +262,89 @@ class Oracle:
         if rc != 0:
             raise MemoryError("oracle run_uniform failed to allocate")
         return float(dt_io[0]), pic
+
+
+def _opt(a):
+    return _p(a) if a is not None else None
+
+
+class LinearOracle(Oracle):
+    """Linear (Cauchy-Kowalevski) kernels: acoustic 3D (4 variables, parameters
+    rho, c) or the 2D elasticity test PDE (9 variables, lambda, mu, rho)."""
+
+    def __init__(self, dim=3, order=4, pde=PDE_ACOUSTIC, cfl=0.9):
+        nvar, npar = (4, 2) if pde == PDE_ACOUSTIC else (9, 3)
+        super().__init__(dim, order, nvar=nvar, npar=npar, pde=pde, cfl=cfl)
+
+    def stp(self, luh, dx, dt, flags=LIN_FLUX | LIN_NCP | LIN_MM, PSi=None, want_levels=False):
+        B, D, V, ND = self.B, self.dim, self.nvar, self.nd
+        luh = np.ascontiguousarray(luh, dtype=np.float64)
+        lQhi = np.zeros(self.nspace * ND)
+        lFhi = np.zeros((D + 1) * self.nspace * V)
+        lQbnd = np.zeros(2 * D * self.nface * ND)
+        lFbnd = np.zeros(2 * D * self.nface * V)
+        lev = np.zeros((B + 1) * self.nspace * ND) if want_levels else None
+        ps = np.ascontiguousarray(PSi, dtype=np.float64) if PSi is not None else None
+        lib().orc_lin_stp(self.ps, flags, _p(luh), _p(np.array(dx, dtype=np.float64)), dt, _opt(ps),
+                          _p(lQhi), _p(lFhi), _p(lQbnd), _p(lFbnd), _opt(lev))
+        return dict(lQhi=lQhi, lFhi=lFhi, lQbnd=lQbnd, lFbnd=lFbnd, levels=lev)
+
+    def volume_integral(self, lFhi, dx=None):
+        lduh = np.zeros(self.nspace * self.nvar)
+        lib().orc_lin_volume(self.ps, _p(lduh), _p(np.ascontiguousarray(lFhi, dtype=np.float64)))
+        return lduh
+
+    def face_integral(self, lduh, lFhbnd_face, direction, orientation, dx):
+        lduh = np.array(lduh, dtype=np.float64)
+        lib().orc_lin_face(self.ps, _p(lduh), _p(np.ascontiguousarray(lFhbnd_face, dtype=np.float64)),
+                           direction, orientation, _p(np.array(dx, dtype=np.float64)))
+        return lduh
+
+    def surface_integral(self, lduh, lFhbnd, dx):
+        lduh = np.array(lduh, dtype=np.float64)
+        lib().orc_lin_surface(self.ps, _p(lduh), _p(np.ascontiguousarray(lFhbnd, dtype=np.float64)),
+                              _p(np.array(dx, dtype=np.float64)))
+        return lduh
+
+    def riemann(self, QL, QR, direction, flags=LIN_FLUX | LIN_NCP | LIN_MM):
+        n = self.nface * self.nvar
+        FL, FR = np.zeros(n), np.zeros(n)
+        lib().orc_lin_riemann(self.ps, flags, _p(FL), _p(FR), _p(np.ascontiguousarray(QL, dtype=np.float64)),
+                              _p(np.ascontiguousarray(QR, dtype=np.float64)), direction)
+        return FL, FR
+
+    def psi(self, centre, h, t, dt, xs, amp, f0, t0):
+        out = np.zeros((self.B + 1) * self.nspace * self.nvar)
+        lib().orc_lin_psi(self.ps, _p(np.array(centre, dtype=np.float64)), h, t, dt,
+                          _p(np.array(xs, dtype=np.float64)), amp, f0, t0, _p(out))
+        return out
+
+    @staticmethod
+    def ricker(t, f0, t0):
+        return lib().orc_ricker(t, f0, t0)
+
+    def run(self, luh, n, h, dt, nsteps, src, receivers, threads=0, want_states=False):
+        """Forward acoustic run in place; src = (xs, amp, f0, t0) or None.
+        Returns (rec[nsteps][nrec], states or None)."""
+        assert luh.dtype == np.float64 and luh.flags["C_CONTIGUOUS"]
+        xr = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1, 3)
+        rec = np.zeros((nsteps, max(len(xr), 1)))
+        st = np.zeros((nsteps,) + luh.shape) if want_states else None
+        xs, amp, f0, t0 = src if src is not None else (None, 0.0, 0.0, 0.0)
+        rc = lib().orc_lin_run(self.ps, _p(luh), n, h, dt, nsteps,
+                               _opt(np.array(xs, dtype=np.float64) if xs is not None else None), amp, f0, t0,
+                               len(xr), _p(xr), _p(rec), _opt(st), threads)
+        if rc != 0:
+            raise MemoryError("oracle lin_run failed to allocate")
+        return rec[:, :len(xr)], st
+
+    def misfit_grad(self, u0, n, h, dt, nsteps, src, receivers, obs, threads=0, want_grad=True):
+        """J and dJ/dc per cell (FP64 discrete adjoint, all states kept)."""
+        xr = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1, 3)
+        obs = np.ascontiguousarray(obs, dtype=np.float64)
+        gc = np.zeros(n ** 3) if want_grad else None
+        xs, amp, f0, t0 = src if src is not None else (None, 0.0, 0.0, 0.0)
+        J = lib().orc_lin_misfit_grad(self.ps, _p(np.ascontiguousarray(u0, dtype=np.float64)), n, h, dt, nsteps,
+                                      _opt(np.array(xs, dtype=np.float64) if xs is not None else None),
+                                      amp, f0, t0, len(xr), _p(xr), _p(obs), _opt(gc), threads)
+        return J, gc

@@ -9,6 +9,12 @@ reads two *data* files as text and writes their numbers out as JSON:
   (``GenericEulerKernelTest2D.cpp`` / ``GenericEulerKernelTest3D.cpp``).
   Every ``const double NAME[N] = {...};`` inside a ``namespace test...`` block
   is kept, grouped by the ``#ifdef Dim2`` / ``#ifdef Dim3`` section it sits in.
+* ``ExaHyPE/exahype/tests/kernels/testdata/elasticity_testdata.cpp`` -- the
+  2D (order 4, 9 variables + 3 parameters) vectors of the linear-kernel tests
+  in ``ElasticityKernelTest2D.cpp`` (namespaces testRiemannSolverLinear,
+  testSpaceTimePredictorLinear, testVolumeIntegralLinear,
+  testSurfaceIntegralLinear).  The reference runs those tests with
+  ``validate(false)``; the vectors still match the generic linear kernels.
 * ``ExaHyPE/kernels/generated/GaussLegendreBasis.csnippet`` -- the operator
   tables (nodes, weights, Kxi, iK1, dudx, FCoeff, fineGridProjector,
   equidistantGridProjector) that the reference generator printed with ~64
@@ -18,7 +24,8 @@ Decimal strings are converted with Python's correctly rounded ``float()``,
 which is the same round-to-nearest a C compiler applies to the literals.
 
 Usage:  python tests/golden/extract_reference_fixtures.py [/root/reference]
-Outputs: tests/golden/reference_testdata.json, tests/golden/legendre_tables.json
+Outputs: tests/golden/reference_testdata.json, tests/golden/legendre_tables.json,
+         tests/golden/elasticity_testdata.json
 """
 import json
 import os
@@ -72,6 +79,30 @@ def extract_testdata(ref):
     return result
 
 
+def extract_elasticity(ref):
+    path = os.path.join(ref, "ExaHyPE/exahype/tests/kernels/testdata/elasticity_testdata.cpp")
+    text = open(path).read()
+    array_re = re.compile(r"const\s+double\s+(\w+)\s*\[([^\]]*)\]\s*=?\s*\{(.*?)\}\s*;", re.S)
+    tokens = re.compile(r"(namespace\s+(test\w+)\s*\{|const\s+double\s+\w+\s*\[[^\]]*\]\s*=?\s*\{.*?\}\s*;)", re.S)
+    result = {}
+    namespace = None
+    for m in tokens.finditer(text):
+        tok = m.group(1)
+        if tok.startswith("namespace"):
+            namespace = m.group(2)
+            continue
+        a = array_re.match(tok)
+        name, size, body = a.group(1), a.group(2), a.group(3)
+        n = 1
+        for f in size.split("*"):
+            n *= int(f)
+        vals = _numbers(body)
+        if len(vals) != n:
+            raise RuntimeError(f"{namespace}::{name}: expected {n} values, got {len(vals)}")
+        result.setdefault(namespace, {})[name] = vals
+    return result
+
+
 def extract_tables(ref):
     path = os.path.join(ref, "ExaHyPE/kernels/generated/GaussLegendreBasis.csnippet")
     text = open(path).read()
@@ -106,6 +137,11 @@ def main():
     with open(os.path.join(HERE, "reference_testdata.json"), "w") as f:
         json.dump({"source": "ExaHyPE/exahype/tests/kernels/testdata/generic_euler_testdata.cpp",
                    "data": td}, f, separators=(",", ":"))
+    el = extract_elasticity(ref)
+    with open(os.path.join(HERE, "elasticity_testdata.json"), "w") as f:
+        json.dump({"source": "ExaHyPE/exahype/tests/kernels/testdata/elasticity_testdata.cpp",
+                   "data": el}, f, separators=(",", ":"))
+    print("elasticity", {k: {n: len(v) for n, v in arrs.items()} for k, arrs in el.items()})
     tb = extract_tables(ref)
     with open(os.path.join(HERE, "legendre_tables.json"), "w") as f:
         json.dump({"source": "ExaHyPE/kernels/generated/GaussLegendreBasis.csnippet",
