@@ -93,7 +93,7 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
     set_error("exa_ctx_create: dim must be 2 or 3");
     return EXA_ERR_ARG;
   }
-  if (pde != EXA_PDE_EULER && pde != EXA_PDE_ADVECTION) {
+  if (pde != EXA_PDE_EULER && pde != EXA_PDE_ADVECTION && pde != EXA_PDE_ACOUSTIC) {
     set_error("exa_ctx_create: unknown pde");
     return EXA_ERR_ARG;
   }
@@ -109,7 +109,16 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
     set_error("exa_ctx_create: cfl must be positive");
     return EXA_ERR_ARG;
   }
-  if (!exa::shape_supported(dim, order, nvar, npar)) {
+  if (pde == EXA_PDE_ACOUSTIC) {
+    if (dim != 3 || nvar != 4 || npar != 2) {
+      set_error("exa_ctx_create: EXA_PDE_ACOUSTIC needs dim 3, nvar 4 (p,u,v,w), npar 2 (rho,c)");
+      return EXA_ERR_ARG;
+    }
+    if (precision != EXA_PREC_FP64 || !exa::acoustic_order_supported(order)) {
+      set_error("exa_ctx_create: acoustic path compiled for FP64 and orders 1..5 only");
+      return EXA_ERR_UNSUPPORTED;
+    }
+  } else if (!exa::shape_supported(dim, order, nvar, npar)) {
     set_error("exa_ctx_create: (dim, order, nvar, npar) not compiled into this library");
     return EXA_ERR_UNSUPPORTED;
   }

@@ -68,6 +68,38 @@ int launch_density_wave(hipStream_t s, long ncells, int n, int D, int B, int ND,
                         const double* d_prm, double* luh);
 int launch_fill(hipStream_t s, double* p, long n, double value);
 
+// ---- linear acoustics (acoustic.hip) ---------------------------------------
+// Device arrays of the acoustic mesh: u[cell][node][4] (p,u,v,w), mat[cell][2]
+// (rho, c), qb[cell][6][B*B][4] (time-averaged variables on the faces),
+// adjoint cotangents ubar (like u), fbar and qbbar (like qb), gc[cell].
+struct AcousticArgs {
+  long ncells = 0;
+  int n = 0;  // cells per direction (periodic n^3 mesh)
+  double h = 0, dt = 0;
+  const DevTables* tab = nullptr;
+  double* u = nullptr;
+  const double* mat = nullptr;
+  double* qb = nullptr;
+  long src_cell = -1;              // cell holding the point source, -1: none
+  const double* src_g = nullptr;   // [B^3] spatial factor of the source moments
+  const double* src_tab = nullptr; // [steps][B] time factors per step
+  const int* step_ptr = nullptr;   // device step counter
+  int step_override = -1;          // >= 0: use this step instead of *step_ptr
+  int nrec = 0;
+  const int* rcell = nullptr;      // [nrec]
+  const double* rw = nullptr;      // [nrec][B^3] interpolation weights
+  double* rec = nullptr;           // [steps][nrec]
+  double* ubar = nullptr;
+  double* fbar = nullptr;
+  double* qbbar = nullptr;
+  double* gc = nullptr;
+};
+enum { kAcPredictUpdate = 0, kAcPredictFaces, kAcCorrector, kAcFbar, kAcRiemannAdj, kAcPredictAdj, kAcReceivers };
+bool acoustic_order_supported(int order);
+int launch_acoustic(int which, int order, const AcousticArgs& a, hipStream_t s);
+int launch_acoustic_receivers_adj(int order, const AcousticArgs& a, const double* res, hipStream_t s);
+int launch_acoustic_advance(int* step, hipStream_t s);
+
 void set_error(const std::string& msg);
 
 }  // namespace exa

@@ -19,7 +19,7 @@ REPO_DIR = os.path.dirname(ENGINE_DIR)
 LIB_PATH = os.path.join(ENGINE_DIR, "libexahype_amd.so")
 HEADER_PATH = os.path.join(REPO_DIR, "include", "exahype_amd.h")
 
-PDE_EULER, PDE_ADVECTION = 0, 2
+PDE_EULER, PDE_ACOUSTIC, PDE_ADVECTION = 0, 1, 2
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 EXA_OK, EXA_ERR_ARG, EXA_ERR_HIP, EXA_ERR_UNSUPPORTED, EXA_ERR_NOMEM = 0, 1, 2, 3, 4
 
@@ -84,6 +84,22 @@ _sig("exa_mesh_picard", [_cp, _ip])
 _sig("exa_mesh_stream", [_cp], ctypes.c_void_p)
 _sig("exa_mesh_set_timing", [_cp, ctypes.c_int])
 _sig("exa_mesh_phase_times", [_cp, _dp, _dp, _dp, _ip])
+_I, _D = ctypes.c_int, ctypes.c_double
+_sig("exa_acoustic_create", [ctypes.POINTER(_cp), _cp, _I, _D])
+_sig("exa_acoustic_destroy", [_cp])
+_sig("exa_acoustic_ncells", [_cp], ctypes.c_long)
+_sig("exa_acoustic_set_solution", [_cp, _dp])
+_sig("exa_acoustic_get_solution", [_cp, _dp])
+_sig("exa_acoustic_set_material", [_cp, _dp, _dp])
+_sig("exa_acoustic_get_material", [_cp, _dp, _dp])
+_sig("exa_acoustic_set_dt", [_cp, _D])
+_sig("exa_acoustic_set_step", [_cp, _I])
+_sig("exa_acoustic_time", [_cp, _ip, _dp, _dp])
+_sig("exa_acoustic_set_source", [_cp, _dp, _D, _D, _D])
+_sig("exa_acoustic_set_receivers", [_cp, _I, _dp])
+_sig("exa_acoustic_run", [_cp, _I, _I, _dp])
+_sig("exa_acoustic_misfit_gradient", [_cp, _I, _I, _dp, _dp, _dp])
+_sig("exa_acoustic_last_times", [_cp, _dp, _dp, _dp])
 
 
 def declared_symbols():
@@ -286,3 +302,89 @@ class Mesh:
         a, b, c, n = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
         _check(_lib.exa_mesh_phase_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(n)))
         return dict(predictor_ms=a.value, riemann_ms=b.value, corrector_ms=c.value, steps=n.value)
+
+
+class Acoustic:
+    """Linear acoustics (p, u, v, w; per-cell rho, c) on a periodic n^3 mesh,
+    forward ADER-DG steps and the misfit gradient with respect to c."""
+
+    def __init__(self, ctx, n, h):
+        a = _cp()
+        _check(_lib.exa_acoustic_create(ctypes.byref(a), ctx.h, n, h))
+        self.h = a
+        self.ctx = ctx
+        self.n = n
+        self.ncells = _lib.exa_acoustic_ncells(a)
+        self.nrec = 0
+
+    def close(self):
+        if self.h:
+            _lib.exa_acoustic_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_solution(self, luh):
+        luh = np.ascontiguousarray(luh, dtype=np.float64)
+        assert luh.size == self.ncells * self.ctx.nspace * 6
+        _check(_lib.exa_acoustic_set_solution(self.h, _p(luh)))
+
+    def get_solution(self):
+        out = np.zeros(self.ncells * self.ctx.nspace * 6)
+        _check(_lib.exa_acoustic_get_solution(self.h, _p(out)))
+        return out
+
+    def set_material(self, rho=None, c=None):
+        r = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
+        cc = None if c is None else np.ascontiguousarray(c, dtype=np.float64)
+        for a in (r, cc):
+            assert a is None or a.size == self.ncells
+        _check(_lib.exa_acoustic_set_material(self.h, _p(r), _p(cc)))
+
+    def get_material(self):
+        r, c = np.zeros(self.ncells), np.zeros(self.ncells)
+        _check(_lib.exa_acoustic_get_material(self.h, _p(r), _p(c)))
+        return r, c
+
+    def set_dt(self, dt=0.0):
+        _check(_lib.exa_acoustic_set_dt(self.h, dt))
+
+    def set_step(self, step):
+        _check(_lib.exa_acoustic_set_step(self.h, step))
+
+    def time(self):
+        s, t, dt = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        _check(_lib.exa_acoustic_time(self.h, ctypes.byref(s), ctypes.byref(t), ctypes.byref(dt)))
+        return s.value, t.value, dt.value
+
+    def set_source(self, x, amp, f0, t0):
+        xs = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        _check(_lib.exa_acoustic_set_source(self.h, _p(xs), amp, f0, t0))
+
+    def set_receivers(self, xr):
+        xr = np.ascontiguousarray(xr, dtype=np.float64).reshape(-1, 3)
+        _check(_lib.exa_acoustic_set_receivers(self.h, len(xr), _p(xr)))
+        self.nrec = len(xr)
+
+    def run(self, nsteps, use_graph=False):
+        """Advance nsteps; returns p at the receivers, shape (nsteps, nrec)."""
+        rec = np.zeros((nsteps, max(self.nrec, 1)))
+        _check(_lib.exa_acoustic_run(self.h, nsteps, 1 if use_graph else 0, _p(rec)))
+        return rec[:, :self.nrec]
+
+    def misfit_gradient(self, nsteps, obs, checkpoint_every=20):
+        obs = np.ascontiguousarray(obs, dtype=np.float64)
+        assert obs.size == nsteps * self.nrec
+        J = np.zeros(1)
+        g = np.zeros(self.ncells)
+        _check(_lib.exa_acoustic_misfit_gradient(self.h, nsteps, checkpoint_every, _p(obs), _p(J), _p(g)))
+        return float(J[0]), g
+
+    def last_times(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(_lib.exa_acoustic_last_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value

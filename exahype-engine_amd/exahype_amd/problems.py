@@ -9,6 +9,10 @@ luh[cell][z][y][x][var], cells x fastest, unit cube / square, n cells per side.
 * density_wave_3d  -- 3D Euler, rho = 1 + 0.2 sin(2 pi (x + y + z)) + sum of
                       M seeded Fourier modes of amplitude 0.01/M (|k_i| <= 4) in
                       rho and in the x-velocity u; v = w = 1, p = 1
+* acoustic_*       -- 3D linear acoustics: per-cell wave speed
+                      c = 1 + sum of 8 seeded Gaussian bumps (+-0.2, width 0.1),
+                      rho = 1, a Ricker point source (f0 = 5) and 64 seeded
+                      receivers around it
 """
 import numpy as np
 
@@ -109,3 +113,57 @@ def density_wave_params(dim, seed=0, modes=8):
 
 def dof_per_cell(dim, order, nvar=5):
     return (order + 1) ** dim * nvar
+
+
+# ---- linear acoustics ---------------------------------------------------------
+ACOUSTIC_SOURCE = (0.5, 0.5, 0.5)
+ACOUSTIC_F0 = 5.0
+ACOUSTIC_T0 = 0.06
+ACOUSTIC_AMP = 1.0
+
+
+def acoustic_bumps(seed=0, nbumps=8, amp=0.2, width=0.1):
+    """(centres[nb,3], signed amplitudes[nb]) of the wave-speed bumps."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0.0, 1.0, size=(nbumps, 3))
+    signs = rng.choice([-1.0, 1.0], size=nbumps)
+    return centres, amp * signs, width
+
+
+def acoustic_wave_speed(n, seed=0, scale=1.0, nbumps=8):
+    """c per cell (cells x fastest) = 1 + scale * sum_b a_b exp(-|x - x_b|^2 / (2 w^2)),
+    evaluated at the cell centres of the unit cube."""
+    centres, amps, width = acoustic_bumps(seed, nbumps)
+    ar = (np.arange(n) + 0.5) / n
+    Z, Y, X = np.meshgrid(ar, ar, ar, indexing="ij")
+    pts = np.stack([X, Y, Z], -1).reshape(-1, 3)
+    c = np.ones(len(pts))
+    for xb, a in zip(centres, amps):
+        c += scale * a * np.exp(-np.sum((pts - xb) ** 2, axis=1) / (2 * width ** 2))
+    return c
+
+
+def acoustic_receivers(seed=0, nrec=64, centre=ACOUSTIC_SOURCE, radius=0.12):
+    """nrec receiver positions: a 4x4x4 lattice of spacing radius*2/3 around
+    the source, each jittered by up to 1/10 of the spacing (seeded)."""
+    rng = np.random.default_rng(seed + 1000)
+    m = int(round(nrec ** (1 / 3)))
+    assert m ** 3 == nrec
+    g = np.linspace(-radius, radius, m)
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([X, Y, Z], -1).reshape(-1, 3) + np.asarray(centre)
+    pts += rng.uniform(-0.1, 0.1, size=pts.shape) * (2 * radius / (m - 1))
+    return pts
+
+
+def acoustic_state(n, order, c, rho=None, u=None):
+    """Reference-layout state luh[cell][B^3][6] = (p,u,v,w,rho,c) with the given
+    per-cell material and variables (zero if u is None)."""
+    B3 = (order + 1) ** 3
+    nc = n ** 3
+    luh = np.zeros((nc, B3, 6))
+    if u is not None:
+        luh[..., :4] = u
+    luh[..., 4] = (np.ones(nc) if rho is None else np.asarray(rho))[:, None]
+    luh[..., 5] = np.asarray(c)[:, None]
+    return luh

@@ -41,6 +41,9 @@ enum exa_status {
  * (AbstractGenericADERDGSolverHeader.template:220,311-313). */
 enum exa_pde {
   EXA_PDE_EULER = 0,      /* ApplicationExamples/Euler/Euler_ADERDG/EulerSolver_ADERDG.cpp:61-147 */
+  EXA_PDE_ACOUSTIC = 1,   /* linear acoustics, 3D: p,u,v,w; parameters rho, c
+                             (ApplicationExamples/Linear/LinearWave3D/MyLinearWaveSolver.cpp:108-209);
+                             linear (Cauchy-Kowalevski) kernels, exa_acoustic_* below */
   EXA_PDE_ADVECTION = 2   /* linear advection F_d = a_d Q, eigenvalues a_d (test PDE)          */
 };
 
@@ -202,6 +205,57 @@ void* exa_mesh_stream(exa_mesh* mesh);
 int exa_mesh_set_timing(exa_mesh* mesh, int enabled);
 int exa_mesh_phase_times(exa_mesh* mesh, double* ms_predictor, double* ms_riemann, double* ms_corrector,
                          int* nsteps);
+
+/* ---------------------------------------------------------------------------
+ * Linear acoustics on a periodic n^3 mesh (context: dim 3, nvar 4, npar 2,
+ * EXA_PDE_ACOUSTIC, EXA_PREC_FP64, orders 1..5), FP64.  One step is the
+ * reference's linear ADER-DG step: spaceTimePredictorLinear + volumeIntegralLinear
+ * (generic/c/3d/spaceTimePredictorLinear.cpph:30-647, volumeIntegralLinear.cpph:21-70),
+ * riemannSolverLinear (riemannSolverLinear.cpph:23-167), faceIntegralLinear
+ * (faceIntegralLinear.cpph:8-101) and solutionUpdate, with one point source
+ * (getPointSources / deltaDistribution, deltaDistribution.cpph:37-146) whose
+ * time function is a Ricker wavelet amp (1 - 2 a) exp(-a), a = (pi f0 (t - t0))^2,
+ * acting on p.  dt is fixed; t = step * dt.
+ * Material (rho, c) is constant per cell (the gradient is per cell).
+ * The misfit gradient is this library's own discrete adjoint (the reference
+ * has none).
+ * ------------------------------------------------------------------------- */
+typedef struct exa_acoustic exa_acoustic;
+
+int exa_acoustic_create(exa_acoustic** out, exa_ctx* ctx, int n, double h);
+int exa_acoustic_destroy(exa_acoustic* ac);
+long exa_acoustic_ncells(const exa_acoustic* ac);
+/* Solution in the reference layout [ncells][B^3][6] (p,u,v,w,rho,c), cells x
+ * fastest.  set: rho and c must be equal at all nodes of a cell
+ * (EXA_ERR_UNSUPPORTED otherwise); they become the cell's material. */
+int exa_acoustic_set_solution(exa_acoustic* ac, const double* luh);
+int exa_acoustic_get_solution(exa_acoustic* ac, double* luh);
+/* Per-cell material rho[ncells], c[ncells] (either may be NULL to keep it). */
+int exa_acoustic_set_material(exa_acoustic* ac, const double* rho, const double* c);
+int exa_acoustic_get_material(exa_acoustic* ac, double* rho, double* c);
+/* dt > 0: fixed step; dt <= 0: CFL * PNPM / sum_d(max c / h) (stableTimeStepSize
+ * with eigenvalues (c, -c, 0, 0)) of the current material. */
+int exa_acoustic_set_dt(exa_acoustic* ac, double dt);
+int exa_acoustic_set_step(exa_acoustic* ac, int step);
+int exa_acoustic_time(exa_acoustic* ac, int* step, double* t, double* dt);
+/* One point source at x[3] (amp = 0 removes it). */
+int exa_acoustic_set_source(exa_acoustic* ac, const double* x, double amp, double f0, double t0);
+/* Receivers at xr[nrec][3]: p interpolated with the cell's basis after every step. */
+int exa_acoustic_set_receivers(exa_acoustic* ac, int nrec, const double* xr);
+/* Advance nsteps steps from the current state; rec (nullable) receives
+ * p[nsteps][nrec] at the receivers.  use_graph != 0 replays a captured
+ * hipGraph of one step.  Synchronous. */
+int exa_acoustic_run(exa_acoustic* ac, int nsteps, int use_graph, double* rec);
+/* Misfit J = 1/2 sum_k sum_r (p_r(t_{k+1}) - obs[k][r])^2 over nsteps steps
+ * from the current state, and dJ/dc per cell (grad_c[ncells]) by the
+ * discrete adjoint with recomputation from checkpoints stored every
+ * checkpoint_every steps.  Leaves the state after the last step (as
+ * exa_acoustic_run).  Synchronous. */
+int exa_acoustic_misfit_gradient(exa_acoustic* ac, int nsteps, int checkpoint_every, const double* obs,
+                                 double* J, double* grad_c);
+/* Milliseconds of the last exa_acoustic_run / exa_acoustic_misfit_gradient:
+ * forward stepping, forward recomputation, adjoint sweep. */
+int exa_acoustic_last_times(exa_acoustic* ac, double* ms_forward, double* ms_recompute, double* ms_adjoint);
 
 #ifdef __cplusplus
 }
