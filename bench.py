@@ -15,6 +15,12 @@ Two forms run in the same process on the same initial state:
 `value` is the FP32 form's DoF-updates/s; the FP64 form, the per-variable
 relative L2 error of FP32 vs FP64 and the speed-up are reported beside it.
 
+`--workload acoustic` instead measures the linear acoustic workload (3D,
+order 4, 54^3 cells, per-cell wave speed with 8 Gaussian bumps, Ricker
+source f0 = 5, 64 receivers, 300 steps): forward DoF-updates/s in FP64 and
+the wall time of the misfit gradient (discrete adjoint, checkpoints every 20
+steps).
+
 A DoF-update is one (node, variable) advanced by one time step:
 cells * (N+1)^3 * nVar * steps / wall time.  The timed region starts with the
 state resident in HBM and brackets exactly K steps with device syncs.
@@ -126,20 +132,125 @@ def cpu_baseline(O, P, order, n_cpu, steps_cpu, threads):
                        f"({t:.2f} s wall, {threads} threads)")
 
 
+def acoustic_bytes_per_cell(B):
+    """Algorithmic HBM bytes per cell and step of the two acoustic kernels.
+    predictor: read u (4 doubles per node) + material (2), write u and the six
+    extrapolated faces (4 doubles per face node); corrector: read u, its own
+    faces and the six neighbour faces (each face is read by both of its
+    cells), material, write u."""
+    N, F = B ** 3, B ** 2
+    pred = 8 * (4 * N + 2 + 4 * N + 6 * F * 4)
+    corr = 8 * (4 * N + 6 * F * 4 + 6 * F * 4 + 2 + 4 * N)
+    return pred, corr
+
+
+def acoustic_cpu_baseline(O, P, order, n_cpu, steps_cpu, threads):
+    c = P.acoustic_wave_speed(n_cpu, seed=0)
+    luh = np.ascontiguousarray(P.acoustic_state(n_cpu, order, c))
+    h = 1.0 / n_cpu
+    dt = 0.9 * O.PNPM[order] * h / (3 * c.max())
+    o = O.LinearOracle(3, order)
+    src = (np.array(P.ACOUSTIC_SOURCE), P.ACOUSTIC_AMP, P.ACOUSTIC_F0, P.ACOUSTIC_T0)
+    t0 = time.perf_counter()
+    o.run(luh, n_cpu, h, dt, steps_cpu, src, P.acoustic_receivers(seed=0), threads=threads)
+    t = time.perf_counter() - t0
+    dofs = n_cpu ** 3 * (order + 1) ** 3 * 4 * steps_cpu
+    return dict(value=dofs / t, unit="DoF-updates/s", cores=threads, kind="port",
+                sample=f"oracle (FP64 C restatement of the reference generic linear kernels, OpenMP over cells) on "
+                       f"{n_cpu}^3 periodic cells, order {order}, {steps_cpu} steps of the same workload "
+                       f"({t:.2f} s wall, {threads} threads)")
+
+
+def bench_acoustic(args, E, P):
+    n, order, steps, every = args.n, args.order, args.steps, 20
+    B = order + 1
+    ctx = E.Context(3, order, nvar=4, npar=2, pde=E.PDE_ACOUSTIC)
+    c_true = P.acoustic_wave_speed(n, seed=args.seed, scale=1.0)
+    c0 = P.acoustic_wave_speed(n, seed=args.seed, scale=0.5)
+    rec = P.acoustic_receivers(seed=args.seed, nrec=64)
+
+    def make(c, dt=0.0):
+        ac = E.Acoustic(ctx, n, 1.0 / n)
+        ac.set_material(rho=np.ones(n ** 3), c=c)
+        ac.set_dt(dt)
+        ac.set_source(np.array(P.ACOUSTIC_SOURCE), P.ACOUSTIC_AMP, P.ACOUSTIC_F0, P.ACOUSTIC_T0)
+        ac.set_receivers(rec)
+        return ac
+
+    truth = make(c_true)
+    dt = truth.time()[2]
+    if args.warmup:
+        truth.run(args.warmup)
+        truth.set_step(0)
+        truth.set_material(c=c_true)
+        truth.set_solution(P.acoustic_state(n, order, c_true))
+    truth.set_timing(True)
+    t0 = time.perf_counter()
+    obs = truth.run(steps)
+    wall = time.perf_counter() - t0
+    ph = truth.phase_times()
+    truth.close()
+    ac = make(c0, dt)
+    t1 = time.perf_counter()
+    J, g = ac.misfit_gradient(steps, obs, checkpoint_every=every)
+    grad_wall = time.perf_counter() - t1
+    ms_fwd, ms_rec, ms_adj = ac.last_times()
+    ac.close()
+    cells = n ** 3
+    dofs = cells * B ** 3 * 4 * steps
+    pred_b, corr_b = acoustic_bytes_per_cell(B)
+    pred_ms, corr_ms = ph["predictor_ms"] / steps, ph["corrector_ms"] / steps
+    dom = ("predictor", pred_b, pred_ms) if pred_ms >= corr_ms else ("corrector", corr_b, corr_ms)
+    achieved = cells * dom[1] / (dom[2] * 1e-3) / 1e9
+    log(f"[bench] acoustic forward: {dofs / wall:.4e} DoF/s, {wall * 1e3 / steps:.3f} ms/step "
+        f"(pred {pred_ms:.3f} corr {corr_ms:.3f}); gradient {grad_wall:.2f} s "
+        f"(fwd {ms_fwd:.0f} ms, recompute {ms_rec:.0f} ms, adjoint {ms_adj:.0f} ms), J={J:.4e}")
+    cpu = None
+    if not args.no_cpu:
+        O = __import__("oracle")
+        try:
+            threads = len(os.sched_getaffinity(0))
+        except AttributeError:
+            threads = os.cpu_count() or 1
+        threads = max(1, min(16, threads))
+        cpu = acoustic_cpu_baseline(O, P, order, args.cpu_n, args.cpu_steps, threads)
+        log(f"[bench] cpu baseline: {cpu['value']:.4e} DoF/s on {threads} threads")
+    out = {
+        "metric": "DoF-updates/s 3D linear acoustics order 4 (forward, FP64); misfit-gradient wall time",
+        "value": dofs / wall, "unit": "DoF-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D linear acoustics ADER-DG order {order}, {n}^3 periodic cells, c = 1 + 8 seeded "
+                               f"Gaussian bumps (+-0.2, width 0.1), rho = 1, Ricker source f0 = 5, 64 receivers, "
+                               f"{steps} steps, dt = {dt:.4e}; gradient by the discrete adjoint, checkpoints "
+                               f"every {every}", "cells": cells, "order": order},
+        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "phases_ms_per_step": {"predictor": pred_ms, "corrector": corr_ms},
+        "gradient_s": grad_wall,
+        "gradient_phases_ms": {"forward": ms_fwd, "recompute": ms_rec, "adjoint": ms_adj},
+        "gradient_over_forward": grad_wall / wall,
+        "misfit": J,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=81, help="cells per dimension")
-    ap.add_argument("--order", type=int, default=5)
+    ap.add_argument("--workload", choices=("euler", "acoustic"), default="euler")
+    ap.add_argument("--n", type=int, default=None, help="cells per dimension (81 euler, 54 acoustic)")
+    ap.add_argument("--order", type=int, default=None, help="order (5 euler, 4 acoustic)")
     ap.add_argument("--picard-fp32", type=int, default=0,
                     help="fixed Picard count of the FP32 form (0: the largest count the FP64 form executed)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--modes", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-n", type=int, default=27)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-n", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--sweep-variants", action="store_true",
                     help="only time the predictor kernel variants (node-column, line, line+MFMA) and exit")
     args = ap.parse_args()
@@ -154,6 +265,21 @@ def main():
 
     import exahype_amd as E
     from exahype_amd import problems as P
+
+    if args.workload == "acoustic":
+        args.n = args.n or 54
+        args.order = args.order or 4
+        args.cpu_n = args.cpu_n or 27
+        args.cpu_steps = args.cpu_steps or 20
+        if "--steps" not in sys.argv:
+            args.steps = 300
+        log(f"[bench] acoustic N={args.order} {args.n}^3 cells, steps={args.steps} warmup={args.warmup}")
+        bench_acoustic(args, E, P)
+        return
+    args.n = args.n or 81
+    args.order = args.order or 5
+    args.cpu_n = args.cpu_n or 27
+    args.cpu_steps = args.cpu_steps or 2
 
     log(f"[bench] 3D Euler N={args.order} {args.n}^3 cells, steps={args.steps} warmup={args.warmup}")
     if args.sweep_variants:

@@ -61,6 +61,10 @@ struct exa_acoustic {
   double* d_res = nullptr;
   long res_cap = 0;
   double ms[3] = {0, 0, 0};
+  // per-kernel timing of exa_acoustic_run (HIP events on the launch stream)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  int timed_steps = 0;
 };
 
 namespace {
@@ -202,12 +206,15 @@ exa::AcousticArgs args(exa_acoustic* ac, double* u) {
 
 // one forward step on u (step index `step`, or the device counter if < 0);
 // receivers recorded into rec[(step - rec_base) * nrec + r] when rec != nullptr
-int enqueue_forward(exa_acoustic* ac, double* u, int step, double* rec) {
+int enqueue_forward(exa_acoustic* ac, double* u, int step, double* rec, hipEvent_t* ev = nullptr) {
   exa::AcousticArgs a = args(ac, u);
   a.step_override = step;
   const int o = ac->k.order;
+  if (ev) HIP_OK(hipEventRecord(ev[0], ac->stream), "event");
   RC_OK(status(exa::launch_acoustic(exa::kAcPredictUpdate, o, a, ac->stream), "acoustic predictor"));
+  if (ev) HIP_OK(hipEventRecord(ev[1], ac->stream), "event");
   RC_OK(status(exa::launch_acoustic(exa::kAcCorrector, o, a, ac->stream), "acoustic corrector"));
+  if (ev) HIP_OK(hipEventRecord(ev[2], ac->stream), "event");
   if (rec && ac->nrec > 0) {
     a.rec = rec;
     RC_OK(status(exa::launch_acoustic(exa::kAcReceivers, o, a, ac->stream), "acoustic receivers"));
@@ -222,6 +229,7 @@ void free_acoustic(exa_acoustic* ac) {
     if (b) (void)hipFree(b);
   if (ac->d_step) (void)hipFree(ac->d_step);
   if (ac->d_rcell) (void)hipFree(ac->d_rcell);
+  for (hipEvent_t e : ac->ev) (void)hipEventDestroy(e);
   if (ac->stream) (void)hipStreamDestroy(ac->stream);
   delete ac;
 }
@@ -493,8 +501,17 @@ int exa_acoustic_run(exa_acoustic* ac, int nsteps, int use_graph, double* rec) {
     if (ge) (void)hipGraphExecDestroy(ge);
     if (g) (void)hipGraphDestroy(g);
   } else {
+    if (ac->timing) {
+      while (ac->ev.size() < size_t(3) * nsteps) {
+        hipEvent_t e = nullptr;
+        HIP_OK(hipEventCreate(&e), "event create");
+        ac->ev.push_back(e);
+      }
+      ac->timed_steps = nsteps;
+    }
     for (int i = 0; !rc && i < nsteps; i++) {
-      rc = enqueue_forward(ac, ac->d_u, s0 + i, ac->d_rec - long(s0) * ac->nrec);
+      rc = enqueue_forward(ac, ac->d_u, s0 + i, ac->d_rec - long(s0) * ac->nrec,
+                           ac->timing ? &ac->ev[size_t(3) * i] : nullptr);
     }
   }
   if (!rc) {
@@ -629,6 +646,29 @@ int exa_acoustic_misfit_gradient(exa_acoustic* ac, int nsteps, int every, const 
   if (rc) return rc;
   *J = Jv;
   return set_step(ac, s0 + K);
+}
+
+int exa_acoustic_set_timing(exa_acoustic* ac, int enabled) {
+  if (!ac) return argerr("exa_acoustic_set_timing: NULL acoustic");
+  ac->timing = enabled != 0;
+  ac->timed_steps = 0;
+  return EXA_OK;
+}
+
+int exa_acoustic_phase_times(exa_acoustic* ac, double* ms_predictor, double* ms_corrector, int* nsteps) {
+  if (!ac) return argerr("exa_acoustic_phase_times: NULL acoustic");
+  double a = 0, b = 0;
+  for (int i = 0; i < ac->timed_steps; i++) {
+    float x = 0, y = 0;
+    HIP_OK(hipEventElapsedTime(&x, ac->ev[size_t(3) * i], ac->ev[size_t(3) * i + 1]), "elapsed");
+    HIP_OK(hipEventElapsedTime(&y, ac->ev[size_t(3) * i + 1], ac->ev[size_t(3) * i + 2]), "elapsed");
+    a += x;
+    b += y;
+  }
+  if (ms_predictor) *ms_predictor = a;
+  if (ms_corrector) *ms_corrector = b;
+  if (nsteps) *nsteps = ac->timed_steps;
+  return EXA_OK;
 }
 
 int exa_acoustic_last_times(exa_acoustic* ac, double* ms_forward, double* ms_recompute, double* ms_adjoint) {

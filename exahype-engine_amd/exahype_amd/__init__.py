@@ -100,6 +100,8 @@ _sig("exa_acoustic_set_receivers", [_cp, _I, _dp])
 _sig("exa_acoustic_run", [_cp, _I, _I, _dp])
 _sig("exa_acoustic_misfit_gradient", [_cp, _I, _I, _dp, _dp, _dp])
 _sig("exa_acoustic_last_times", [_cp, _dp, _dp, _dp])
+_sig("exa_acoustic_set_timing", [_cp, _I])
+_sig("exa_acoustic_phase_times", [_cp, _dp, _dp, _ip])
 
 
 def declared_symbols():
@@ -385,6 +387,16 @@ class Acoustic:
         return float(J[0]), g
 
     def last_times(self):
+        """(forward, recompute, adjoint) milliseconds of the last run / gradient."""
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _check(_lib.exa_acoustic_last_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def set_timing(self, enabled):
+        _check(_lib.exa_acoustic_set_timing(self.h, 1 if enabled else 0))
+
+    def phase_times(self):
+        """Per-kernel milliseconds summed over the last eager run: predictor, corrector, steps."""
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _check(_lib.exa_acoustic_phase_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
+        return dict(predictor_ms=a.value, corrector_ms=b.value, steps=n.value)

@@ -256,6 +256,11 @@ int exa_acoustic_misfit_gradient(exa_acoustic* ac, int nsteps, int checkpoint_ev
 /* Milliseconds of the last exa_acoustic_run / exa_acoustic_misfit_gradient:
  * forward stepping, forward recomputation, adjoint sweep. */
 int exa_acoustic_last_times(exa_acoustic* ac, double* ms_forward, double* ms_recompute, double* ms_adjoint);
+/* Per-kernel timing of exa_acoustic_run (eager path): HIP events around the
+ * predictor and the corrector of every step; milliseconds summed over the
+ * last run's steps. */
+int exa_acoustic_set_timing(exa_acoustic* ac, int enabled);
+int exa_acoustic_phase_times(exa_acoustic* ac, double* ms_predictor, double* ms_corrector, int* nsteps);
 
 #ifdef __cplusplus
 }
