@@ -30,7 +30,12 @@
  *                   matching the generic kernels) + closed-form acoustic plane
  *                   wave (exact time average, exact upwind fluctuations)
  *   linear adjoint  finite differences + dot-product (transpose) identity
- *   amr projections constant preservation + polynomial reproduction
+ *   amr projections constant preservation (the reference's
+ *                   testFaceUnknownsProjection / testVolumeUnknownsProjection)
+ *                   + exact reproduction of polynomials (prolongation) and
+ *                   exact L2 projection (restriction) at level differences 1, 2
+ *   amr mesh step   free-stream preservation, conservation, reduction to the
+ *                   uniform driver for an unrefined / fully refined mesh
  */
 #ifndef EXAHYPE_AMD_ORACLE_H
 #define EXAHYPE_AMD_ORACLE_H
@@ -122,6 +127,22 @@ int orc_lin_run(const orc_solver* s, double* luh, int n, double h, double dt, in
 double orc_lin_misfit_grad(const orc_solver* s, const double* u0, int n, double h, double dt,
                            int nsteps, const double* src_x, double amp, double f0, double t0,
                            int nrec, const double* xr, const double* obs, double* gc, int threads);
+
+/* ---- AMR projections and the static two-level mesh (aderdg_amr_oracle.c) -- */
+void orc_face_prolongation(const orc_solver* s, double* lQf, double* lFf, const double* lQc, const double* lFc,
+                           int level_coarse, int level_fine, const int* subface);
+void orc_face_restriction(const orc_solver* s, double* lQc, double* lFc, const double* lQf, const double* lFf,
+                          int level_coarse, int level_fine, const int* subface);
+void orc_face_flux_restriction(const orc_solver* s, double* lFc, const double* lFf, const int* subface,
+                               int level_delta);
+void orc_volume_prolongation(const orc_solver* s, double* luhF, const double* luhC, int level_coarse,
+                             int level_fine, const int* subcell);
+void orc_volume_restriction(const orc_solver* s, double* luhC, const double* luhF, int level_coarse,
+                            int level_fine, const int* subcell);
+int orc_amr_nleaves(int n, const unsigned char* refined);
+void orc_amr_leaves(int n, const unsigned char* refined, int* level, int* coords);
+int orc_amr_run(const orc_solver* s, int n, const unsigned char* refined, double* luh, int steps, double* dt_io,
+                int threads);
 
 #ifdef __cplusplus
 }

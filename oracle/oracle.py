@@ -112,6 +112,17 @@ def _lib():
     lib.orc_lin_run.restype = I
     lib.orc_lin_misfit_grad.argtypes = [sp, dp, I, D, D, I, dp, D, D, D, I, dp, dp, dp, I]
     lib.orc_lin_misfit_grad.restype = D
+    u8p = ctypes.POINTER(ctypes.c_ubyte)
+    lib.orc_face_prolongation.argtypes = [sp, dp, dp, dp, dp, I, I, ip]
+    lib.orc_face_restriction.argtypes = [sp, dp, dp, dp, dp, I, I, ip]
+    lib.orc_face_flux_restriction.argtypes = [sp, dp, dp, ip, I]
+    lib.orc_volume_prolongation.argtypes = [sp, dp, dp, I, I, ip]
+    lib.orc_volume_restriction.argtypes = [sp, dp, dp, I, I, ip]
+    lib.orc_amr_nleaves.argtypes = [I, u8p]
+    lib.orc_amr_nleaves.restype = I
+    lib.orc_amr_leaves.argtypes = [I, u8p, ip, ip]
+    lib.orc_amr_run.argtypes = [sp, I, u8p, dp, I, dp, I]
+    lib.orc_amr_run.restype = I
     return lib
 
 
@@ -264,8 +275,77 @@ class Oracle:
         return float(dt_io[0]), pic
 
 
+def _u8(a):
+    assert a.dtype == np.uint8 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
+
+
+def _iv(v):
+    return _ip(np.ascontiguousarray(v, dtype=np.int32))
+
+
 def _opt(a):
     return _p(a) if a is not None else None
+
+
+class AmrOracle(Oracle):
+    """AMR projections (any dimension) and the static two-level 3D mesh."""
+
+    def face_prolongation(self, lQc, lFc, subface, level_coarse=0, level_fine=1):
+        lQf = np.zeros(self.nface * self.nd)
+        lFf = np.zeros(self.nface * self.nvar)
+        sub = np.ascontiguousarray(subface, dtype=np.int32)
+        lib().orc_face_prolongation(self.ps, _p(lQf), _p(lFf), _p(np.ascontiguousarray(lQc, dtype=np.float64)),
+                                    _p(np.ascontiguousarray(lFc, dtype=np.float64)), level_coarse, level_fine, _ip(sub))
+        return lQf, lFf
+
+    def face_restriction(self, lQc, lFc, lQf, lFf, subface, level_coarse=0, level_fine=1):
+        lQc = np.array(lQc, dtype=np.float64)
+        lFc = np.array(lFc, dtype=np.float64)
+        sub = np.ascontiguousarray(subface, dtype=np.int32)
+        lib().orc_face_restriction(self.ps, _p(lQc), _p(lFc), _p(np.ascontiguousarray(lQf, dtype=np.float64)),
+                                   _p(np.ascontiguousarray(lFf, dtype=np.float64)), level_coarse, level_fine, _ip(sub))
+        return lQc, lFc
+
+    def face_flux_restriction(self, lFf, subface, level_delta=1):
+        out = np.zeros(self.nface * self.nvar)
+        sub = np.ascontiguousarray(subface, dtype=np.int32)
+        lib().orc_face_flux_restriction(self.ps, _p(out), _p(np.ascontiguousarray(lFf, dtype=np.float64)), _ip(sub),
+                                        level_delta)
+        return out
+
+    def volume_prolongation(self, luhC, subcell, level_coarse=0, level_fine=1):
+        out = np.zeros(self.nspace * self.nd)
+        sub = np.ascontiguousarray(subcell, dtype=np.int32)
+        lib().orc_volume_prolongation(self.ps, _p(out), _p(np.ascontiguousarray(luhC, dtype=np.float64)),
+                                      level_coarse, level_fine, _ip(sub))
+        return out
+
+    def volume_restriction(self, luhC, luhF, subcell, level_coarse=0, level_fine=1):
+        luhC = np.array(luhC, dtype=np.float64)
+        sub = np.ascontiguousarray(subcell, dtype=np.int32)
+        lib().orc_volume_restriction(self.ps, _p(luhC), _p(np.ascontiguousarray(luhF, dtype=np.float64)),
+                                     level_coarse, level_fine, _ip(sub))
+        return luhC
+
+    @staticmethod
+    def amr_leaves(n, refined):
+        r = np.ascontiguousarray(refined, dtype=np.uint8)
+        L = lib().orc_amr_nleaves(n, _u8(r))
+        level = np.zeros(L, dtype=np.int32)
+        coords = np.zeros((L, 3), dtype=np.int32)
+        lib().orc_amr_leaves(n, _u8(r), _ip(level), _ip(coords))
+        return level, coords
+
+    def amr_run(self, luh, n, refined, steps, dt=0.0, threads=0):
+        """Advance the leaves' state in place; returns the dt of the next step."""
+        assert luh.dtype == np.float64 and luh.flags["C_CONTIGUOUS"]
+        r = np.ascontiguousarray(refined, dtype=np.uint8)
+        dt_io = np.array([dt], dtype=np.float64)
+        rc = lib().orc_amr_run(self.ps, n, _u8(r), _p(luh), steps, _p(dt_io), threads)
+        if rc != 0:
+            raise RuntimeError(f"orc_amr_run failed ({rc})")
+        return float(dt_io[0])
 
 
 class LinearOracle(Oracle):
