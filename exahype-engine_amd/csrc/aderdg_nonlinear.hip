@@ -104,12 +104,15 @@ __device__ __forceinline__ int stride_of(int d) {
 // ---------------------------------------------------------------------------
 // Space-time predictor + volume integral
 // ---------------------------------------------------------------------------
+// One tile (CPB cells) of the predictor; called by predictor_kernel for
+// tile blk.  Threads of invalid cells return at the end, after the last
+// barrier of the tile.
 template <int D, int B, int V, int NP, typename R>
-__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_kernel(
-    long ncells, const double* __restrict__ luh, const double* __restrict__ hcell,
-    const double* __restrict__ dtp, double* __restrict__ lduh, double* __restrict__ lQhbnd,
-    double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, PdeParams P,
-    int maxIter, int adaptive) {
+__device__ __forceinline__ void predictor_tile(
+    long blk, long ncells, const double* __restrict__ luh, const double* __restrict__ hcell,
+    const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
+    double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive) {
   using S = Shape<D, B, V, NP>;
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, NSP = S::NSP;
   constexpr int SLAB = NS * D * V;                       // one flux slab, per cell
@@ -123,14 +126,14 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_kerne
   const int cib = tid / NS;
   const int s = tid - cib * NS;
   const bool inslot = cib < CPB;            // thread maps to a cell slot of this block
-  const long cell = (long)blockIdx.x * CPB + cib;
+  const long cell = blk * CPB + cib;
   const bool valid = inslot && (cell < ncells);
   const int cslot = inslot ? cib : 0;       // only dereferenced by in-slot threads
 
   int z, y, x;
   node_zyx<D, B>(s, z, y, x);
   const double* w = T->w;
-  const double dt = *dtp;
+  const double dt = dtp[(cinst && valid) ? cinst[cell] : 0];
   const double h = valid ? hcell[cell] : 1.0;
   const double invDx = 1.0 / h;
 
@@ -370,6 +373,31 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_kerne
   if (s == 0 && picard) picard[cell] = iters;
 }
 
+// Predictor + volume integral, one workgroup per tile of CPB cells.
+template <int D, int B, int V, int NP, typename R>
+__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_kernel(
+    long ncells, long ntiles, const double* __restrict__ luh, const double* __restrict__ hcell,
+    const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
+    double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive) {
+  predictor_tile<D, B, V, NP, R>(blockIdx.x, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard, T, P,
+                                 maxIter, adaptive);
+}
+
+// Persistent form: a grid smaller than ntiles loops over the tiles.
+template <int D, int B, int V, int NP, typename R>
+__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_persistent_kernel(
+    long ncells, long ntiles, const double* __restrict__ luh, const double* __restrict__ hcell,
+    const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
+    double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive) {
+  for (long blk = blockIdx.x; blk < ntiles; blk += gridDim.x) {
+    predictor_tile<D, B, V, NP, R>(blk, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard, T, P,
+                                   maxIter, adaptive);
+    __syncthreads();  // LDS of this tile fully consumed before the next one
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Rusanov flux, one wavefront per face
 // ---------------------------------------------------------------------------
@@ -387,7 +415,7 @@ __global__ __launch_bounds__(256) void riemann_kernel(long nfaces, const int4* _
   using S = Shape<D, B, V, NP>;
   constexpr int ND = S::ND, NF = S::NF;
   const int lane = threadIdx.x & 63;
-  const long f = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long f = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (f >= nfaces) return;  // uniform per wavefront; no block barriers below
   const int4 fc = faces[f];
   const int d = fc.z;
@@ -456,18 +484,19 @@ __device__ __forceinline__ double block_min(double v, double* sred) {
 }
 
 template <int D, int B, int V, int NP>
-__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_kernel(
-    long ncells, double* __restrict__ luh, const double* __restrict__ lduh, const double* __restrict__ lFhbnd,
-    const double* __restrict__ hcell, const double* __restrict__ dtp, double* __restrict__ dt_next,
-    const DevTables* __restrict__ T, PdeParams P, double cflpnpm, int do_update) {
+__device__ __forceinline__ void corrector_tile(
+    long blk, long ncells, double* __restrict__ luh, const double* __restrict__ lduh,
+    const double* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
+    const int* __restrict__ cinst, double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P,
+    double cflpnpm, int do_update, double* sred) {
   using S = Shape<D, B, V, NP>;
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, THREADS = S::THREADS;
-  __shared__ double sred[THREADS / 64];
   const int tid = threadIdx.x;
   const int cib = tid / NS;
   const int s = tid - cib * NS;
-  const long cell = (long)blockIdx.x * CPB + cib;
+  const long cell = blk * CPB + cib;
   const bool valid = (cib < CPB) && (cell < ncells);
+  const int inst = (cinst && valid) ? cinst[cell] : 0;
   double mydt = DBL_MAX;
   if (valid) {
     int z, y, x;
@@ -478,7 +507,7 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_kerne
 #pragma unroll
     for (int v = 0; v < ND; v++) Q[v] = luh[(cell * NS + s) * ND + v];
     if (do_update) {
-      const double dt = *dtp;
+      const double dt = dtp[inst];
       double du[V];
 #pragma unroll
       for (int v = 0; v < V; v++) du[v] = lduh[(cell * NS + s) * V + v];
@@ -507,8 +536,41 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_kerne
     }
     mydt = node_cfl_dt<D, B, V, NP>(Q, P, 1.0 / h, cflpnpm);
   }
-  const double bmin = block_min<THREADS>(mydt, sred);
-  if (tid == 0 && bmin < DBL_MAX) atomic_min_pos_double(dt_next, bmin);
+  // cells of one instance are contiguous: a tile whose first and last cell
+  // share the instance reduces once, otherwise every cell merges its own dt
+  const long first = blk * CPB, last = (first + CPB - 1 < ncells) ? first + CPB - 1 : ncells - 1;
+  const bool uniform = !cinst || cinst[first] == cinst[last];
+  if (uniform) {
+    const double bmin = block_min<THREADS>(mydt, sred);
+    if (tid == 0 && bmin < DBL_MAX) atomic_min_pos_double(dt_next + (cinst ? cinst[first] : 0), bmin);
+  } else if (valid && mydt < DBL_MAX) {
+    atomic_min_pos_double(dt_next + inst, mydt);
+  }
+}
+
+template <int D, int B, int V, int NP>
+__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_kernel(
+    long ncells, long ntiles, double* __restrict__ luh, const double* __restrict__ lduh,
+    const double* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
+    const int* __restrict__ cinst, double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P,
+    double cflpnpm, int do_update) {
+  __shared__ double sred[Shape<D, B, V, NP>::THREADS / 64];
+  corrector_tile<D, B, V, NP>(blockIdx.x, ncells, luh, lduh, lFhbnd, hcell, dtp, cinst, dt_next, T, P, cflpnpm,
+                              do_update, sred);
+}
+
+template <int D, int B, int V, int NP>
+__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_persistent_kernel(
+    long ncells, long ntiles, double* __restrict__ luh, const double* __restrict__ lduh,
+    const double* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
+    const int* __restrict__ cinst, double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P,
+    double cflpnpm, int do_update) {
+  __shared__ double sred[Shape<D, B, V, NP>::THREADS / 64];
+  for (long blk = blockIdx.x; blk < ntiles; blk += gridDim.x) {
+    corrector_tile<D, B, V, NP>(blk, ncells, luh, lduh, lFhbnd, hcell, dtp, cinst, dt_next, T, P, cflpnpm,
+                                do_update, sred);
+    __syncthreads();
+  }
 }
 
 // Single-cell helpers used by the per-cell compatibility entry points.
@@ -547,11 +609,13 @@ __global__ void solution_update_kernel(double* __restrict__ luh, const double* _
   for (int v = 0; v < V; v++) luh[s * ND + v] = luhOld[s * ND + v] + lduh[s * V + v] * upd;
 }
 
-__global__ void advance_time_kernel(double* state) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    state[0] += state[1];
-    state[1] = state[2];
-    state[2] = DBL_MAX;
+// state[i] = {t, dt, dt_next, -} per instance; strided layout: t[ninst], dt[ninst], dt_next[ninst]
+__global__ void advance_time_kernel(double* state, int ninst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ninst) {
+    state[i] += state[ninst + i];
+    state[ninst + i] = state[2 * ninst + i];
+    state[2 * ninst + i] = DBL_MAX;
   }
 }
 
@@ -566,32 +630,49 @@ __global__ void fill_kernel(double* p, long n, double value) {
 template <int D, int B, int V, int NP>
 struct Launch {
   using S = Shape<D, B, V, NP>;
+  static long grid_of(long tiles, const LaunchOpts& o) { return (o.grid > 0 && o.grid < tiles) ? o.grid : tiles; }
   static int predictor(const KernelCfg& k, hipStream_t st, long n, const double* luh, const double* h,
-                       const double* dt, double* lduh, double* qb, double* fb, int* pic) {
-    const long blocks = (n + S::CPB - 1) / S::CPB;
+                       const double* dt, double* lduh, double* qb, double* fb, int* pic, const LaunchOpts& o) {
+    const long tiles = (n + S::CPB - 1) / S::CPB;
+    const long grid = grid_of(tiles, o);
     const int adaptive = k.picard_max <= 0;
     const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
+    const bool persistent = grid < tiles;
     if (k.precision == 0) {
-      hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, double>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh,
-                         h, dt, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+      if (persistent)
+        hipLaunchKernelGGL((predictor_persistent_kernel<D, B, V, NP, double>), dim3(grid), dim3(S::THREADS), 0, st,
+                           n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+      else
+        hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, double>), dim3(grid), dim3(S::THREADS), 0, st, n, tiles,
+                           luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
     } else {
-      hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, float>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh,
-                         h, dt, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+      if (persistent)
+        hipLaunchKernelGGL((predictor_persistent_kernel<D, B, V, NP, float>), dim3(grid), dim3(S::THREADS), 0, st,
+                           n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+      else
+        hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, float>), dim3(grid), dim3(S::THREADS), 0, st, n, tiles,
+                           luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
     }
     return static_cast<int>(hipGetLastError());
   }
   static int riemann(const KernelCfg& k, hipStream_t st, long nf, const int* faces, const double* qb,
-                     double* fb) {
-    const long blocks = (nf + 3) / 4;
-    hipLaunchKernelGGL((riemann_kernel<D, B, V, NP>), dim3(blocks), dim3(256), 0, st, nf,
+                     double* fb, const LaunchOpts& o) {
+    const int waves = (o.riemann_waves >= 1 && o.riemann_waves <= 16) ? o.riemann_waves : 4;
+    const long blocks = (nf + waves - 1) / waves;
+    hipLaunchKernelGGL((riemann_kernel<D, B, V, NP>), dim3(blocks), dim3(64 * waves), 0, st, nf,
                        reinterpret_cast<const int4*>(faces), qb, fb, k.tab, k.pde);
     return static_cast<int>(hipGetLastError());
   }
   static int corrector(const KernelCfg& k, hipStream_t st, long n, double* luh, double* lduh, const double* fb,
-                       const double* h, const double* dt, double* dt_next, int do_update) {
-    const long blocks = (n + S::CPB - 1) / S::CPB;
-    hipLaunchKernelGGL((corrector_kernel<D, B, V, NP>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, lduh, fb,
-                       h, dt, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update);
+                       const double* h, const double* dt, double* dt_next, int do_update, const LaunchOpts& o) {
+    const long tiles = (n + S::CPB - 1) / S::CPB;
+    const long grid = grid_of(tiles, o);
+    if (grid < tiles)
+      hipLaunchKernelGGL((corrector_persistent_kernel<D, B, V, NP>), dim3(grid), dim3(S::THREADS), 0, st, n, tiles,
+                         luh, lduh, fb, h, dt, o.cinst, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update);
+    else
+      hipLaunchKernelGGL((corrector_kernel<D, B, V, NP>), dim3(grid), dim3(S::THREADS), 0, st, n, tiles, luh, lduh,
+                         fb, h, dt, o.cinst, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update);
     return static_cast<int>(hipGetLastError());
   }
   static int face_integral(const KernelCfg& k, hipStream_t st, double* lduh, const double* F, int d, int o,
@@ -611,10 +692,10 @@ struct Launch {
 struct Entry {
   int D, B, V, NP;
   int (*predictor)(const KernelCfg&, hipStream_t, long, const double*, const double*, const double*, double*,
-                   double*, double*, int*);
-  int (*riemann)(const KernelCfg&, hipStream_t, long, const int*, const double*, double*);
+                   double*, double*, int*, const LaunchOpts&);
+  int (*riemann)(const KernelCfg&, hipStream_t, long, const int*, const double*, double*, const LaunchOpts&);
   int (*corrector)(const KernelCfg&, hipStream_t, long, double*, double*, const double*, const double*,
-                   const double*, double*, int);
+                   const double*, double*, int, const LaunchOpts&);
   int (*face_integral)(const KernelCfg&, hipStream_t, double*, const double*, int, int, double);
   int (*solution_update)(const KernelCfg&, hipStream_t, double*, const double*, const double*, const double*);
 };
@@ -650,7 +731,7 @@ bool shape_supported(int dim, int order, int nvar, int npar) {
 }
 
 int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* luh, const double* h,
-                     const double* dt, double* lduh, double* qb, double* fb, int* pic) {
+                     const double* dt, double* lduh, double* qb, double* fb, int* pic, const LaunchOpts& o) {
   const Entry* e = find_entry(k);
   if (!e) return -1;
   if (n <= 0) return 0;
@@ -658,33 +739,35 @@ int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* lu
   if (lines_supported(k)) {
     const int v = k.pred_variant;
     const bool fp32 = k.precision != 0;
-    if ((v == 0 && fp32) || v == 3) return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, fp32);
-    if (v == 2) return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, false);
+    if ((v == 0 && fp32) || v == 3)
+      return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, fp32, o.cinst);
+    if (v == 2) return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, false, o.cinst);
   }
-  return e->predictor(k, s, n, luh, h, dt, lduh, qb, fb, pic);
+  return e->predictor(k, s, n, luh, h, dt, lduh, qb, fb, pic, o);
 }
 
-int launch_riemann(const KernelCfg& k, hipStream_t s, long nf, const int* faces, const double* qb, double* fb) {
+int launch_riemann(const KernelCfg& k, hipStream_t s, long nf, const int* faces, const double* qb, double* fb,
+                   const LaunchOpts& o) {
   const Entry* e = find_entry(k);
   if (!e) return -1;
   if (nf <= 0) return 0;
-  return e->riemann(k, s, nf, faces, qb, fb);
+  return e->riemann(k, s, nf, faces, qb, fb, o);
 }
 
 int launch_corrector(const KernelCfg& k, hipStream_t s, long n, double* luh, double* lduh, const double* fb,
-                     const double* h, const double* dt, double* dt_next) {
+                     const double* h, const double* dt, double* dt_next, const LaunchOpts& o) {
   const Entry* e = find_entry(k);
   if (!e) return -1;
   if (n <= 0) return 0;
-  return e->corrector(k, s, n, luh, lduh, fb, h, dt, dt_next, 1);
+  return e->corrector(k, s, n, luh, lduh, fb, h, dt, dt_next, 1, o);
 }
 
 int launch_stable_dt(const KernelCfg& k, hipStream_t s, long n, const double* luh, const double* h,
-                     double* dt_out) {
+                     double* dt_out, const LaunchOpts& o) {
   const Entry* e = find_entry(k);
   if (!e) return -1;
   if (n <= 0) return 0;
-  return e->corrector(k, s, n, const_cast<double*>(luh), nullptr, nullptr, h, nullptr, dt_out, 0);
+  return e->corrector(k, s, n, const_cast<double*>(luh), nullptr, nullptr, h, nullptr, dt_out, 0, o);
 }
 
 int launch_face_integral(const KernelCfg& k, hipStream_t s, double* lduh, const double* F, int d, int o,
@@ -701,8 +784,8 @@ int launch_solution_update(const KernelCfg& k, hipStream_t s, double* luh, const
   return e->solution_update(k, s, luh, luhOld, lduh, dt);
 }
 
-int launch_advance_time(hipStream_t s, double* state) {
-  hipLaunchKernelGGL(advance_time_kernel, dim3(1), dim3(64), 0, s, state);
+int launch_advance_time(hipStream_t s, double* state, int ninst) {
+  hipLaunchKernelGGL(advance_time_kernel, dim3((ninst + 255) / 256), dim3(256), 0, s, state, ninst);
   return static_cast<int>(hipGetLastError());
 }
 

@@ -2,6 +2,7 @@
 // batched device entry points.  See include/exahype_amd.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstring>
 #include <string>
@@ -160,6 +161,7 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
     h.Pt[i] = c->tab.iK1[i] * c->tab.w[i % B];
     h.Ptf[i] = static_cast<float>(h.Pt[i]);
   }
+  for (int i = 0; i < 3 * B * B; i++) h.fgp[i] = c->tab.fgp[i];
   hipError_t e = hipMalloc(&c->d_tab, sizeof(exa::DevTables));
   if (e == hipSuccess) e = hipMemcpy(c->d_tab, &h, sizeof(h), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -338,7 +340,7 @@ int exa_riemannSolver(exa_ctx* c, double* FL, double* FR, const double* QL, cons
 }
 
 int exa_faceIntegral(exa_ctx* c, double* out, const double* lFhbnd, int direction, int orientation,
-                     int levelDelta, const double* dx, double dt, int addToUpdate) {
+                     const int* subfaceIndex, int levelDelta, const double* dx, double dt, int addToUpdate) {
   if (!c || !out || !lFhbnd || !dx) {
     set_error("exa_faceIntegral: NULL argument");
     return EXA_ERR_ARG;
@@ -347,19 +349,54 @@ int exa_faceIntegral(exa_ctx* c, double* out, const double* lFhbnd, int directio
     set_error("exa_faceIntegral: direction/orientation out of range");
     return EXA_ERR_ARG;
   }
-  if (levelDelta != 0) {
-    set_error("exa_faceIntegral: levelDelta > 0 (coarse-fine faces) is not supported by this entry point");
+  if (levelDelta < 0 || levelDelta > 8 || (levelDelta > 0 && !subfaceIndex)) {
+    set_error("exa_faceIntegral: levelDelta out of range or subfaceIndex missing");
+    return EXA_ERR_ARG;
+  }
+  if (c->k.pde.id == EXA_PDE_ACOUSTIC) {
+    set_error("exa_faceIntegral: the linear (acoustic) per-cell path is not compiled into this entry point");
     return EXA_ERR_UNSUPPORTED;
   }
   const int NS = nspace(c), NF = nface(c), ND = ndata(c), V = c->k.nvar;
-  const size_t total = size_t(NS) * ND + size_t(NS) * V + size_t(NF) * V + 1;
+  const size_t total = size_t(NS) * ND + size_t(NS) * V + 3 * size_t(NF) * V + 1;
   int rc = ensure_scratch(c, total * sizeof(double));
   if (rc) return rc;
   double* d_luh = c->d_scratch;
   double* d_lduh = d_luh + size_t(NS) * ND;
   double* d_F = d_lduh + size_t(NS) * V;
-  double* d_dt = d_F + size_t(NF) * V;
+  double* d_t1 = d_F + size_t(NF) * V;
+  double* d_t2 = d_t1 + size_t(NF) * V;
+  double* d_dt = d_t2 + size_t(NF) * V;
   HIP_OK(hipMemcpyAsync(d_F, lFhbnd, NF * V * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy F");
+  if (levelDelta > 0) {
+    // faceIntegral glue with levelDelta > 0
+    // (AbstractGenericADERDGSolverImplementation.template:265-270): restrict the
+    // fine flux over levelDelta levels (faceUnknownsRestriction, lowest
+    // significance first) and integrate it with the coarse cell size into out.
+    int cur[2] = {subfaceIndex[0], c->k.dim == 3 ? subfaceIndex[1] : 0};
+    const double* src = d_F;
+    double* dst = d_t1;
+    for (int l = 1; l <= levelDelta; l++) {
+      const int s0 = cur[0] % 3, s1 = cur[1] % 3;
+      cur[0] /= 3;
+      cur[1] /= 3;
+      HIP_OK(hipMemsetAsync(dst, 0, NF * V * sizeof(double), c->stream), "zero");
+      rc = launch_status(exa::launch_face_project(c->k, c->stream, V, src, dst, s0, s1, 1), "face restriction");
+      if (rc) return rc;
+      src = dst;
+      dst = (dst == d_t1) ? d_t2 : d_t1;
+    }
+    double scale = 1.0;
+    for (int l = 0; l < levelDelta; l++) scale *= 3.0;
+    HIP_OK(hipMemcpyAsync(d_lduh, out, NS * V * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy lduh");
+    rc = launch_status(exa::launch_face_integral(c->k, c->stream, d_lduh, src, direction, orientation,
+                                                 1.0 / (scale * dx[direction])),
+                       "face integral");
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(out, d_lduh, NS * V * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy lduh");
+    HIP_OK(hipStreamSynchronize(c->stream), "sync");
+    return EXA_OK;
+  }
   if (addToUpdate) {
     HIP_OK(hipMemcpyAsync(d_lduh, out, NS * V * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy lduh");
   } else {
@@ -390,7 +427,7 @@ int exa_surfaceIntegral(exa_ctx* c, double* lduh, const double* lFhbnd, const do
   const int NF = nface(c), V = c->k.nvar, D = c->k.dim;
   for (int d = 0; d < D; d++)
     for (int o = 0; o < 2; o++) {
-      const int rc = exa_faceIntegral(c, lduh, lFhbnd + size_t(2 * d + o) * NF * V, d, o, 0, dx, 0.0, 1);
+      const int rc = exa_faceIntegral(c, lduh, lFhbnd + size_t(2 * d + o) * NF * V, d, o, nullptr, 0, dx, 0.0, 1);
       if (rc) return rc;
     }
   return EXA_OK;
@@ -454,6 +491,114 @@ int exa_stableTimeStepSize(exa_ctx* c, const double* luh, const double* dx, doub
 // ---------------------------------------------------------------------------
 // Batched device entry points
 // ---------------------------------------------------------------------------
+
+// ---- AMR projections (per cell), generic/c/{2d,3d}/amrRoutines.cpph ------------
+namespace {
+
+// Multi-level projection of one face (K = dim-1) or volume (K = dim) array
+// of nv values per node.  Prolongation decodes sub from the most significant
+// base-3 digit (amrRoutines.cpph:69-90), restriction from the least (:196-224).
+int project_levels(exa_ctx* c, bool face, int nv, const double* h_in, double* h_out, const int* sub, int delta,
+                   bool prolong) {
+  const int K = face ? c->k.dim - 1 : c->k.dim;
+  const int n = (face ? nface(c) : nspace(c)) * nv;
+  int rc = ensure_scratch(c, 3 * size_t(n) * sizeof(double));
+  if (rc) return rc;
+  double* a = c->d_scratch;
+  double* b = a + n;
+  HIP_OK(hipMemcpyAsync(a, h_in, n * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy in");
+  int cur[3] = {sub[0], K > 1 ? sub[1] : 0, K > 2 ? sub[2] : 0};
+  for (int l = 1; l <= delta; l++) {
+    int si[3] = {0, 0, 0};
+    if (prolong) {
+      int sig = 1;
+      for (int q = 0; q < delta - l; q++) sig *= 3;
+      for (int k = 0; k < K; k++) {
+        si[k] = cur[k] / sig;
+        cur[k] %= sig;
+      }
+    } else {
+      for (int k = 0; k < K; k++) {
+        si[k] = cur[k] % 3;
+        cur[k] /= 3;
+      }
+      HIP_OK(hipMemsetAsync(b, 0, n * sizeof(double), c->stream), "zero");
+    }
+    const int mode = prolong ? 0 : 1;
+    rc = face ? exa::launch_face_project(c->k, c->stream, nv, a, b, si[0], si[1], mode)
+              : exa::launch_volume_project(c->k, c->stream, nv, a, b, si[0], si[1], si[2], mode);
+    rc = launch_status(rc, "projection");
+    if (rc) return rc;
+    std::swap(a, b);
+  }
+  if (prolong) {
+    HIP_OK(hipMemcpyAsync(h_out, a, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy out");
+    HIP_OK(hipStreamSynchronize(c->stream), "sync");
+  } else {
+    std::vector<double> tmp(n);
+    HIP_OK(hipMemcpyAsync(tmp.data(), a, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy out");
+    HIP_OK(hipStreamSynchronize(c->stream), "sync");
+    for (int i = 0; i < n; i++) h_out[i] += tmp[i];  // the reference accumulates into the coarse array
+  }
+  return EXA_OK;
+}
+
+int check_levels(exa_ctx* c, const void* a, const void* b, const void* d, const int* sub, int lc, int lf,
+                 const char* what) {
+  if (!c || !a || !b || !d || !sub) {
+    set_error(std::string(what) + ": NULL argument");
+    return EXA_ERR_ARG;
+  }
+  if (lf <= lc || lf - lc > 8) {
+    set_error(std::string(what) + ": fineGridLevel must exceed coarseGridLevel (by at most 8)");
+    return EXA_ERR_ARG;
+  }
+  return EXA_OK;
+}
+
+}  // namespace
+
+int exa_faceUnknownsProlongation(exa_ctx* c, double* lQhbndFine, double* lFhbndFine, const double* lQhbndCoarse,
+                                 const double* lFhbndCoarse, int coarseGridLevel, int fineGridLevel,
+                                 const int* subfaceIndex) {
+  int rc = check_levels(c, lQhbndFine, lFhbndFine, lQhbndCoarse, subfaceIndex, coarseGridLevel, fineGridLevel,
+                        "exa_faceUnknownsProlongation");
+  if (rc) return rc;
+  if (!lFhbndCoarse) return check_levels(c, nullptr, nullptr, nullptr, nullptr, 0, 0, "exa_faceUnknownsProlongation");
+  const int delta = fineGridLevel - coarseGridLevel;
+  rc = project_levels(c, true, ndata(c), lQhbndCoarse, lQhbndFine, subfaceIndex, delta, true);
+  if (!rc) rc = project_levels(c, true, c->k.nvar, lFhbndCoarse, lFhbndFine, subfaceIndex, delta, true);
+  return rc;
+}
+
+int exa_faceUnknownsRestriction(exa_ctx* c, double* lQhbndCoarse, double* lFhbndCoarse, const double* lQhbndFine,
+                                const double* lFhbndFine, int coarseGridLevel, int fineGridLevel,
+                                const int* subfaceIndex) {
+  int rc = check_levels(c, lQhbndCoarse, lFhbndCoarse, lQhbndFine, subfaceIndex, coarseGridLevel, fineGridLevel,
+                        "exa_faceUnknownsRestriction");
+  if (rc) return rc;
+  if (!lFhbndFine) return check_levels(c, nullptr, nullptr, nullptr, nullptr, 0, 0, "exa_faceUnknownsRestriction");
+  const int delta = fineGridLevel - coarseGridLevel;
+  rc = project_levels(c, true, ndata(c), lQhbndFine, lQhbndCoarse, subfaceIndex, delta, false);
+  if (!rc) rc = project_levels(c, true, c->k.nvar, lFhbndFine, lFhbndCoarse, subfaceIndex, delta, false);
+  return rc;
+}
+
+int exa_volumeUnknownsProlongation(exa_ctx* c, double* luhFine, const double* luhCoarse, int coarseGridLevel,
+                                   int fineGridLevel, const int* subcellIndex) {
+  const int rc = check_levels(c, luhFine, luhCoarse, luhCoarse, subcellIndex, coarseGridLevel, fineGridLevel,
+                              "exa_volumeUnknownsProlongation");
+  if (rc) return rc;
+  return project_levels(c, false, ndata(c), luhCoarse, luhFine, subcellIndex, fineGridLevel - coarseGridLevel, true);
+}
+
+int exa_volumeUnknownsRestriction(exa_ctx* c, double* luhCoarse, const double* luhFine, int coarseGridLevel,
+                                  int fineGridLevel, const int* subcellIndex) {
+  const int rc = check_levels(c, luhCoarse, luhFine, luhFine, subcellIndex, coarseGridLevel, fineGridLevel,
+                              "exa_volumeUnknownsRestriction");
+  if (rc) return rc;
+  return project_levels(c, false, ndata(c), luhFine, luhCoarse, subcellIndex, fineGridLevel - coarseGridLevel, false);
+}
 
 int exa_batch_predictor(exa_ctx* c, void* stream, long ncells, const double* d_luh, const double* d_h,
                         const double* d_dt, double* d_lduh, double* d_lQhbnd, double* d_lFhbnd, int* d_picard) {

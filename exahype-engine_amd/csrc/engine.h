@@ -29,6 +29,7 @@ struct DevTables {
   float dudxf[kMaxBasis * kMaxBasis];
   double Pt[kMaxBasis * kMaxBasis];  // Pt[i][j] = iK1[i][j] * w[j]
   float Ptf[kMaxBasis * kMaxBasis];
+  double fgp[3 * kMaxBasis * kMaxBasis];  // fineGridProjector[sub][n][m] = phi_n((sub + x_m) / 3), [(sub*B+n)*B+m]
 };
 
 struct KernelCfg {
@@ -41,16 +42,30 @@ struct KernelCfg {
   int pred_variant;      // 0 auto, 1 node-column, 2 line (VALU), 3 line (FP32 matrix cores)
 };
 
+// Launch options of the batched kernels.
+//   cinst          per-cell instance index (nullable): dt[cinst[c]], dt_next[cinst[c]];
+//                  cells of one instance must be contiguous
+//   grid           > 0: persistent launch with this many workgroups looping over the
+//                  cell tiles (predictor, corrector); 0: one workgroup per tile
+//   riemann_waves  faces (wavefronts) per Riemann workgroup, 1..16
+struct LaunchOpts {
+  const int* cinst = nullptr;
+  long grid = 0;
+  int riemann_waves = 4;
+};
+
 // Launchers (return hipError_t as int; 0 on success, -1 for an unsupported
 // (dim, order, nvar, npar, precision) combination).
 int launch_predictor(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
-                     const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard);
+                     const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard,
+                     const LaunchOpts& o = LaunchOpts());
 int launch_riemann(const KernelCfg& k, hipStream_t s, long nfaces, const int* faces, const double* lQhbnd,
-                   double* lFhbnd);
+                   double* lFhbnd, const LaunchOpts& o = LaunchOpts());
 int launch_corrector(const KernelCfg& k, hipStream_t s, long ncells, double* luh, double* lduh,
-                     const double* lFhbnd, const double* h, const double* dt, double* dt_next);
+                     const double* lFhbnd, const double* h, const double* dt, double* dt_next,
+                     const LaunchOpts& o = LaunchOpts());
 int launch_stable_dt(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
-                     double* dt_out);
+                     double* dt_out, const LaunchOpts& o = LaunchOpts());
 // Single-cell helper kernels for the compatibility entry points.
 int launch_face_integral(const KernelCfg& k, hipStream_t s, double* lduh, const double* lFface, int direction,
                          int orientation, double invdx_scaled);
@@ -60,13 +75,30 @@ int launch_solution_update(const KernelCfg& k, hipStream_t s, double* luh, const
 bool lines_supported(const KernelCfg& k);
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
                              const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard,
-                             bool mfma);
-// Time bookkeeping of the device step: t += dt; dt = dt_next; dt_next = DBL_MAX.
-int launch_advance_time(hipStream_t s, double* state);
+                             bool mfma, const int* cinst = nullptr);
+// Time bookkeeping of the device step per instance, state = t[ninst], dt[ninst],
+// dt_next[ninst]: t += dt; dt = dt_next; dt_next = DBL_MAX.
+int launch_advance_time(hipStream_t s, double* state, int ninst = 1);
 // Initial condition EXA_IC_DENSITY_WAVE on a uniform n^D mesh (initial.hip).
 int launch_density_wave(hipStream_t s, long ncells, int n, int D, int B, int ND, double h, const double* d_xi,
                         const double* d_prm, double* luh);
 int launch_fill(hipStream_t s, double* p, long n, double value);
+
+// ---- AMR projections (amr.hip), single level, 2D or 3D ---------------------
+// Face prolongation: per entry {coarse cell, face, subface s0 + 3 s1, target
+// cell}: lQhbnd / lFhbnd of the target's face = the coarse face prolongated
+// (overwrite).  Face restriction: per entry {coarse cell, face, first of
+// 3^(D-1) consecutive source cells in subface order, -}: the coarse cell's
+// lFhbnd on that face = sum of the sources' restricted fluxes (overwrite).
+int launch_face_prolong(const KernelCfg& k, hipStream_t s, long n, const int* entries, double* qb, double* fb);
+int launch_face_restrict(const KernelCfg& k, hipStream_t s, long n, const int* entries, double* fb);
+// Single-level projections of explicit arrays (compatibility entry points):
+// face arrays [B^(D-1)][nv], volume arrays [B^D][nv]; sub[k] in 0..2.
+// mode 0: out = P(in) (prolongation); mode 1: out += R(in) (restriction).
+int launch_face_project(const KernelCfg& k, hipStream_t s, int nv, const double* in, double* out, int s0, int s1,
+                        int mode);
+int launch_volume_project(const KernelCfg& k, hipStream_t s, int nv, const double* in, double* out, int s0, int s1,
+                          int s2, int mode);
 
 // ---- linear acoustics (acoustic.hip) ---------------------------------------
 // Device arrays of the acoustic mesh: u[cell][node][4] (p,u,v,w), mat[cell][2]

@@ -128,8 +128,9 @@ __device__ __forceinline__ void euler_flux_dir(const R* Q, R irho, R p, int d, R
 template <int B, typename R, bool MF>
 __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel(
     long ncells, const double* __restrict__ luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
-    double* __restrict__ lduh, double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
-    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive) {
+    const int* __restrict__ cinst, double* __restrict__ lduh, double* __restrict__ lQhbnd,
+    double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, PdeParams P,
+    int maxIter, int adaptive) {
   using S = LShape<B>;
   constexpr int V = S::V, NS = S::NS, NF = S::NF, TILE = S::TILE;
   static_assert(S::CPB == 1, "one cell per workgroup");
@@ -147,7 +148,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
   const long cell = blockIdx.x;
   const int t = l / (B * B), z = (l / B) % B, y = l % B;
 
-  const double dt = *dtp;
+  const double dt = dtp[cinst ? cinst[cell] : 0];
   const double h = hcell[cell];
   const double* w = T->w;
   // operators (uniform across lanes; read through the scalar cache at use):
@@ -453,7 +454,8 @@ bool lines_supported(const KernelCfg& k) {
 }
 
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const double* luh, const double* h,
-                             const double* dt, double* lduh, double* qb, double* fb, int* pic, bool mfma) {
+                             const double* dt, double* lduh, double* qb, double* fb, int* pic, bool mfma,
+                             const int* cinst) {
   constexpr int B = 6;
   using S = LShape<B>;
   const long blocks = (n + S::CPB - 1) / S::CPB;
@@ -461,14 +463,14 @@ int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const d
   const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
   if (k.precision == 0) {
     hipLaunchKernelGGL((predictor3d_lines_kernel<B, double, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h, dt,
-                       lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+                       cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
   } else {
     if (mfma)
       hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h,
-                         dt, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+                         dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
     else
       hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh,
-                         h, dt, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+                         h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
   }
   return static_cast<int>(hipGetLastError());
 }

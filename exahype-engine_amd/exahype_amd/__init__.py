@@ -59,7 +59,7 @@ _sig("exa_fusedSpaceTimePredictorVolumeIntegral",
      [_cp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ip])
 _sig("exa_riemannSolver", [_cp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int,
                            ctypes.c_int, ctypes.c_int])
-_sig("exa_faceIntegral", [_cp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
+_sig("exa_faceIntegral", [_cp, _dp, _dp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int, _dp, ctypes.c_double,
                           ctypes.c_int])
 _sig("exa_surfaceIntegral", [_cp, _dp, _dp, _dp])
 _sig("exa_addUpdateToSolution", [_cp, _dp, _dp, _dp, ctypes.c_double])
@@ -85,6 +85,36 @@ _sig("exa_mesh_stream", [_cp], ctypes.c_void_p)
 _sig("exa_mesh_set_timing", [_cp, ctypes.c_int])
 _sig("exa_mesh_phase_times", [_cp, _dp, _dp, _dp, _ip])
 _I, _D = ctypes.c_int, ctypes.c_double
+_u8p = ctypes.POINTER(ctypes.c_ubyte)
+_sig("exa_faceUnknownsProlongation", [_cp, _dp, _dp, _dp, _dp, _I, _I, _ip])
+_sig("exa_faceUnknownsRestriction", [_cp, _dp, _dp, _dp, _dp, _I, _I, _ip])
+_sig("exa_volumeUnknownsProlongation", [_cp, _dp, _dp, _I, _I, _ip])
+_sig("exa_volumeUnknownsRestriction", [_cp, _dp, _dp, _I, _I, _ip])
+_sig("exa_amr_create", [ctypes.POINTER(_cp), _cp, _I, _u8p])
+_sig("exa_amr_destroy", [_cp])
+_sig("exa_amr_nleaves", [_cp], ctypes.c_long)
+_sig("exa_amr_leaves", [_cp, _ip, _ip])
+_sig("exa_amr_set_solution", [_cp, _dp])
+_sig("exa_amr_get_solution", [_cp, _dp])
+_sig("exa_amr_init_dt", [_cp, _D])
+_sig("exa_amr_run", [_cp, _I, _I])
+_sig("exa_amr_sync", [_cp])
+_sig("exa_amr_time", [_cp, _dp, _dp])
+_sig("exa_amr_autotune", [_cp, _ip, _dp])
+_sig("exa_ensemble_create", [ctypes.POINTER(_cp), _cp, _I, _ip])
+_sig("exa_ensemble_destroy", [_cp])
+_sig("exa_ensemble_ncells", [_cp, _I], ctypes.c_long)
+_sig("exa_ensemble_set_solution", [_cp, _I, _dp])
+_sig("exa_ensemble_get_solution", [_cp, _I, _dp])
+_sig("exa_ensemble_init_dt", [_cp, _D])
+_sig("exa_ensemble_run", [_cp, _I, _I])
+_sig("exa_ensemble_sync", [_cp])
+_sig("exa_ensemble_time", [_cp, _I, _dp, _dp])
+_sig("exa_ensemble_autotune", [_cp, _ip, _dp])
+_sig("exa_group_create", [ctypes.POINTER(_cp), _I, ctypes.POINTER(_cp), _I, ctypes.POINTER(_cp)])
+_sig("exa_group_destroy", [_cp])
+_sig("exa_group_run", [_cp, _I, _I])
+_sig("exa_group_reset", [_cp])
 _sig("exa_acoustic_create", [ctypes.POINTER(_cp), _cp, _I, _D])
 _sig("exa_acoustic_destroy", [_cp])
 _sig("exa_acoustic_ncells", [_cp], ctypes.c_long)
@@ -198,11 +228,51 @@ class Context:
                                       _p(np.ascontiguousarray(QR, dtype=np.float64)), t, dt, _p(dxa), direction, 0, 0))
         return FL, FR
 
-    def faceIntegral(self, out, lFhbnd_face, direction, orientation, dx, dt=0.0, addToUpdate=True):
+    def faceIntegral(self, out, lFhbnd_face, direction, orientation, dx, dt=0.0, addToUpdate=True,
+                     subfaceIndex=None, levelDelta=0):
         out = np.array(out, dtype=np.float64)
+        sub = None if subfaceIndex is None else np.ascontiguousarray(subfaceIndex, dtype=np.int32)
         _check(_lib.exa_faceIntegral(self.h, _p(out), _p(np.ascontiguousarray(lFhbnd_face, dtype=np.float64)),
-                                     direction, orientation, 0, _p(np.array(dx, dtype=np.float64)), dt,
-                                     int(bool(addToUpdate))))
+                                     direction, orientation, _ip(sub), levelDelta,
+                                     _p(np.array(dx, dtype=np.float64)), dt, int(bool(addToUpdate))))
+        return out
+
+    # AMR projections (reference amrRoutines.cpph); prolongations return the
+    # fine arrays, restrictions return coarse + restricted fine
+    def faceUnknownsProlongation(self, lQhbndCoarse, lFhbndCoarse, subfaceIndex, coarseGridLevel=0,
+                                 fineGridLevel=1):
+        qf = np.zeros(self.nface * self.nd)
+        ff = np.zeros(self.nface * self.nvar)
+        sub = np.ascontiguousarray(subfaceIndex, dtype=np.int32)
+        _check(_lib.exa_faceUnknownsProlongation(self.h, _p(qf), _p(ff),
+                                                 _p(np.ascontiguousarray(lQhbndCoarse, dtype=np.float64)),
+                                                 _p(np.ascontiguousarray(lFhbndCoarse, dtype=np.float64)),
+                                                 coarseGridLevel, fineGridLevel, _ip(sub)))
+        return qf, ff
+
+    def faceUnknownsRestriction(self, lQhbndCoarse, lFhbndCoarse, lQhbndFine, lFhbndFine, subfaceIndex,
+                                coarseGridLevel=0, fineGridLevel=1):
+        qc = np.array(lQhbndCoarse, dtype=np.float64)
+        fc = np.array(lFhbndCoarse, dtype=np.float64)
+        sub = np.ascontiguousarray(subfaceIndex, dtype=np.int32)
+        _check(_lib.exa_faceUnknownsRestriction(self.h, _p(qc), _p(fc),
+                                                _p(np.ascontiguousarray(lQhbndFine, dtype=np.float64)),
+                                                _p(np.ascontiguousarray(lFhbndFine, dtype=np.float64)),
+                                                coarseGridLevel, fineGridLevel, _ip(sub)))
+        return qc, fc
+
+    def volumeUnknownsProlongation(self, luhCoarse, subcellIndex, coarseGridLevel=0, fineGridLevel=1):
+        out = np.zeros(self.nspace * self.nd)
+        sub = np.ascontiguousarray(subcellIndex, dtype=np.int32)
+        _check(_lib.exa_volumeUnknownsProlongation(self.h, _p(out), _p(np.ascontiguousarray(luhCoarse, dtype=np.float64)),
+                                                   coarseGridLevel, fineGridLevel, _ip(sub)))
+        return out
+
+    def volumeUnknownsRestriction(self, luhCoarse, luhFine, subcellIndex, coarseGridLevel=0, fineGridLevel=1):
+        out = np.array(luhCoarse, dtype=np.float64)
+        sub = np.ascontiguousarray(subcellIndex, dtype=np.int32)
+        _check(_lib.exa_volumeUnknownsRestriction(self.h, _p(out), _p(np.ascontiguousarray(luhFine, dtype=np.float64)),
+                                                  coarseGridLevel, fineGridLevel, _ip(sub)))
         return out
 
     def surfaceIntegral(self, lduh, lFhbnd, dx):
@@ -304,6 +374,161 @@ class Mesh:
         a, b, c, n = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
         _check(_lib.exa_mesh_phase_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(n)))
         return dict(predictor_ms=a.value, riemann_ms=b.value, corrector_ms=c.value, steps=n.value)
+
+
+def _ipv(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+class Amr:
+    """Static two-level 3D mesh (refined coarse cells split into 3^3 children)."""
+
+    def __init__(self, ctx, n, refined):
+        self.refined = np.ascontiguousarray(refined, dtype=np.uint8)
+        assert self.refined.size == n ** 3
+        m = _cp()
+        _check(_lib.exa_amr_create(ctypes.byref(m), ctx.h, n, self.refined.ctypes.data_as(_u8p)))
+        self.h = m
+        self.ctx = ctx
+        self.n = n
+        self.nleaves = _lib.exa_amr_nleaves(m)
+
+    def close(self):
+        if self.h:
+            _lib.exa_amr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def leaves(self):
+        level = np.zeros(self.nleaves, dtype=np.int32)
+        coords = np.zeros((self.nleaves, 3), dtype=np.int32)
+        _check(_lib.exa_amr_leaves(self.h, _ipv(level), _ipv(coords)))
+        return level, coords
+
+    def set_solution(self, luh):
+        luh = np.ascontiguousarray(luh, dtype=np.float64)
+        assert luh.size == self.nleaves * self.ctx.nspace * self.ctx.nd
+        _check(_lib.exa_amr_set_solution(self.h, _p(luh)))
+
+    def get_solution(self):
+        out = np.zeros(self.nleaves * self.ctx.nspace * self.ctx.nd)
+        _check(_lib.exa_amr_get_solution(self.h, _p(out)))
+        return out
+
+    def init_dt(self, dt=0.0):
+        _check(_lib.exa_amr_init_dt(self.h, dt))
+
+    def run(self, nsteps, use_graph=False):
+        _check(_lib.exa_amr_run(self.h, nsteps, 1 if use_graph else 0))
+
+    def sync(self):
+        _check(_lib.exa_amr_sync(self.h))
+
+    def time(self):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        _check(_lib.exa_amr_time(self.h, ctypes.byref(t), ctypes.byref(dt)))
+        return t.value, dt.value
+
+    def autotune(self):
+        cfg = np.zeros(3, dtype=np.int32)
+        ms = np.zeros(3)
+        _check(_lib.exa_amr_autotune(self.h, _ipv(cfg), _p(ms)))
+        return dict(predictor_grid=int(cfg[0]), riemann_waves=int(cfg[1]), corrector_grid=int(cfg[2]),
+                    ms=ms.tolist())
+
+
+class Ensemble:
+    """Independent uniform periodic meshes of n[i] cells per side, stepped together."""
+
+    def __init__(self, ctx, n):
+        self.sizes = np.ascontiguousarray(n, dtype=np.int32)
+        m = _cp()
+        _check(_lib.exa_ensemble_create(ctypes.byref(m), ctx.h, len(self.sizes), _ipv(self.sizes)))
+        self.h = m
+        self.ctx = ctx
+        self.ninst = len(self.sizes)
+
+    def close(self):
+        if self.h:
+            _lib.exa_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ncells(self, inst=-1):
+        return _lib.exa_ensemble_ncells(self.h, inst)
+
+    def set_solution(self, inst, luh):
+        luh = np.ascontiguousarray(luh, dtype=np.float64)
+        assert luh.size == self.ncells(inst) * self.ctx.nspace * self.ctx.nd
+        _check(_lib.exa_ensemble_set_solution(self.h, inst, _p(luh)))
+
+    def get_solution(self, inst):
+        out = np.zeros(self.ncells(inst) * self.ctx.nspace * self.ctx.nd)
+        _check(_lib.exa_ensemble_get_solution(self.h, inst, _p(out)))
+        return out
+
+    def init_dt(self, dt=0.0):
+        _check(_lib.exa_ensemble_init_dt(self.h, dt))
+
+    def run(self, nsteps, use_graph=False):
+        _check(_lib.exa_ensemble_run(self.h, nsteps, 1 if use_graph else 0))
+
+    def sync(self):
+        _check(_lib.exa_ensemble_sync(self.h))
+
+    def time(self, inst):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        _check(_lib.exa_ensemble_time(self.h, inst, ctypes.byref(t), ctypes.byref(dt)))
+        return t.value, dt.value
+
+    def autotune(self):
+        cfg = np.zeros(3, dtype=np.int32)
+        ms = np.zeros(3)
+        _check(_lib.exa_ensemble_autotune(self.h, _ipv(cfg), _p(ms)))
+        return dict(predictor_grid=int(cfg[0]), riemann_waves=int(cfg[1]), corrector_grid=int(cfg[2]),
+                    ms=ms.tolist())
+
+
+class Group:
+    """Two-level meshes and ensembles stepped by one captured graph per step."""
+
+    def __init__(self, amrs=(), ensembles=()):
+        self.members = list(amrs) + list(ensembles)  # keep them alive
+        A = (_cp * max(len(amrs), 1))(*[a.h for a in amrs])
+        En = (_cp * max(len(ensembles), 1))(*[e.h for e in ensembles])
+        g = _cp()
+        _check(_lib.exa_group_create(ctypes.byref(g), len(amrs), A, len(ensembles), En))
+        self.h = g
+
+    def close(self):
+        if self.h:
+            _lib.exa_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps, use_graph=True):
+        _check(_lib.exa_group_run(self.h, nsteps, 1 if use_graph else 0))
+
+    def reset(self):
+        _check(_lib.exa_group_reset(self.h))
 
 
 class Acoustic:

@@ -9,6 +9,10 @@ luh[cell][z][y][x][var], cells x fastest, unit cube / square, n cells per side.
 * density_wave_3d  -- 3D Euler, rho = 1 + 0.2 sin(2 pi (x + y + z)) + sum of
                       M seeded Fourier modes of amplitude 0.01/M (|k_i| <= 4) in
                       rho and in the x-velocity u; v = w = 1, p = 1
+* amr_* / ensemble_* -- batched uneven work: a static two-level 3D mesh (27^3
+                      coarse cells, those with centre within 0.25 of the cube
+                      centre split 3x) and 64 independent 2D meshes with seeded
+                      sizes between 9 and 81 cells per side
 * acoustic_*       -- 3D linear acoustics: per-cell wave speed
                       c = 1 + sum of 8 seeded Gaussian bumps (+-0.2, width 0.1),
                       rho = 1, a Ricker point source (f0 = 5) and 64 seeded
@@ -167,3 +171,38 @@ def acoustic_state(n, order, c, rho=None, u=None):
     luh[..., 4] = (np.ones(nc) if rho is None else np.asarray(rho))[:, None]
     luh[..., 5] = np.asarray(c)[:, None]
     return luh
+
+
+# ---- batched uneven work ---------------------------------------------------------
+def amr_ball_refinement(n=27, radius=0.25, centre=(0.5, 0.5, 0.5)):
+    """refined[c] = 1 for coarse cells (x fastest) whose centre lies within radius."""
+    ar = (np.arange(n) + 0.5) / n
+    Z, Y, X = np.meshgrid(ar, ar, ar, indexing="ij")
+    d2 = (X - centre[0]) ** 2 + (Y - centre[1]) ** 2 + (Z - centre[2]) ** 2
+    return (d2.ravel() <= radius ** 2).astype(np.uint8)
+
+
+def amr_leaf_points(order, n, level, coords):
+    """Node coordinates [nleaves][B^3][3] of the leaves of a two-level mesh."""
+    x = _nodes(order)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    loc = np.stack([X.ravel(), Y.ravel(), Z.ravel()], -1)
+    h = np.where(np.asarray(level) == 0, 1.0 / n, 1.0 / (3 * n))
+    return (np.asarray(coords)[:, None, :] + loc[None]) * h[:, None, None]
+
+
+def euler_state(rho, u, v, w, p):
+    return np.stack([rho, rho * u, rho * v, rho * w, p / (GAMMA - 1) + 0.5 * rho * (u * u + v * v + w * w)], -1)
+
+
+def amr_density_wave(order, n, level, coords, amp=0.2):
+    """3D density wave rho = 1 + amp sin(2 pi (x+y+z)), u = v = w = 1, p = 1 on the leaves."""
+    pts = amr_leaf_points(order, n, level, coords)
+    rho = 1.0 + amp * np.sin(2 * np.pi * (pts[..., 0] + pts[..., 1] + pts[..., 2]))
+    one = np.ones_like(rho)
+    return np.ascontiguousarray(euler_state(rho, one, one, one, one))
+
+
+def ensemble_sizes(seed=0, count=64, lo=9, hi=81):
+    """Seeded cells per side of the independent 2D instances."""
+    return np.random.default_rng(seed).integers(lo, hi + 1, size=count).astype(np.int32)

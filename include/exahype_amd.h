@@ -120,11 +120,15 @@ int exa_riemannSolver(exa_ctx* ctx, double* FL, double* FR, const double* QL, co
                       double dt, const double* dx, int direction, int isBoundaryFace, int faceIndex);
 
 /* faceIntegral (…Implementation.template:247-285 -> generic/c/3d/faceIntegralNonlinear.cpph:21-103).
- * Conforming faces only (levelDelta == 0).  addToUpdate != 0: out is lduh
- * and the face integral is accumulated into it; addToUpdate == 0: out is luh
- * and out += dt/w * faceIntegral (as the reference glue). */
+ * levelDelta == 0: addToUpdate != 0: out is lduh and the face integral is
+ * accumulated into it; addToUpdate == 0: out is luh and out += dt/w *
+ * faceIntegral (as the reference glue).  levelDelta > 0 (a fine face seen from
+ * a coarser cell, subfaceIndex[dim-1] in 0..3^levelDelta-1): the flux is
+ * restricted over levelDelta levels (faceUnknownsRestriction) and integrated
+ * with the coarse size 3^levelDelta dx into out (an update, as the reference
+ * glue :265-270).  subfaceIndex may be NULL when levelDelta == 0. */
 int exa_faceIntegral(exa_ctx* ctx, double* out, const double* lFhbnd, int direction, int orientation,
-                     int levelDelta, const double* dx, double dt, int addToUpdate);
+                     const int* subfaceIndex, int levelDelta, const double* dx, double dt, int addToUpdate);
 
 /* surfaceIntegralNonlinear (generic/c/3d/surfaceIntegralNonlinear.cpph:21-90): all 2D faces. */
 int exa_surfaceIntegral(exa_ctx* ctx, double* lduh, const double* lFhbnd, const double* dx);
@@ -205,6 +209,92 @@ void* exa_mesh_stream(exa_mesh* mesh);
 int exa_mesh_set_timing(exa_mesh* mesh, int enabled);
 int exa_mesh_phase_times(exa_mesh* mesh, double* ms_predictor, double* ms_riemann, double* ms_corrector,
                          int* nsteps);
+
+/* ---------------------------------------------------------------------------
+ * AMR projections between a cell of level l and its descendants at level
+ * l + delta (3^D children per level), reference generic/c/{2d,3d}/amrRoutines.cpph.
+ * Host arrays per cell, layouts as lQhbnd face / luh.  Prolongations overwrite
+ * the fine array; restrictions accumulate into the coarse array (as the
+ * reference).  subfaceIndex[dim-1] / subcellIndex[dim] in 0..3^delta-1.
+ * ------------------------------------------------------------------------- */
+/* faceUnknownsProlongation (3d/amrRoutines.cpph:43-143) */
+int exa_faceUnknownsProlongation(exa_ctx* ctx, double* lQhbndFine, double* lFhbndFine, const double* lQhbndCoarse,
+                                 const double* lFhbndCoarse, int coarseGridLevel, int fineGridLevel,
+                                 const int* subfaceIndex);
+/* faceUnknownsRestriction (3d/amrRoutines.cpph:248-314) */
+int exa_faceUnknownsRestriction(exa_ctx* ctx, double* lQhbndCoarse, double* lFhbndCoarse, const double* lQhbndFine,
+                                const double* lFhbndFine, int coarseGridLevel, int fineGridLevel,
+                                const int* subfaceIndex);
+/* volumeUnknownsProlongation (3d/amrRoutines.cpph:364-419) */
+int exa_volumeUnknownsProlongation(exa_ctx* ctx, double* luhFine, const double* luhCoarse, int coarseGridLevel,
+                                   int fineGridLevel, const int* subcellIndex);
+/* volumeUnknownsRestriction (3d/amrRoutines.cpph:483-554) */
+int exa_volumeUnknownsRestriction(exa_ctx* ctx, double* luhCoarse, const double* luhFine, int coarseGridLevel,
+                                  int fineGridLevel, const int* subcellIndex);
+
+/* ---------------------------------------------------------------------------
+ * Static two-level mesh (3D, nonlinear path): n^3 coarse cells on the
+ * periodic unit cube; every coarse cell with refined[c] != 0 (c = (z*n+y)*n+x)
+ * is split into 3^3 children.  Leaves are ordered by coarse cell, a refined
+ * cell contributing its 27 children (x fastest).  One global dt (min of the
+ * leaves' stableTimeStepSize with their own cell size).  Coarse-fine faces
+ * follow the reference's virtual-cell coupling: the coarse face data are
+ * prolongated to the 9 subfaces (prolongateFaceDataToVirtualCell,
+ * ADERDGSolver.cpp:1340-1399), the Riemann problems are solved on the fine
+ * subfaces, and the restricted fluctuations are integrated over the coarse
+ * face with the coarse cell size (restrictToTopMostParent, :1472-1563).
+ * ------------------------------------------------------------------------- */
+typedef struct exa_amr exa_amr;
+
+int exa_amr_create(exa_amr** out, exa_ctx* ctx, int n, const unsigned char* refined);
+int exa_amr_destroy(exa_amr* mesh);
+long exa_amr_nleaves(const exa_amr* mesh);
+/* level[nleaves] (0 coarse, 1 fine) and integer coordinates coords[nleaves][3]
+ * on the leaf's level (either may be NULL). */
+int exa_amr_leaves(exa_amr* mesh, int* level, int* coords);
+int exa_amr_set_solution(exa_amr* mesh, const double* luh);  /* [nleaves][B^3][nData] */
+int exa_amr_get_solution(exa_amr* mesh, double* luh);
+int exa_amr_init_dt(exa_amr* mesh, double dt);  /* dt <= 0: from the state */
+int exa_amr_run(exa_amr* mesh, int nsteps, int use_graph);
+int exa_amr_sync(exa_amr* mesh);
+int exa_amr_time(exa_amr* mesh, double* t, double* dt);
+/* Autotuner: times the candidate launch geometries on the mesh's own data and
+ * keeps the fastest: cfg[3] = {predictor grid (0 = one workgroup per tile,
+ * else persistent workgroups), Riemann faces per workgroup, corrector grid};
+ * ms[3] = their times.  The state is restored afterwards. */
+int exa_amr_autotune(exa_amr* mesh, int* cfg, double* ms);
+
+/* ---------------------------------------------------------------------------
+ * Ensemble: ninst independent uniform periodic meshes (n[i] cells per side on
+ * the unit square / cube), each with its own dt, stepped together: every
+ * kernel is one launch over the cells of all instances.
+ * ------------------------------------------------------------------------- */
+typedef struct exa_ensemble exa_ensemble;
+
+int exa_ensemble_create(exa_ensemble** out, exa_ctx* ctx, int ninst, const int* n);
+int exa_ensemble_destroy(exa_ensemble* ens);
+/* cells of instance inst (inst < 0: all instances) */
+long exa_ensemble_ncells(const exa_ensemble* ens, int inst);
+int exa_ensemble_set_solution(exa_ensemble* ens, int inst, const double* luh);
+int exa_ensemble_get_solution(exa_ensemble* ens, int inst, double* luh);
+int exa_ensemble_init_dt(exa_ensemble* ens, double dt);  /* dt <= 0: per instance from its state */
+int exa_ensemble_run(exa_ensemble* ens, int nsteps, int use_graph);
+int exa_ensemble_sync(exa_ensemble* ens);
+int exa_ensemble_time(exa_ensemble* ens, int inst, double* t, double* dt);
+int exa_ensemble_autotune(exa_ensemble* ens, int* cfg, double* ms);
+
+/* ---------------------------------------------------------------------------
+ * Group: step two-level meshes and ensembles together.  use_graph != 0
+ * captures one step of every member as one hipGraph (members on parallel
+ * branches) and replays it once per step.  Synchronous.  Re-tuning a member
+ * after the first captured run needs exa_group_reset.
+ * ------------------------------------------------------------------------- */
+typedef struct exa_group exa_group;
+
+int exa_group_create(exa_group** out, int namr, exa_amr* const* amrs, int nens, exa_ensemble* const* ens);
+int exa_group_destroy(exa_group* group);
+int exa_group_run(exa_group* group, int nsteps, int use_graph);
+int exa_group_reset(exa_group* group);
 
 /* ---------------------------------------------------------------------------
  * Linear acoustics on a periodic n^3 mesh (context: dim 3, nvar 4, npar 2,
