@@ -15,6 +15,13 @@ Two forms run in the same process on the same initial state:
 `value` is the FP32 form's DoF-updates/s; the FP64 form, the per-variable
 relative L2 error of FP32 vs FP64 and the speed-up are reported beside it.
 
+`--workload batched` measures the batched uneven workload: a static
+two-level 3D Euler mesh of order 4 (27^3 coarse cells, those within 0.25 of
+the centre refined 3x, one global dt) plus 64 independent 2D Euler meshes of
+order 4 with seeded sizes between 9^2 and 81^2 cells (one dt each), stepped
+together by one captured hipGraph per step after the autotuner has chosen the
+launch geometry of every member.
+
 `--workload acoustic` instead measures the linear acoustic workload (3D,
 order 4, 54^3 cells, per-cell wave speed with 8 Gaussian bumps, Ricker
 source f0 = 5, 64 receivers, 300 steps): forward DoF-updates/s in FP64 and
@@ -236,12 +243,100 @@ def bench_acoustic(args, E, P):
     print(json.dumps(out), flush=True)
 
 
+def bench_batched(args, E, P):
+    order, n, steps = 4, 27, args.steps
+    B = order + 1
+    c3 = E.Context(3, order, 5, 0)
+    c2 = E.Context(2, order, 5, 0)
+    refined = P.amr_ball_refinement(n, radius=0.25)
+    sizes = P.ensemble_sizes(seed=args.seed, count=64)
+    m = E.Amr(c3, n, refined)
+    level, coords = m.leaves()
+    u3 = P.amr_density_wave(order, n, level, coords)
+    e = E.Ensemble(c2, sizes)
+    u2 = [P.density_wave_2d(int(s), order).ravel() for s in sizes]
+
+    def reset():
+        m.set_solution(u3)
+        m.init_dt(0.0)
+        for i, u in enumerate(u2):
+            e.set_solution(i, u)
+        e.init_dt(0.0)
+
+    reset()
+    t0 = time.perf_counter()
+    tune3 = m.autotune()
+    tune2 = e.autotune()
+    tune_s = time.perf_counter() - t0
+    g = E.Group([m], [e])
+    if args.warmup:
+        g.run(args.warmup, use_graph=True)
+    reset()
+    t0 = time.perf_counter()
+    g.run(steps, use_graph=True)  # synchronous
+    wall = time.perf_counter() - t0
+    dofs_step = (m.nleaves * B ** 3 + int(np.sum(sizes.astype(np.int64) ** 2)) * B ** 2) * 5
+    it3 = executed_iterations(m.picard(), 3, 2 * B, True)
+    it2 = executed_iterations(e.picard(), 2, 2 * B, True)
+    f3 = float(np.sum(predictor_flops_per_cell(3, B, 5, 5, it3)))
+    f2 = float(np.sum(predictor_flops_per_cell(2, B, 5, 5, it2)))
+    ach = f3 / (tune3["ms"][0] * 1e-3) / 1e12
+    log(f"[bench] batched: {dofs_step * steps / wall:.4e} DoF/s, {wall * 1e3 / steps:.3f} ms/step; "
+        f"3D two-level {m.nleaves} leaves (pred {tune3['ms'][0]:.3f} ms, riem {tune3['ms'][1]:.3f}, "
+        f"corr {tune3['ms'][2]:.3f}), 2D ensemble {e.ncells()} cells (pred {tune2['ms'][0]:.3f} ms, "
+        f"riem {tune2['ms'][1]:.3f}, corr {tune2['ms'][2]:.3f}); tuning {tune_s:.2f} s")
+    cpu = None
+    if not args.no_cpu:
+        O = __import__("oracle")
+        try:
+            threads = len(os.sched_getaffinity(0))
+        except AttributeError:
+            threads = os.cpu_count() or 1
+        threads = max(1, min(16, threads))
+        o2 = O.Oracle(2, order)
+        sample = [int(s) for s in sizes[:16]]
+        t1 = time.perf_counter()
+        for s_ in sample:
+            luh = np.ascontiguousarray(P.density_wave_2d(s_, order).ravel())
+            o2.run_uniform(luh, s_, 1.0 / s_, 2, 0.0, threads=threads)
+        o3 = O.AmrOracle(3, order)
+        r9 = P.amr_ball_refinement(9, radius=0.25)
+        lv, co = O.AmrOracle.amr_leaves(9, r9)
+        u = P.amr_density_wave(order, 9, lv, co)
+        o3.amr_run(u, 9, r9, 2, threads=threads)
+        t = time.perf_counter() - t1
+        dofs = 2 * (sum(s_ * s_ for s_ in sample) * B ** 2 + len(lv) * B ** 3) * 5
+        cpu = dict(value=dofs / t, unit="DoF-updates/s", cores=threads, kind="port",
+                   sample=f"oracle on the first 16 2D instances and a 9^3 two-level mesh with the same ball, "
+                          f"2 steps each ({t:.2f} s wall, {threads} threads)")
+        log(f"[bench] cpu baseline: {cpu['value']:.4e} DoF/s on {threads} threads")
+    out = {
+        "metric": "DoF-updates/s batched uneven work: 3D Euler order-4 two-level mesh + 64 2D Euler instances",
+        "value": dofs_step * steps / wall, "unit": "DoF-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D Euler order {order} on 27^3 coarse cells with the ball r=0.25 refined 3x "
+                               f"({m.nleaves} leaves, one global dt) + 64 2D Euler order {order} instances of "
+                               f"{int(sizes.min())}..{int(sizes.max())} cells per side ({e.ncells()} cells, one dt "
+                               f"each); one captured hipGraph per step, autotuned launch geometry",
+                   "leaves_3d": int(m.nleaves), "cells_2d": int(e.ncells())},
+        "roofline": {"bound": "fp64-valu", "kernel": "3D predictor", "achieved": ach,
+                     "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS["fp64"],
+                     "traffic": None},
+        "autotune": {"amr": tune3, "ensemble": tune2, "seconds": tune_s},
+        "predictor_tflops_2d": f2 / (tune2["ms"][0] * 1e-3) / 1e12,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("euler", "acoustic"), default="euler")
+    ap.add_argument("--workload", choices=("euler", "acoustic", "batched"), default="euler")
     ap.add_argument("--n", type=int, default=None, help="cells per dimension (81 euler, 54 acoustic)")
     ap.add_argument("--order", type=int, default=None, help="order (5 euler, 4 acoustic)")
     ap.add_argument("--picard-fp32", type=int, default=0,
@@ -275,6 +370,10 @@ def main():
             args.steps = 300
         log(f"[bench] acoustic N={args.order} {args.n}^3 cells, steps={args.steps} warmup={args.warmup}")
         bench_acoustic(args, E, P)
+        return
+    if args.workload == "batched":
+        log(f"[bench] batched uneven work, steps={args.steps} warmup={args.warmup}")
+        bench_batched(args, E, P)
         return
     args.n = args.n or 81
     args.order = args.order or 5

@@ -109,10 +109,10 @@ __device__ __forceinline__ int stride_of(int d) {
 // barrier of the tile.
 template <int D, int B, int V, int NP, typename R>
 __device__ __forceinline__ void predictor_tile(
-    long blk, long ncells, const double* __restrict__ luh, const double* __restrict__ hcell,
+    long blk, long ncells, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
     double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
-    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive) {
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd) {
   using S = Shape<D, B, V, NP>;
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, NSP = S::NSP;
   constexpr int SLAB = NS * D * V;                       // one flux slab, per cell
@@ -305,8 +305,15 @@ __device__ __forceinline__ void predictor_tile(
         for (int m = 0; m < B; m++)
           du[v] += T->Kxi[c * B + m] * Fh[(d * NS + base[d] + m * stride_of<D, B>(d)) * V + v] * upd;
     }
+    if (luh_upd) {
+      // volume part of the solution update in place (solutionUpdate.cpph:28-67: dt / w)
+      const double upd = dt / W;
 #pragma unroll
-    for (int v = 0; v < V; v++) lduh[(cell * NS + s) * V + v] = du[v];
+      for (int v = 0; v < V; v++) luh_upd[(cell * NS + s) * ND + v] += du[v] * upd;
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; v++) lduh[(cell * NS + s) * V + v] = du[v];
+    }
   }
 
   // ---- extrapolation to the 2D faces (faces zeroed, then FCoeff sums) ----
@@ -376,24 +383,24 @@ __device__ __forceinline__ void predictor_tile(
 // Predictor + volume integral, one workgroup per tile of CPB cells.
 template <int D, int B, int V, int NP, typename R>
 __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_kernel(
-    long ncells, long ntiles, const double* __restrict__ luh, const double* __restrict__ hcell,
+    long ncells, long ntiles, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
     double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
-    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive) {
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd) {
   predictor_tile<D, B, V, NP, R>(blockIdx.x, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard, T, P,
-                                 maxIter, adaptive);
+                                 maxIter, adaptive, luh_upd);
 }
 
 // Persistent form: a grid smaller than ntiles loops over the tiles.
 template <int D, int B, int V, int NP, typename R>
 __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_persistent_kernel(
-    long ncells, long ntiles, const double* __restrict__ luh, const double* __restrict__ hcell,
+    long ncells, long ntiles, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
     double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
-    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive) {
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd) {
   for (long blk = blockIdx.x; blk < ntiles; blk += gridDim.x) {
     predictor_tile<D, B, V, NP, R>(blk, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard, T, P,
-                                   maxIter, adaptive);
+                                   maxIter, adaptive, luh_upd);
     __syncthreads();  // LDS of this tile fully consumed before the next one
   }
 }
@@ -573,6 +580,147 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_persi
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused Riemann + face integrals + update + CFL for a uniform periodic n^D
+// mesh (cells x fastest).  Every cell solves the Rusanov problems of its own
+// 2D faces from both sides' extrapolated data (riemannSolverNonlinear.cpph:37-128)
+// and keeps the flux; the two cells of a face evaluate it with the same
+// operations in the same order, so both see the same flux bits (conservation
+// as with one solve per face).  The volume part of the update is already in
+// luh (predictor with LaunchOpts::luh_update).
+//   phase 1: w_fn * Q of both sides per face node -> LDS
+//   phase 2: face averages, one thread per (face, side, variable), fixed order
+//   phase 3: flux per face node -> LDS
+//   phase 4: face integrals + update + stable dt per node
+// ---------------------------------------------------------------------------
+template <int D, int B, int V, int NP>
+struct FusedShape {
+  using S = Shape<D, B, V, NP>;
+  static constexpr int NTASK = 2 * D * S::NF;  // face nodes per cell
+  static constexpr int RED = NTASK * 2 * S::ND;  // partial products per cell
+  static constexpr int AVG = 2 * D * 2 * S::ND;  // averages per cell
+  static constexpr int LDS_BYTES = S::CPB * (RED + AVG) * 8;
+  static constexpr bool ok = LDS_BYTES <= 60 * 1024;
+};
+
+template <int D>
+__device__ __forceinline__ long periodic_neighbour(long cell, int n, int d, int dir) {
+  long ix = cell % n, iy = (cell / n) % n, iz = (D == 3) ? cell / (long(n) * n) : 0;
+  if (d == 0) ix = (ix + dir + n) % n;
+  if (d == 1) iy = (iy + dir + n) % n;
+  if (d == 2) iz = (iz + dir + n) % n;
+  return (D == 3) ? (iz * n + iy) * n + ix : iy * n + ix;
+}
+
+template <int D, int B, int V, int NP>
+__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riemann_kernel(
+    long ncells, int n, double* __restrict__ luh, const double* __restrict__ lQhbnd,
+    const double* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
+    double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P, double cflpnpm) {
+  using S = Shape<D, B, V, NP>;
+  using FS = FusedShape<D, B, V, NP>;
+  constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, THREADS = S::THREADS;
+  constexpr int NTASK = FS::NTASK;
+  __shared__ double sred[THREADS / 64];
+  __shared__ double sbuf[CPB * (FS::RED + FS::AVG)];
+  const int tid = threadIdx.x;
+  const int cib = tid / NS;
+  const int s = tid - cib * NS;
+  const long cell = long(blockIdx.x) * CPB + cib;
+  const bool valid = (cib < CPB) && (cell < ncells);
+  double* red = sbuf + (valid ? cib : 0) * (FS::RED + FS::AVG);  // [task][side][ND]; later fluxes [task][V]
+  double* avg = red + FS::RED;                                   // [face][side][ND]
+  const double* w = T->w;
+  // phase 1
+  if (valid) {
+    for (int task = s; task < NTASK; task += NS) {
+      const int f = task / NF, fn = task - f * NF, d = f >> 1, o = f & 1;
+      const long nb = periodic_neighbour<D>(cell, n, d, o ? 1 : -1);
+      const double* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
+      const double* qn = lQhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * ND;
+      const double* QL = o ? qo : qn;
+      const double* QR = o ? qn : qo;
+      const double wj = (D == 3) ? w[fn / B] * w[fn % B] : w[fn];
+#pragma unroll
+      for (int k = 0; k < ND; k++) {
+        red[(task * 2 + 0) * ND + k] = wj * QL[k];
+        red[(task * 2 + 1) * ND + k] = wj * QR[k];
+      }
+    }
+  }
+  __syncthreads();
+  // phase 2
+  if (valid) {
+    for (int q = s; q < 2 * D * 2 * ND; q += NS) {
+      const int f = q / (2 * ND), side = (q / ND) & 1, k = q % ND;
+      double acc = 0.0;
+      for (int fn = 0; fn < NF; fn++) acc += red[((f * NF + fn) * 2 + side) * ND + k];
+      avg[(f * 2 + side) * ND + k] = acc;
+    }
+  }
+  __syncthreads();
+  // phase 3: fluxes into red[task][V] (red's partial products were consumed
+  // in phase 2; phase 3 reads only avg and global memory)
+  if (valid) {
+    for (int task = s; task < NTASK; task += NS) {
+      const int f = task / NF, fn = task - f * NF, d = f >> 1, o = f & 1;
+      const long nb = periodic_neighbour<D>(cell, n, d, o ? 1 : -1);
+      const double* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
+      const double* qn = lQhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * ND;
+      const double* fo = lFhbnd + ((cell * 2 * D + f) * NF + fn) * V;
+      const double* fnb = lFhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * V;
+      const double* QL = o ? qo : qn;
+      const double* QR = o ? qn : qo;
+      const double* FL = o ? fo : fnb;
+      const double* FR = o ? fnb : fo;
+      const double smax = fmax(pde_max_abs_eig<D>(P, avg + (f * 2 + 0) * ND, d),
+                               pde_max_abs_eig<D>(P, avg + (f * 2 + 1) * ND, d));
+#pragma unroll
+      for (int k = 0; k < V; k++) red[task * V + k] = 0.5 * (FR[k] + FL[k]) - 0.5 * smax * (QR[k] - QL[k]);
+    }
+  }
+  __syncthreads();
+  // phase 4
+  double mydt = DBL_MAX;
+  if (valid) {
+    int z, y, x;
+    node_zyx<D, B>(s, z, y, x);
+    const double h = hcell[cell];
+    const double dt = *dtp;
+    double Q[ND];
+#pragma unroll
+    for (int v = 0; v < ND; v++) Q[v] = luh[(cell * NS + s) * ND + v];
+    double du[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) du[v] = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int c = coord<D, B>(d, z, y, x);
+      const int fn = face_node<D, B>(d, z, y, x);
+      const double tw = transverse_weight<D>(w, d, z, y, x);
+#pragma unroll
+      for (int o = 0; o < 2; o++) {
+        const double scaling = 1.0 / h * (2.0 * o - 1.0);
+        const double sw = tw * scaling;
+        const double fcoef = o ? T->F1[c] : T->F0[c];
+        const double* F = red + ((2 * d + o) * NF + fn) * V;
+#pragma unroll
+        for (int v = 0; v < V; v++) du[v] -= sw * fcoef * F[v];
+      }
+    }
+    const double Wn = (D == 3) ? w[z] * w[y] * w[x] : w[y] * w[x];
+    const double upd = dt / Wn;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      Q[v] = Q[v] + du[v] * upd;
+      luh[(cell * NS + s) * ND + v] = Q[v];
+    }
+    mydt = node_cfl_dt<D, B, V, NP>(Q, P, 1.0 / h, cflpnpm);
+  }
+  const double bmin = block_min<THREADS>(mydt, sred);
+  if (tid == 0 && bmin < DBL_MAX) atomic_min_pos_double(dt_next, bmin);
+}
+
 // Single-cell helpers used by the per-cell compatibility entry points.
 template <int D, int B, int V, int NP>
 __global__ void face_integral_kernel(double* __restrict__ lduh, const double* __restrict__ F, int d, int o,
@@ -641,17 +789,17 @@ struct Launch {
     if (k.precision == 0) {
       if (persistent)
         hipLaunchKernelGGL((predictor_persistent_kernel<D, B, V, NP, double>), dim3(grid), dim3(S::THREADS), 0, st,
-                           n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+                           n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update);
       else
         hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, double>), dim3(grid), dim3(S::THREADS), 0, st, n, tiles,
-                           luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+                           luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update);
     } else {
       if (persistent)
         hipLaunchKernelGGL((predictor_persistent_kernel<D, B, V, NP, float>), dim3(grid), dim3(S::THREADS), 0, st,
-                           n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+                           n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update);
       else
         hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, float>), dim3(grid), dim3(S::THREADS), 0, st, n, tiles,
-                           luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+                           luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update);
     }
     return static_cast<int>(hipGetLastError());
   }
@@ -675,6 +823,18 @@ struct Launch {
                          fb, h, dt, o.cinst, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update);
     return static_cast<int>(hipGetLastError());
   }
+  static int corrector_riemann(const KernelCfg& k, hipStream_t st, long n, int nside, double* luh,
+                               const double* qb, const double* fb, const double* h, const double* dt,
+                               double* dt_next) {
+    if constexpr (FusedShape<D, B, V, NP>::ok) {
+      const long blocks = (n + S::CPB - 1) / S::CPB;
+      hipLaunchKernelGGL((corrector_riemann_kernel<D, B, V, NP>), dim3(blocks), dim3(S::THREADS), 0, st, n, nside,
+                         luh, qb, fb, h, dt, dt_next, k.tab, k.pde, k.cfl * k.pnpm);
+      return static_cast<int>(hipGetLastError());
+    } else {
+      return -1;
+    }
+  }
   static int face_integral(const KernelCfg& k, hipStream_t st, double* lduh, const double* F, int d, int o,
                            double invdx) {
     hipLaunchKernelGGL((face_integral_kernel<D, B, V, NP>), dim3(1), dim3(ceil64(S::NS)), 0, st, lduh, F, d, o,
@@ -696,6 +856,8 @@ struct Entry {
   int (*riemann)(const KernelCfg&, hipStream_t, long, const int*, const double*, double*, const LaunchOpts&);
   int (*corrector)(const KernelCfg&, hipStream_t, long, double*, double*, const double*, const double*,
                    const double*, double*, int, const LaunchOpts&);
+  int (*corrector_riemann)(const KernelCfg&, hipStream_t, long, int, double*, const double*, const double*,
+                           const double*, const double*, double*);
   int (*face_integral)(const KernelCfg&, hipStream_t, double*, const double*, int, int, double);
   int (*solution_update)(const KernelCfg&, hipStream_t, double*, const double*, const double*, const double*);
 };
@@ -703,7 +865,8 @@ struct Entry {
 #define EXA_ENTRY(D, B, V, NP)                                                                      \
   Entry {                                                                                           \
     D, B, V, NP, &Launch<D, B, V, NP>::predictor, &Launch<D, B, V, NP>::riemann,                    \
-        &Launch<D, B, V, NP>::corrector, &Launch<D, B, V, NP>::face_integral,                       \
+        &Launch<D, B, V, NP>::corrector, &Launch<D, B, V, NP>::corrector_riemann,                   \
+        &Launch<D, B, V, NP>::face_integral,                                                        \
         &Launch<D, B, V, NP>::solution_update                                                       \
   }
 
@@ -740,8 +903,9 @@ int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* lu
     const int v = k.pred_variant;
     const bool fp32 = k.precision != 0;
     if ((v == 0 && fp32) || v == 3)
-      return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, fp32, o.cinst);
-    if (v == 2) return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, false, o.cinst);
+      return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, fp32, o.cinst, o.luh_update);
+    if (v == 2)
+      return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, false, o.cinst, o.luh_update);
   }
   return e->predictor(k, s, n, luh, h, dt, lduh, qb, fb, pic, o);
 }
@@ -760,6 +924,20 @@ int launch_corrector(const KernelCfg& k, hipStream_t s, long n, double* luh, dou
   if (!e) return -1;
   if (n <= 0) return 0;
   return e->corrector(k, s, n, luh, lduh, fb, h, dt, dt_next, 1, o);
+}
+
+int launch_corrector_riemann(const KernelCfg& k, hipStream_t s, long n, int nside, double* luh,
+                             const double* qb, const double* fb, const double* h, const double* dt,
+                             double* dt_next) {
+  const Entry* e = find_entry(k);
+  if (!e) return -1;
+  if (n <= 0) return 0;
+  return e->corrector_riemann(k, s, n, nside, luh, qb, fb, h, dt, dt_next);
+}
+
+bool corrector_riemann_supported(const KernelCfg& k) {
+  const Entry* e = find_entry(k);
+  return e && e->corrector_riemann(k, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr) != -1;
 }
 
 int launch_stable_dt(const KernelCfg& k, hipStream_t s, long n, const double* luh, const double* h,

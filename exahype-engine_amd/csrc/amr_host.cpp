@@ -132,6 +132,7 @@ struct CellSet {
   double *d_luh = nullptr, *d_lduh = nullptr, *d_qb = nullptr, *d_fb = nullptr, *d_h = nullptr;
   double* d_state = nullptr;  // t[ninst], dt[ninst], dt_next[ninst]
   int* d_cinst = nullptr;
+  int* d_picard = nullptr;  // per-cell Picard count of the last step
   int* d_faces = nullptr;
   long nfaces = 0;
   int* d_prolong = nullptr;
@@ -150,7 +151,7 @@ struct CellSet {
     double* b[] = {d_luh, d_lduh, d_qb, d_fb, d_h, d_state};
     for (double* p : b)
       if (p) (void)hipFree(p);
-    int* ib[] = {d_cinst, d_faces, d_prolong, d_restrict};
+    int* ib[] = {d_cinst, d_picard, d_faces, d_prolong, d_restrict};
     for (int* p : ib)
       if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -169,6 +170,8 @@ struct CellSet {
     HIP_OK(hipMalloc(&d_qb, sizeof(double) * nslots * 2 * D * NF * ND), "hipMalloc(lQhbnd)");
     HIP_OK(hipMalloc(&d_fb, sizeof(double) * nslots * 2 * D * NF * V), "hipMalloc(lFhbnd)");
     HIP_OK(hipMalloc(&d_state, sizeof(double) * 3 * ninst), "hipMalloc(state)");
+    HIP_OK(hipMalloc(&d_picard, sizeof(int) * ncells), "hipMalloc(picard)");
+    HIP_OK(hipMemset(d_picard, 0, sizeof(int) * ncells), "clear");
     std::vector<double> st(3 * ninst, 0.0);
     for (int i = ninst; i < 3 * ninst; i++) st[i] = DBL_MAX;
     HIP_OK(hipMemcpy(d_state, st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice), "state");
@@ -185,7 +188,8 @@ struct CellSet {
   int enqueue(hipStream_t s) {
     double* dt = d_state + ninst;
     double* dtn = d_state + 2 * ninst;
-    RC_OK(status(exa::launch_predictor(k, s, ncells, d_luh, d_h, dt, d_lduh, d_qb, d_fb, nullptr, with_cinst(tune.pred)),
+    RC_OK(status(exa::launch_predictor(k, s, ncells, d_luh, d_h, dt, d_lduh, d_qb, d_fb, d_picard,
+                                       with_cinst(tune.pred)),
                  "predictor"));
     if (nprolong) RC_OK(status(exa::launch_face_prolong(k, s, nprolong, d_prolong, d_qb, d_fb), "face prolongation"));
     RC_OK(status(exa::launch_riemann(k, s, nfaces, d_faces, d_qb, d_fb, tune.riem), "riemann"));
@@ -525,6 +529,13 @@ int exa_amr_time(exa_amr* a, double* t, double* dt) {
   return EXA_OK;
 }
 
+int exa_amr_picard(exa_amr* a, int* counts) {
+  if (!a || !counts) return argerr("exa_amr_picard: NULL argument");
+  HIP_OK(hipStreamSynchronize(a->cs.stream), "sync");
+  HIP_OK(hipMemcpy(counts, a->cs.d_picard, sizeof(int) * a->cs.ncells, hipMemcpyDeviceToHost), "picard");
+  return EXA_OK;
+}
+
 int exa_amr_autotune(exa_amr* a, int* cfg, double* ms) {
   if (!a) return argerr("exa_amr_autotune: NULL mesh");
   return a->cs.autotune(cfg, ms);
@@ -648,6 +659,13 @@ int exa_ensemble_time(exa_ensemble* e, int inst, double* t, double* dt) {
   HIP_OK(hipMemcpy(st.data(), e->cs.d_state, sizeof(double) * st.size(), hipMemcpyDeviceToHost), "time");
   if (t) *t = st[inst];
   if (dt) *dt = st[m + inst];
+  return EXA_OK;
+}
+
+int exa_ensemble_picard(exa_ensemble* e, int* counts) {
+  if (!e || !counts) return argerr("exa_ensemble_picard: NULL argument");
+  HIP_OK(hipStreamSynchronize(e->cs.stream), "sync");
+  HIP_OK(hipMemcpy(counts, e->cs.d_picard, sizeof(int) * e->cs.ncells, hipMemcpyDeviceToHost), "picard");
   return EXA_OK;
 }
 

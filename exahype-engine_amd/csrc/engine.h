@@ -48,10 +48,13 @@ struct KernelCfg {
 //   grid           > 0: persistent launch with this many workgroups looping over the
 //                  cell tiles (predictor, corrector); 0: one workgroup per tile
 //   riemann_waves  faces (wavefronts) per Riemann workgroup, 1..16
+//   luh_update     predictor only: add the volume part of the update to this luh
+//                  (the predictor's own input) instead of writing lduh
 struct LaunchOpts {
   const int* cinst = nullptr;
   long grid = 0;
   int riemann_waves = 4;
+  double* luh_update = nullptr;
 };
 
 // Launchers (return hipError_t as int; 0 on success, -1 for an unsupported
@@ -66,6 +69,13 @@ int launch_corrector(const KernelCfg& k, hipStream_t s, long ncells, double* luh
                      const LaunchOpts& o = LaunchOpts());
 int launch_stable_dt(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
                      double* dt_out, const LaunchOpts& o = LaunchOpts());
+// Fused Riemann + face integrals + update + CFL on a uniform periodic mesh of
+// nside cells per direction (cells x fastest); luh already holds the volume
+// part of the update.  -1 when the shape's LDS need exceeds the kernel budget.
+int launch_corrector_riemann(const KernelCfg& k, hipStream_t s, long ncells, int nside, double* luh,
+                             const double* lQhbnd, const double* lFhbnd, const double* h, const double* dt,
+                             double* dt_next);
+bool corrector_riemann_supported(const KernelCfg& k);
 // Single-cell helper kernels for the compatibility entry points.
 int launch_face_integral(const KernelCfg& k, hipStream_t s, double* lduh, const double* lFface, int direction,
                          int orientation, double invdx_scaled);
@@ -75,7 +85,7 @@ int launch_solution_update(const KernelCfg& k, hipStream_t s, double* luh, const
 bool lines_supported(const KernelCfg& k);
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
                              const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard,
-                             bool mfma, const int* cinst = nullptr);
+                             bool mfma, const int* cinst = nullptr, double* luh_update = nullptr);
 // Time bookkeeping of the device step per instance, state = t[ninst], dt[ninst],
 // dt_next[ninst]: t += dt; dt = dt_next; dt_next = DBL_MAX.
 int launch_advance_time(hipStream_t s, double* state, int ninst = 1);

@@ -40,6 +40,9 @@ struct exa_mesh {
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 4 events per timed step: before/after each phase
   int timed_steps = 0;
+  // fused step: the predictor adds the volume update to luh and one kernel
+  // solves each cell's face Riemann problems and applies the face update
+  bool fused = false;
 };
 
 namespace {
@@ -69,6 +72,24 @@ int enqueue_step(exa_mesh* m, hipEvent_t* ev) {
   double* st = m->d_state;
   const bool with_events = ev != nullptr;
   if (with_events) HIP_OK(hipEventRecord(ev[0], m->stream), "event");
+  if (m->fused) {
+    exa::LaunchOpts o;
+    o.luh_update = m->d_luh;
+    int rc = status(exa::launch_predictor(m->k, m->stream, m->ncells, m->d_luh, m->d_h, st + 1, m->d_lduh, m->d_qb,
+                                          m->d_fb, m->d_picard, o),
+                    "predictor");
+    if (rc) return rc;
+    if (with_events) HIP_OK(hipEventRecord(ev[1], m->stream), "event");
+    if (with_events) HIP_OK(hipEventRecord(ev[2], m->stream), "event");
+    rc = status(exa::launch_corrector_riemann(m->k, m->stream, m->ncells, m->n, m->d_luh, m->d_qb, m->d_fb, m->d_h,
+                                              st + 1, st + 2),
+                "corrector");
+    if (rc) return rc;
+    rc = status(exa::launch_advance_time(m->stream, st), "advance time");
+    if (rc) return rc;
+    if (with_events) HIP_OK(hipEventRecord(ev[3], m->stream), "event");
+    return EXA_OK;
+  }
   int rc = status(exa::launch_predictor(m->k, m->stream, m->ncells, m->d_luh, m->d_h, st + 1, m->d_lduh, m->d_qb,
                                         m->d_fb, m->d_picard),
                   "predictor");
@@ -161,6 +182,7 @@ int exa_mesh_create_uniform(exa_mesh** out, exa_ctx* ctx, int n, double h) {
     free_mesh(m);
     return fail(e, "exa_mesh_create_uniform(upload)");
   }
+  m->fused = exa::corrector_riemann_supported(m->k);
   *out = m;
   return EXA_OK;
 }
@@ -315,6 +337,20 @@ int exa_mesh_picard(exa_mesh* m, int* counts) {
 }
 
 void* exa_mesh_stream(exa_mesh* m) { return m ? static_cast<void*>(m->stream) : nullptr; }
+
+int exa_mesh_set_fused(exa_mesh* m, int enabled) {
+  if (!m) return EXA_ERR_ARG;
+  if (enabled && !exa::corrector_riemann_supported(m->k)) {
+    exa::set_error("exa_mesh_set_fused: the fused corrector is not compiled for this shape");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  m->fused = enabled != 0;
+  if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->graph_exec = nullptr;
+  m->graph = nullptr;
+  return EXA_OK;
+}
 
 int exa_mesh_set_timing(exa_mesh* m, int enabled) {
   if (!m) return EXA_ERR_ARG;

@@ -127,10 +127,10 @@ __device__ __forceinline__ void euler_flux_dir(const R* Q, R irho, R p, int d, R
 
 template <int B, typename R, bool MF>
 __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel(
-    long ncells, const double* __restrict__ luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
+    long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ lduh, double* __restrict__ lQhbnd,
     double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, PdeParams P,
-    int maxIter, int adaptive) {
+    int maxIter, int adaptive, double* luh_upd) {
   using S = LShape<B>;
   constexpr int V = S::V, NS = S::NS, NF = S::NF, TILE = S::TILE;
   static_assert(S::CPB == 1, "one cell per workgroup");
@@ -422,8 +422,15 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
           for (int m = 0; m < B; m++)
             du[v] += T->Kxi[c * B + m] * Fh[(d * NS + base + m * stride) * V + v] * upd;
       }
+      if (luh_upd) {
+        // volume part of the solution update in place (solutionUpdate.cpph:28-67)
+        const double upd = dt / (w[sz] * w[sy] * w[sx]);
 #pragma unroll
-      for (int v = 0; v < V; v++) lduh[(cell * NS + s) * V + v] = du[v];
+        for (int v = 0; v < V; v++) luh_upd[(cell * NS + s) * V + v] += du[v] * upd;
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) lduh[(cell * NS + s) * V + v] = du[v];
+      }
       // extrapolation to the six faces; lane l produces outputs l, l + NS, ...
       constexpr int NQ = 6 * NF * V;
       for (int o = l; o < NQ; o += NS) {
@@ -455,7 +462,7 @@ bool lines_supported(const KernelCfg& k) {
 
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const double* luh, const double* h,
                              const double* dt, double* lduh, double* qb, double* fb, int* pic, bool mfma,
-                             const int* cinst) {
+                             const int* cinst, double* luh_upd) {
   constexpr int B = 6;
   using S = LShape<B>;
   const long blocks = (n + S::CPB - 1) / S::CPB;
@@ -463,14 +470,14 @@ int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const d
   const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
   if (k.precision == 0) {
     hipLaunchKernelGGL((predictor3d_lines_kernel<B, double, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h, dt,
-                       cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+                       cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd);
   } else {
     if (mfma)
       hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h,
-                         dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+                         dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd);
     else
       hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh,
-                         h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive);
+                         h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd);
   }
   return static_cast<int>(hipGetLastError());
 }

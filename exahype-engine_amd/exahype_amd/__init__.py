@@ -83,6 +83,7 @@ _sig("exa_mesh_time", [_cp, _dp, _dp])
 _sig("exa_mesh_picard", [_cp, _ip])
 _sig("exa_mesh_stream", [_cp], ctypes.c_void_p)
 _sig("exa_mesh_set_timing", [_cp, ctypes.c_int])
+_sig("exa_mesh_set_fused", [_cp, ctypes.c_int])
 _sig("exa_mesh_phase_times", [_cp, _dp, _dp, _dp, _ip])
 _I, _D = ctypes.c_int, ctypes.c_double
 _u8p = ctypes.POINTER(ctypes.c_ubyte)
@@ -101,6 +102,7 @@ _sig("exa_amr_run", [_cp, _I, _I])
 _sig("exa_amr_sync", [_cp])
 _sig("exa_amr_time", [_cp, _dp, _dp])
 _sig("exa_amr_autotune", [_cp, _ip, _dp])
+_sig("exa_amr_picard", [_cp, _ip])
 _sig("exa_ensemble_create", [ctypes.POINTER(_cp), _cp, _I, _ip])
 _sig("exa_ensemble_destroy", [_cp])
 _sig("exa_ensemble_ncells", [_cp, _I], ctypes.c_long)
@@ -111,6 +113,7 @@ _sig("exa_ensemble_run", [_cp, _I, _I])
 _sig("exa_ensemble_sync", [_cp])
 _sig("exa_ensemble_time", [_cp, _I, _dp, _dp])
 _sig("exa_ensemble_autotune", [_cp, _ip, _dp])
+_sig("exa_ensemble_picard", [_cp, _ip])
 _sig("exa_group_create", [ctypes.POINTER(_cp), _I, ctypes.POINTER(_cp), _I, ctypes.POINTER(_cp)])
 _sig("exa_group_destroy", [_cp])
 _sig("exa_group_run", [_cp, _I, _I])
@@ -437,6 +440,11 @@ class Amr:
         _check(_lib.exa_amr_time(self.h, ctypes.byref(t), ctypes.byref(dt)))
         return t.value, dt.value
 
+    def picard(self):
+        out = np.zeros(self.nleaves, dtype=np.int32)
+        _check(_lib.exa_amr_picard(self.h, _ipv(out)))
+        return out
+
     def autotune(self):
         cfg = np.zeros(3, dtype=np.int32)
         ms = np.zeros(3)
@@ -493,6 +501,11 @@ class Ensemble:
         t, dt = ctypes.c_double(), ctypes.c_double()
         _check(_lib.exa_ensemble_time(self.h, inst, ctypes.byref(t), ctypes.byref(dt)))
         return t.value, dt.value
+
+    def picard(self):
+        out = np.zeros(self.ncells(-1), dtype=np.int32)
+        _check(_lib.exa_ensemble_picard(self.h, _ipv(out)))
+        return out
 
     def autotune(self):
         cfg = np.zeros(3, dtype=np.int32)

@@ -202,6 +202,13 @@ int exa_mesh_sync(exa_mesh* mesh);
 /* Current time, dt of the next step, per-cell Picard counts of the last step. */
 int exa_mesh_time(exa_mesh* mesh, double* t, double* dt);
 int exa_mesh_picard(exa_mesh* mesh, int* counts);
+/* Step form: enabled != 0 (default where compiled) fuses the step into two
+ * kernels -- the predictor adds the volume part of the update to luh, then one
+ * kernel solves every cell's face Riemann problems from both sides' data and
+ * applies the face part and the CFL reduction; 0 runs the reference's four
+ * phases (predictor into lduh, Riemann per face, face integrals + update, dt)
+ * as separate kernels.  Results agree to rounding. */
+int exa_mesh_set_fused(exa_mesh* mesh, int enabled);
 /* The mesh's HIP stream (hipStream_t as void*). */
 void* exa_mesh_stream(exa_mesh* mesh);
 /* Per-phase timing of the last exa_mesh_run with timing enabled: milliseconds
@@ -258,6 +265,8 @@ int exa_amr_init_dt(exa_amr* mesh, double dt);  /* dt <= 0: from the state */
 int exa_amr_run(exa_amr* mesh, int nsteps, int use_graph);
 int exa_amr_sync(exa_amr* mesh);
 int exa_amr_time(exa_amr* mesh, double* t, double* dt);
+/* per-leaf Picard counts of the last step (conventions of exa_mesh_picard) */
+int exa_amr_picard(exa_amr* mesh, int* counts);
 /* Autotuner: times the candidate launch geometries on the mesh's own data and
  * keeps the fastest: cfg[3] = {predictor grid (0 = one workgroup per tile,
  * else persistent workgroups), Riemann faces per workgroup, corrector grid};
@@ -281,6 +290,8 @@ int exa_ensemble_init_dt(exa_ensemble* ens, double dt);  /* dt <= 0: per instanc
 int exa_ensemble_run(exa_ensemble* ens, int nsteps, int use_graph);
 int exa_ensemble_sync(exa_ensemble* ens);
 int exa_ensemble_time(exa_ensemble* ens, int inst, double* t, double* dt);
+/* per-cell Picard counts of the last step, all instances */
+int exa_ensemble_picard(exa_ensemble* ens, int* counts);
 int exa_ensemble_autotune(exa_ensemble* ens, int* cfg, double* ms);
 
 /* ---------------------------------------------------------------------------
