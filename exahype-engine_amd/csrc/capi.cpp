@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cfloat>
 #include <cstring>
 #include <string>
@@ -33,6 +34,9 @@ struct exa_ctx {
   // scratch for the per-cell compatibility entry points
   double* d_scratch = nullptr;
   size_t scratch_bytes = 0;
+  // point source of the linear path (per-cell predictor)
+  bool has_src = false;
+  double src_x[3] = {0, 0, 0}, src_amp = 0, src_f0 = 0, src_t0 = 0;
 };
 
 namespace {
@@ -76,6 +80,63 @@ int nface(const exa_ctx* c) {
   return c->k.dim == 3 ? B * B : B;
 }
 int ndata(const exa_ctx* c) { return c->k.nvar + c->k.npar; }
+
+bool is_linear(const exa_ctx* c) { return c->k.pde.id == EXA_PDE_ACOUSTIC; }
+
+double lagrange(const double* x, int B, int k, double s) {
+  double r = 1.0;
+  for (int m = 0; m < B; m++)
+    if (m != k) r *= (s - x[m]) / (x[k] - x[m]);
+  return r;
+}
+
+double ricker(double t, double f0, double t0) {
+  const double pi = 3.14159265358979323846;
+  const double a = pi * pi * f0 * f0 * (t - t0) * (t - t0);
+  return (1.0 - 2.0 * a) * std::exp(-a);
+}
+
+// Point-source data of one cell (getPointSources / deltaDistribution,
+// generic/c/3d/deltaDistribution.cpph:37-146): returns false when the source
+// is not inside [centre - dx/2, centre + dx/2] (closed, as the reference).
+bool point_source_terms(const exa_ctx* c, const double* centre, const double* dx, double t, double dt,
+                        std::vector<double>& g, std::vector<double>& row) {
+  if (!c->has_src) return false;
+  for (int d = 0; d < 3; d++)
+    if (!(centre[d] - 0.5 * dx[d] <= c->src_x[d] && c->src_x[d] <= centre[d] + 0.5 * dx[d])) return false;
+  const int B = c->k.order + 1;
+  const double* w = c->tab.w;
+  const double* x = c->tab.x;
+  double xr[3];
+  for (int d = 0; d < 3; d++) xr[d] = (c->src_x[d] - centre[d]) / dx[d] + 0.5;
+  g.assign(size_t(B) * B * B, 0.0);
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++)
+      for (int k = 0; k < B; k++)
+        g[(i * B + j) * B + k] = 1.0 / (w[k] * dx[0] * w[j] * dx[1] * w[i] * dx[2]) *
+                                 (lagrange(x, B, k, xr[0]) * lagrange(x, B, j, xr[1]) * lagrange(x, B, i, xr[2]));
+  std::vector<double> fn(B + 1), psd(B), tmp(B);
+  for (int m = 0; m < B + 1; m++) fn[m] = c->src_amp * ricker(m == 0 ? t : dt * x[m - 1] + t, c->src_f0, c->src_t0);
+  row.assign(B + 1, 0.0);
+  double sum = 0.0;
+  for (int m = 1; m < B + 1; m++) sum += w[m - 1] * fn[m];
+  row[0] = dt * sum;
+  if (B > 1) row[1] = fn[0];
+  for (int m = 0; m < B; m++) psd[m] = fn[m + 1];
+  for (int i = 2; i < B; i++) {
+    for (int tt = 0; tt < B; tt++) {
+      double s2 = 0.0;
+      for (int q = 0; q < B; q++) s2 += psd[q] * c->tab.dudx[tt * B + q] * (1.0 / dt);
+      tmp[tt] = s2;
+    }
+    double v = 0.0;
+    for (int tt = 0; tt < B; tt++) v += tmp[tt] * c->tab.F0[tt];
+    row[i] = v;
+    psd = tmp;
+  }
+  row[B] = sum;
+  return true;
+}
 
 }  // namespace
 
@@ -194,6 +255,24 @@ int exa_ctx_set_advection(exa_ctx* c, const double* a) {
   return EXA_OK;
 }
 
+int exa_ctx_set_point_source(exa_ctx* c, const double* x, double amp, double f0, double t0) {
+  if (!c) {
+    set_error("exa_ctx_set_point_source: NULL context");
+    return EXA_ERR_ARG;
+  }
+  if (!is_linear(c)) {
+    set_error("exa_ctx_set_point_source: point sources are compiled for the linear path only");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  c->has_src = x != nullptr && amp != 0.0;
+  if (x)
+    for (int d = 0; d < 3; d++) c->src_x[d] = x[d];
+  c->src_amp = amp;
+  c->src_f0 = f0;
+  c->src_t0 = t0;
+  return EXA_OK;
+}
+
 int exa_ctx_set_predictor_variant(exa_ctx* c, int variant) {
   if (!c || variant < 0 || variant > 3) {
     set_error("exa_ctx_set_predictor_variant: invalid argument");
@@ -250,11 +329,66 @@ int exa_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* c, double* lduh, double* 
                                               const double* dx, double t, double dt,
                                               int addVolumeIntegralResultToUpdate, int* picard_iterations) {
   (void)lGradQhbnd;
-  (void)centre;
-  (void)t;
   if (!c || !lduh || !lQhbnd || !lFhbnd || !luh || !dx) {
     set_error("exa_fusedSpaceTimePredictorVolumeIntegral: NULL argument");
     return EXA_ERR_ARG;
+  }
+  if (is_linear(c)) {
+    // linear glue (AbstractGenericADERDGSolverImplementation.template:114-165):
+    // point sources of the cell, spaceTimePredictorLinear, volumeIntegralLinear
+    if (!centre) {
+      set_error("exa_fusedSpaceTimePredictorVolumeIntegral: centre is required by the linear path");
+      return EXA_ERR_ARG;
+    }
+    const int NS = nspace(c), NF = nface(c);
+    std::vector<double> g, row;
+    const bool src = point_source_terms(c, centre, dx, t, dt, g, row);
+    const size_t nL = size_t(NS) * 6, nD = size_t(NS) * 4, nQ = 6 * size_t(NF) * 6, nF = 6 * size_t(NF) * 4;
+    const size_t total = nL + nD + nQ + nF + (src ? g.size() + row.size() : 0);
+    int rc = ensure_scratch(c, total * sizeof(double));
+    if (rc) return rc;
+    double* d_luh = c->d_scratch;
+    double* d_lduh = d_luh + nL;
+    double* d_qb = d_lduh + nD;
+    double* d_fb = d_qb + nQ;
+    double* d_g = d_fb + nF;
+    double* d_row = d_g + (src ? g.size() : 0);
+    HIP_OK(hipMemcpyAsync(d_luh, luh, nL * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy luh");
+    if (src) {
+      HIP_OK(hipMemcpyAsync(d_g, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+      HIP_OK(hipMemcpyAsync(d_row, row.data(), row.size() * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+    }
+    exa::LinearCellArgs a;
+    a.op = exa::kLinPredictor;
+    a.tab = c->d_tab;
+    a.luh = d_luh;
+    for (int d = 0; d < 3; d++) a.dx[d] = dx[d];
+    a.dt = dt;
+    a.sg = src ? d_g : nullptr;
+    a.srow = src ? d_row : nullptr;
+    a.lduh = d_lduh;
+    a.lQb = d_qb;
+    a.lFb = d_fb;
+    rc = launch_status(exa::launch_linear_cell(c->k.order, a, c->stream), "linear predictor");
+    if (rc) return rc;
+    if (!addVolumeIntegralResultToUpdate) {
+      exa::LinearCellArgs u;
+      u.op = exa::kLinUpdate;
+      u.tab = c->d_tab;
+      u.luh_out = d_luh;
+      u.lduh = d_lduh;
+      u.dt = dt;
+      rc = launch_status(exa::launch_linear_cell(c->k.order, u, c->stream), "update");
+      if (rc) return rc;
+      HIP_OK(hipMemcpyAsync(luh, d_luh, nL * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy luh");
+    } else {
+      HIP_OK(hipMemcpyAsync(lduh, d_lduh, nD * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy lduh");
+    }
+    HIP_OK(hipMemcpyAsync(lQhbnd, d_qb, nQ * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy lQhbnd");
+    HIP_OK(hipMemcpyAsync(lFhbnd, d_fb, nF * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy lFhbnd");
+    HIP_OK(hipStreamSynchronize(c->stream), "sync");
+    if (picard_iterations) *picard_iterations = 0;
+    return EXA_OK;
   }
   for (int d = 1; d < c->k.dim; d++)
     if (dx[d] != dx[0]) {
@@ -313,6 +447,32 @@ int exa_riemannSolver(exa_ctx* c, double* FL, double* FR, const double* QL, cons
     set_error("exa_riemannSolver: direction out of range");
     return EXA_ERR_ARG;
   }
+  if (is_linear(c)) {  // riemannSolverLinear: FL, FR are outputs (fluctuations)
+    const int NF = nface(c);
+    const size_t nq = size_t(NF) * 6, nf = size_t(NF) * 4;
+    int rc = ensure_scratch(c, (2 * nq + 2 * nf) * sizeof(double));
+    if (rc) return rc;
+    double* d_ql = c->d_scratch;
+    double* d_qr = d_ql + nq;
+    double* d_fl = d_qr + nq;
+    double* d_fr = d_fl + nf;
+    HIP_OK(hipMemcpyAsync(d_ql, QL, nq * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy QL");
+    HIP_OK(hipMemcpyAsync(d_qr, QR, nq * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy QR");
+    exa::LinearCellArgs a;
+    a.op = exa::kLinRiemann;
+    a.tab = c->d_tab;
+    a.FL = d_fl;
+    a.FR = d_fr;
+    a.QL = d_ql;
+    a.QR = d_qr;
+    a.direction = direction;
+    rc = launch_status(exa::launch_linear_cell(c->k.order, a, c->stream), "linear riemann");
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(FL, d_fl, nf * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy FL");
+    HIP_OK(hipMemcpyAsync(FR, d_fr, nf * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy FR");
+    HIP_OK(hipStreamSynchronize(c->stream), "sync");
+    return EXA_OK;
+  }
   const int NF = nface(c), ND = ndata(c), V = c->k.nvar, D = c->k.dim;
   // two pseudo cells: cell 0 holds the left state on face 2d+1, cell 1 the right state on face 2d
   const size_t qcell = size_t(2 * D) * NF * ND, fcell = size_t(2 * D) * NF * V;
@@ -353,9 +513,31 @@ int exa_faceIntegral(exa_ctx* c, double* out, const double* lFhbnd, int directio
     set_error("exa_faceIntegral: levelDelta out of range or subfaceIndex missing");
     return EXA_ERR_ARG;
   }
-  if (c->k.pde.id == EXA_PDE_ACOUSTIC) {
-    set_error("exa_faceIntegral: the linear (acoustic) per-cell path is not compiled into this entry point");
-    return EXA_ERR_UNSUPPORTED;
+  if (is_linear(c)) {  // faceIntegralLinear (no sign factor); coarse-fine faces restrict first
+    if (levelDelta > 0 || !addToUpdate) {
+      set_error("exa_faceIntegral: the linear path supports conforming faces added to the update");
+      return EXA_ERR_UNSUPPORTED;
+    }
+    const int NS = nspace(c), NF = nface(c);
+    int rc = ensure_scratch(c, (size_t(NS) * 4 + size_t(NF) * 4) * sizeof(double));
+    if (rc) return rc;
+    double* d_lduh = c->d_scratch;
+    double* d_F = d_lduh + size_t(NS) * 4;
+    HIP_OK(hipMemcpyAsync(d_lduh, out, size_t(NS) * 4 * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+    HIP_OK(hipMemcpyAsync(d_F, lFhbnd, size_t(NF) * 4 * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+    exa::LinearCellArgs a;
+    a.op = exa::kLinFace;
+    a.tab = c->d_tab;
+    a.lduh = d_lduh;
+    a.Fface = d_F;
+    a.direction = direction;
+    a.orientation = orientation;
+    for (int d = 0; d < 3; d++) a.dx[d] = dx[d];
+    rc = launch_status(exa::launch_linear_cell(c->k.order, a, c->stream), "linear face integral");
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(out, d_lduh, size_t(NS) * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    HIP_OK(hipStreamSynchronize(c->stream), "sync");
+    return EXA_OK;
   }
   const int NS = nspace(c), NF = nface(c), ND = ndata(c), V = c->k.nvar;
   const size_t total = size_t(NS) * ND + size_t(NS) * V + 3 * size_t(NF) * V + 1;
@@ -438,6 +620,29 @@ int exa_addUpdateToSolution(exa_ctx* c, double* luh, const double* luhOld, const
     set_error("exa_addUpdateToSolution: NULL argument");
     return EXA_ERR_ARG;
   }
+  if (is_linear(c)) {
+    const int NS = nspace(c);
+    int rc = ensure_scratch(c, size_t(NS) * 10 * sizeof(double));
+    if (rc) return rc;
+    double* d_luh = c->d_scratch;
+    double* d_lduh = d_luh + size_t(NS) * 6;
+    HIP_OK(hipMemcpyAsync(d_luh, luhOld, size_t(NS) * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+    HIP_OK(hipMemcpyAsync(d_lduh, lduh, size_t(NS) * 4 * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+    exa::LinearCellArgs a;
+    a.op = exa::kLinUpdate;
+    a.tab = c->d_tab;
+    a.luh_out = d_luh;
+    a.lduh = d_lduh;
+    a.dt = dt;
+    rc = launch_status(exa::launch_linear_cell(c->k.order, a, c->stream), "update");
+    if (rc) return rc;
+    std::vector<double> tmp(size_t(NS) * 6);
+    HIP_OK(hipMemcpyAsync(tmp.data(), d_luh, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    HIP_OK(hipStreamSynchronize(c->stream), "sync");
+    for (int n = 0; n < NS; n++)
+      for (int v = 0; v < 4; v++) luh[n * 6 + v] = tmp[size_t(n) * 6 + v];
+    return EXA_OK;
+  }
   const int NS = nspace(c), ND = ndata(c), V = c->k.nvar;
   const size_t total = 2 * size_t(NS) * ND + size_t(NS) * V + 1;
   int rc = ensure_scratch(c, total * sizeof(double));
@@ -465,6 +670,26 @@ int exa_stableTimeStepSize(exa_ctx* c, const double* luh, const double* dx, doub
   if (!c || !luh || !dx || !dt_out) {
     set_error("exa_stableTimeStepSize: NULL argument");
     return EXA_ERR_ARG;
+  }
+  if (is_linear(c)) {
+    const int NS = nspace(c);
+    int rc = ensure_scratch(c, (size_t(NS) * 6 + 1) * sizeof(double));
+    if (rc) return rc;
+    double* d_luh = c->d_scratch;
+    double* d_out = d_luh + size_t(NS) * 6;
+    HIP_OK(hipMemcpyAsync(d_luh, luh, size_t(NS) * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+    exa::LinearCellArgs a;
+    a.op = exa::kLinDt;
+    a.tab = c->d_tab;
+    a.luh = d_luh;
+    for (int d = 0; d < 3; d++) a.dx[d] = dx[d];
+    a.cflpnpm = c->k.cfl * c->k.pnpm;
+    a.out = d_out;
+    rc = launch_status(exa::launch_linear_cell(c->k.order, a, c->stream), "stable dt");
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(dt_out, d_out, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    HIP_OK(hipStreamSynchronize(c->stream), "sync");
+    return EXA_OK;
   }
   for (int d = 1; d < c->k.dim; d++)
     if (dx[d] != dx[0]) {

@@ -94,6 +94,28 @@ int launch_density_wave(hipStream_t s, long ncells, int n, int D, int B, int ND,
                         const double* d_prm, double* luh);
 int launch_fill(hipStream_t s, double* p, long n, double value);
 
+// ---- per-cell linear (acoustic) kernels (linear_cell.hip) ------------------
+enum { kLinPredictor = 0, kLinRiemann, kLinFace, kLinUpdate, kLinDt };
+struct LinearCellArgs {
+  int op = 0;
+  const DevTables* tab = nullptr;
+  const double* luh = nullptr;  // [B^3][6]
+  double* luh_out = nullptr;    // update: luh in place
+  double dx[3] = {1, 1, 1};
+  double dt = 0, cflpnpm = 0;
+  const double* sg = nullptr;    // point source: spatial factor [B^3]
+  const double* srow = nullptr;  // point source: {sigma0, s_1..s_{B-1}, sum_t w_t f_t}
+  double* lduh = nullptr;        // [B^3][4]
+  double* lQb = nullptr;         // [6][B^2][6]
+  double* lFb = nullptr;         // [6][B^2][4]
+  double *FL = nullptr, *FR = nullptr;
+  const double *QL = nullptr, *QR = nullptr;
+  const double* Fface = nullptr;  // [B^2][4]
+  int direction = 0, orientation = 0;
+  double* out = nullptr;
+};
+int launch_linear_cell(int order, const LinearCellArgs& a, hipStream_t s);
+
 // ---- AMR projections (amr.hip), single level, 2D or 3D ---------------------
 // Face prolongation: per entry {coarse cell, face, subface s0 + 3 s1, target
 // cell}: lQhbnd / lFhbnd of the target's face = the coarse face prolongated

@@ -52,6 +52,7 @@ _sig("exa_ctx_create", [ctypes.POINTER(_cp), ctypes.c_int, ctypes.c_int, ctypes.
                         ctypes.c_int, ctypes.c_int, ctypes.c_double])
 _sig("exa_ctx_destroy", [_cp])
 _sig("exa_ctx_set_advection", [_cp, _dp])
+_sig("exa_ctx_set_point_source", [_cp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double])
 _sig("exa_ctx_set_predictor_variant", [_cp, ctypes.c_int])
 _sig("exa_tables", [ctypes.c_int] + [_dp] * 9)
 _sig("exa_ctx_tables", [_cp] + [_dp] * 8)
@@ -206,17 +207,22 @@ class Context:
         """0: fastest kernel for the shape (default), 1: generic node-column kernel."""
         _check(_lib.exa_ctx_set_predictor_variant(self.h, variant))
 
+    def set_point_source(self, x, amp, f0, t0):
+        xs = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        _check(_lib.exa_ctx_set_point_source(self.h, _p(xs), amp, f0, t0))
+
     def set_advection(self, a):
         _check(_lib.exa_ctx_set_advection(self.h, _p(np.ascontiguousarray(a, dtype=np.float64))))
 
     # -- per-cell compatibility API ---------------------------------------
-    def fusedSpaceTimePredictorVolumeIntegral(self, luh, dx, t, dt, addVolumeIntegralResultToUpdate=True):
+    def fusedSpaceTimePredictorVolumeIntegral(self, luh, dx, t, dt, addVolumeIntegralResultToUpdate=True,
+                                              centre=None):
         D, V, ND = self.dim, self.nvar, self.nd
         luh = np.array(luh, dtype=np.float64)
         lduh = np.zeros(self.nspace * V)
         lQhbnd = np.zeros(2 * D * self.nface * ND)
         lFhbnd = np.zeros(2 * D * self.nface * V)
-        centre = np.zeros(D)
+        centre = np.zeros(D) if centre is None else np.ascontiguousarray(centre, dtype=np.float64)
         pic = ctypes.c_int(0)
         _check(_lib.exa_fusedSpaceTimePredictorVolumeIntegral(
             self.h, _p(lduh), _p(lQhbnd), None, _p(lFhbnd), _p(luh), _p(centre),

@@ -77,6 +77,12 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
 int exa_ctx_destroy(exa_ctx* ctx);
 const char* exa_last_error(void);
 
+/* Point source of the linear path's per-cell predictor (getPointSources /
+ * deltaDistribution, generic/c/3d/deltaDistribution.cpph:37-146): pressure
+ * source at x[3] with the Ricker time function amp (1 - 2a) exp(-a),
+ * a = (pi f0 (t - t0))^2; x == NULL or amp == 0 removes it. */
+int exa_ctx_set_point_source(exa_ctx* ctx, const double* x, double amp, double f0, double t0);
+
 /* Advection velocity for EXA_PDE_ADVECTION (a[dim]). */
 int exa_ctx_set_advection(exa_ctx* ctx, const double* a);
 
@@ -109,6 +115,12 @@ int exa_ctx_tables(const exa_ctx* ctx, double* w, double* x, double* Kxi, double
  * *picard_iterations with the reference's per-dimension convention
  * (3D: 0-based index of the converged iteration, 2D: MaxIterations+2 on
  * convergence; MaxIterations when not converged). */
+/* For an EXA_PDE_ACOUSTIC context the per-cell entry points run the linear
+ * kernels (spaceTimePredictorLinear + volumeIntegralLinear with the context's
+ * point source, riemannSolverLinear with FL/FR as outputs, faceIntegralLinear
+ * for conforming faces added to the update, solutionUpdate, stableTimeStepSize
+ * with eigenvalues (c, -c, 0, 0)); parameters may vary per node; the Picard
+ * count is 0. */
 int exa_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* ctx, double* lduh, double* lQhbnd, double* lGradQhbnd,
                                               double* lFhbnd, double* luh, const double* centre,
                                               const double* dx, double t, double dt,

@@ -240,3 +240,84 @@ def test_full_workload_gradient_torch_autograd_vs_finite_differences():
     eps = 1e-5
     fd = (J_of(c0 + eps * dc) - J_of(c0 - eps * dc)) / (2 * eps)
     assert abs(fd - g @ dc) <= 1e-6 * abs(fd), (fd, g @ dc)
+
+
+# ---- per-cell compatibility entry points of the linear path -------------------
+def cell_state(order, seed):
+    rng = np.random.default_rng(seed)
+    N = (order + 1) ** 3
+    luh = np.zeros((N, 6))
+    luh[:, :4] = rng.normal(size=(N, 4))
+    luh[:, 4] = rng.uniform(0.8, 1.5, size=N)  # parameters may vary per node here
+    luh[:, 5] = rng.uniform(0.7, 1.3, size=N)
+    return luh.ravel()
+
+
+@pytest.mark.parametrize("order", [1, 2, 4, 5])
+def test_cell_predictor_matches_oracle(order):
+    c = ctx(order)
+    o = O.LinearOracle(3, order)
+    luh = cell_state(order, order)
+    dx = [0.1, 0.12, 0.09]
+    dt = 0.004
+    r = c.fusedSpaceTimePredictorVolumeIntegral(luh, dx, 0.0, dt, centre=[0.5, 0.5, 0.5])
+    ref = o.stp(luh, dx, dt)
+    assert rel_l2(r["lduh"], o.volume_integral(ref["lFhi"])) <= 1e-12
+    assert rel_l2(r["lQhbnd"], ref["lQbnd"]) <= 1e-12
+    assert rel_l2(r["lFhbnd"], ref["lFbnd"]) <= 1e-12
+    # addVolumeIntegralResultToUpdate = false: luh += dt/W * volume integral
+    r2 = c.fusedSpaceTimePredictorVolumeIntegral(luh, dx, 0.0, dt, addVolumeIntegralResultToUpdate=False,
+                                                 centre=[0.5, 0.5, 0.5])
+    assert rel_l2(r2["luh"], o.solution_update(luh, r["lduh"], dt)) <= 1e-13
+
+
+def test_cell_predictor_with_point_source_matches_oracle():
+    order = 4
+    c = E.Context(3, order, nvar=4, npar=2, pde=E.PDE_ACOUSTIC)
+    o = O.LinearOracle(3, order)
+    h, t, dt = 0.1, 0.013, 0.003
+    centre = np.array([0.45, 0.55, 0.35])
+    xs = centre + h * (np.array([0.3, 0.61, 0.47]) - 0.5)
+    c.set_point_source(xs, 2.0, 5.0, 0.02)
+    luh = cell_state(order, 3)
+    r = c.fusedSpaceTimePredictorVolumeIntegral(luh, [h] * 3, t, dt, centre=centre)
+    PSi = o.psi(centre, h, t, dt, xs, 2.0, 5.0, 0.02)
+    ref = o.stp(luh, [h] * 3, dt, PSi=PSi)
+    assert rel_l2(r["lduh"], o.volume_integral(ref["lFhi"])) <= 1e-12
+    assert rel_l2(r["lQhbnd"], ref["lQbnd"]) <= 1e-12
+    assert rel_l2(r["lFhbnd"], ref["lFbnd"]) <= 1e-12
+    # a cell that does not contain the source sees none
+    far = c.fusedSpaceTimePredictorVolumeIntegral(luh, [h] * 3, t, dt, centre=centre + 0.3)
+    plain = o.stp(luh, [h] * 3, dt)
+    assert rel_l2(far["lduh"], o.volume_integral(plain["lFhi"])) <= 1e-12
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_cell_riemann_face_update_dt_match_oracle(order):
+    c = ctx(order)
+    o = O.LinearOracle(3, order)
+    rng = np.random.default_rng(order + 30)
+    F = (order + 1) ** 2
+    QL = np.zeros((F, 6))
+    QR = np.zeros((F, 6))
+    QL[:, :4], QR[:, :4] = rng.normal(size=(F, 4)), rng.normal(size=(F, 4))
+    QL[:, 4:], QR[:, 4:] = (1.1, 0.9), (1.3, 1.2)
+    for d in range(3):
+        FL, FR = c.riemannSolver(np.zeros(F * 4), np.zeros(F * 4), QL.ravel(), QR.ravel(), d)
+        rFL, rFR = o.riemann(QL.ravel(), QR.ravel(), d)
+        assert rel_l2(FL, rFL) <= 1e-13 and rel_l2(FR, rFR) <= 1e-13
+    N = (order + 1) ** 3
+    lduh = rng.normal(size=N * 4)
+    face = rng.normal(size=F * 4)
+    dx = [0.1, 0.12, 0.09]
+    for d in range(3):
+        for orient in range(2):
+            got = c.faceIntegral(lduh, face, d, orient, dx)
+            assert rel_l2(got, o.face_integral(lduh, face, d, orient, dx)) <= 1e-13
+    lFb = rng.normal(size=6 * F * 4)
+    assert rel_l2(c.surfaceIntegral(lduh, lFb, dx), o.surface_integral(lduh, lFb, dx)) <= 1e-13
+    luh = cell_state(order, 5)
+    dtv = c.stableTimeStepSize(luh, dx)
+    cmax = np.abs(luh.reshape(-1, 6)[:, 5])
+    expect = np.min(0.9 * O.PNPM[order] / (cmax / dx[0] + cmax / dx[1] + cmax / dx[2]))
+    assert abs(dtv - expect) <= 1e-14 * expect
