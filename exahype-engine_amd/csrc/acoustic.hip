@@ -180,6 +180,221 @@ __global__ void __launch_bounds__(kAcThreads) ac_predict_kernel(AcousticArgs a) 
 }
 
 // ---------------------------------------------------------------------------
+// forward predictor, x-line form: one thread per x-line (z, y) of a cell.
+// x-derivatives come from the thread's registers; only the fields needed
+// along y and z (p, v for y; p, w for z) go through LDS, double-buffered so
+// each level needs one barrier.  Same mathematics as ac_predict_kernel; the
+// derivative sums are scaled by 1/h after summation (rounding differs from
+// the node form at the 1e-16 level).
+// ---------------------------------------------------------------------------
+template <int B>
+struct AcLineShape {
+  static constexpr int N = B * B * B;
+  static constexpr int F = B * B;    // lines per cell
+  static constexpr int BUF = 3 * N;  // p, v, w of one level
+  static constexpr int CPB_T = kAcThreads / F;
+  static constexpr int CPB_L = (60 * 1024) / (2 * BUF * 8);  // two level buffers per cell
+  static constexpr int CPB = CPB_T < CPB_L ? CPB_T : CPB_L;    // cells per block
+};
+
+// Derivative pass of the x-line form: from the thread's x-line (P, U in
+// registers) and the cell's p, v, w in LDS (sb), the divergence of u and the
+// gradient of p at the B nodes of the line.
+template <int B>
+__device__ __forceinline__ void line_derivatives(const double* __restrict__ Dm, double invh, int z, int y,
+                                                 const double (&P)[B], const double (&U)[B],
+                                                 const double* sb, double (&div)[B], double (&px)[B],
+                                                 double (&py)[B], double (&pz)[B]) {
+  constexpr int N = B * B * B;
+  double qv[B], qw[B];
+  // x: from the line's own registers
+#pragma unroll
+  for (int x = 0; x < B; x++) {
+    double sp = 0.0, su = 0.0;
+#pragma unroll
+    for (int m = 0; m < B; m++) {
+      sp += Dm[x * B + m] * P[m];
+      su += Dm[x * B + m] * U[m];
+    }
+    px[x] = sp;
+    div[x] = su;
+    py[x] = pz[x] = qv[x] = qw[x] = 0.0;
+  }
+  // y and z: one row of p, v (y) and p, w (z) per step m
+#pragma unroll 1
+  for (int m = 0; m < B; m++) {
+    const double cy = Dm[y * B + m], cz = Dm[z * B + m];
+    const double* ry = sb + (z * B + m) * B;  // (z, m, x)
+    const double* rz = sb + (m * B + y) * B;  // (m, y, x)
+#pragma unroll
+    for (int x = 0; x < B; x++) {
+      py[x] += cy * ry[x];
+      qv[x] += cy * ry[N + x];
+      pz[x] += cz * rz[x];
+      qw[x] += cz * rz[2 * N + x];
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < B; x++) {
+    px[x] *= invh;
+    py[x] *= invh;
+    pz[x] *= invh;
+    div[x] = div[x] * invh + qv[x] * invh + qw[x] * invh;
+  }
+}
+
+// Levels 1..B-1 give the time average qh = sum_i a_i L_i; by linearity the
+// volume update is dt * A(qh) - sigma0 g (A: one Cauchy-Kowalevski level,
+// i.e. the reference's -sum_d lFhi[d] of the linear flux), so a last
+// derivative pass on qh replaces per-level update accumulators.
+template <int B, bool UPDATE>
+__global__ void __launch_bounds__(kAcThreads) ac_predict_lines_kernel(AcousticArgs a) {
+  using S = AcLineShape<B>;
+  constexpr int N = S::N, F = S::F, CPB = S::CPB;
+  __shared__ double sbuf[2][CPB][S::BUF];
+  const int t = threadIdx.x;
+  const int lc = t / F, ln = t - lc * F;
+  const int z = ln / B, y = ln - z * B;
+  const long cell = long(blockIdx.x) * CPB + lc;
+  const bool slot = lc < CPB;
+  const int cs = slot ? lc : 0;
+  const bool on = slot && cell < a.ncells;
+  const double invh = 1.0 / a.h;
+  const double* Dm = a.tab->dudx;  // dudx[i][m]
+  double P[B], U[B], Vv[B], Wv[B];
+  double rho = 1.0, c = 1.0;
+  if (on) {
+    const double4* src = reinterpret_cast<const double4*>(a.u) + cell * N + ln * B;
+#pragma unroll
+    for (int x = 0; x < B; x++) {
+      const double4 q = src[x];
+      P[x] = q.x;
+      U[x] = q.y;
+      Vv[x] = q.z;
+      Wv[x] = q.w;
+    }
+    rho = a.mat[2 * cell];
+    c = a.mat[2 * cell + 1];
+  } else {
+#pragma unroll
+    for (int x = 0; x < B; x++) P[x] = U[x] = Vv[x] = Wv[x] = 0.0;
+  }
+  const double lam = rho * c * c, irho = 1.0 / rho;
+  const int step = a.step_override >= 0 ? a.step_override : *a.step_ptr;
+  const bool src = on && cell == a.src_cell;
+  const double* gp = src ? a.src_g + ln * B : nullptr;
+  const double* stp = src ? a.src_tab + long(step) * B : nullptr;
+  if (src) {
+#pragma unroll
+    for (int x = 0; x < B; x++) P[x] += stp[0] * gp[x];  // level 0
+  }
+  double qh[4][B];
+#pragma unroll
+  for (int x = 0; x < B; x++) {
+    qh[0][x] = P[x];
+    qh[1][x] = U[x];
+    qh[2][x] = Vv[x];
+    qh[3][x] = Wv[x];
+  }
+  double an = 0.5 * a.dt;  // a_{i+1}
+#pragma unroll 1
+  for (int i = 0; i + 1 < B; i++) {
+    double* sb = sbuf[i & 1][cs];
+    if (slot) {
+#pragma unroll
+      for (int x = 0; x < B; x++) {
+        sb[ln * B + x] = P[x];
+        sb[N + ln * B + x] = Vv[x];
+        sb[2 * N + ln * B + x] = Wv[x];
+      }
+    }
+    __syncthreads();
+    double div[B], px[B], py[B], pz[B];
+    line_derivatives<B>(Dm, invh, z, y, P, U, sb, div, px, py, pz);
+    const double si = src ? stp[i + 1] : 0.0;
+#pragma unroll
+    for (int x = 0; x < B; x++) {
+      P[x] = lam * div[x] + (src ? si * gp[x] : 0.0);
+      U[x] = irho * px[x];
+      Vv[x] = irho * py[x];
+      Wv[x] = irho * pz[x];
+      qh[0][x] += an * P[x];
+      qh[1][x] += an * U[x];
+      qh[2][x] += an * Vv[x];
+      qh[3][x] += an * Wv[x];
+    }
+    an *= a.dt / (i + 3);
+  }
+  // A(qh): the volume update, and qh on the faces
+  constexpr int LAST = (B - 1) & 1;
+  double* sb = sbuf[LAST][cs];
+  if (slot) {
+#pragma unroll
+    for (int x = 0; x < B; x++) {
+      sb[ln * B + x] = qh[0][x];
+      sb[N + ln * B + x] = qh[2][x];
+      sb[2 * N + ln * B + x] = qh[3][x];
+    }
+  }
+  __syncthreads();  // also: every read of the other buffer (last level) is done
+  double* su = sbuf[LAST ^ 1][cs];
+  if (slot) {
+#pragma unroll
+    for (int x = 0; x < B; x++) su[ln * B + x] = qh[1][x];
+  }
+  if (UPDATE && on) {
+    double div[B], px[B], py[B], pz[B];
+    line_derivatives<B>(Dm, invh, z, y, qh[0], qh[1], sb, div, px, py, pz);
+    const double s0 = src ? stp[0] : 0.0;
+    double4* dst = reinterpret_cast<double4*>(a.u) + cell * N + ln * B;
+#pragma unroll
+    for (int x = 0; x < B; x++) {
+      const double4 q = dst[x];
+      dst[x] = make_double4(q.x + a.dt * (lam * div[x]) - (src ? s0 * gp[x] : 0.0), q.y + a.dt * (irho * px[x]),
+                            q.z + a.dt * (irho * py[x]), q.w + a.dt * (irho * pz[x]));
+    }
+  }
+  __syncthreads();  // u-component of qh in LDS
+  if (on) {
+    const double* F0 = a.tab->F0;
+    const double* F1 = a.tab->F1;
+    double4* qb = reinterpret_cast<double4*>(a.qb) + cell * 6 * F;
+    // x-faces, node (z, y) = ln, from registers
+    double l0[4] = {0, 0, 0, 0}, l1[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int x = 0; x < B; x++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        l0[v] += qh[v][x] * F0[x];
+        l1[v] += qh[v][x] * F1[x];
+      }
+    qb[0 * F + ln] = make_double4(l0[0], l0[1], l0[2], l0[3]);
+    qb[1 * F + ln] = make_double4(l1[0], l1[1], l1[2], l1[3]);
+    // y-face node (z, x = y) and z-face node (y' = z, x' = y)
+    double y0[4] = {0, 0, 0, 0}, y1[4] = {0, 0, 0, 0}, z0[4] = {0, 0, 0, 0}, z1[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int l = 0; l < B; l++) {
+      const int ny = (z * B + l) * B + y;  // (z, l, x = y)
+      const int nz = (l * B + z) * B + y;  // (l, y' = z, x' = y)
+      const double f0 = F0[l], f1 = F1[l];
+      const double vy[4] = {sb[ny], su[ny], sb[N + ny], sb[2 * N + ny]};
+      const double vz[4] = {sb[nz], su[nz], sb[N + nz], sb[2 * N + nz]};
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        y0[v] += vy[v] * f0;
+        y1[v] += vy[v] * f1;
+        z0[v] += vz[v] * f0;
+        z1[v] += vz[v] * f1;
+      }
+    }
+    qb[2 * F + z * B + y] = make_double4(y0[0], y0[1], y0[2], y0[3]);
+    qb[3 * F + z * B + y] = make_double4(y1[0], y1[1], y1[2], y1[3]);
+    qb[4 * F + z * B + y] = make_double4(z0[0], z0[1], z0[2], z0[3]);
+    qb[5 * F + z * B + y] = make_double4(z1[0], z1[1], z1[2], z1[3]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // forward corrector: Riemann fluctuations of the six faces + face update
 // ---------------------------------------------------------------------------
 template <int B>
@@ -522,12 +737,19 @@ template <int B>
 int launch_all(int which, const AcousticArgs& a, hipStream_t s) {
   using S = AcShape<B>;
   const unsigned blocks = static_cast<unsigned>((a.ncells + S::CPB - 1) / S::CPB);
+  const unsigned lblocks = static_cast<unsigned>((a.ncells + AcLineShape<B>::CPB - 1) / AcLineShape<B>::CPB);
   switch (which) {
     case kAcPredictUpdate:
-      ac_predict_kernel<B, true><<<blocks, kAcThreads, 0, s>>>(a);
+      if (a.pred_variant == 0)
+        ac_predict_lines_kernel<B, true><<<lblocks, kAcThreads, 0, s>>>(a);
+      else
+        ac_predict_kernel<B, true><<<blocks, kAcThreads, 0, s>>>(a);
       break;
     case kAcPredictFaces:
-      ac_predict_kernel<B, false><<<blocks, kAcThreads, 0, s>>>(a);
+      if (a.pred_variant == 0)
+        ac_predict_lines_kernel<B, false><<<lblocks, kAcThreads, 0, s>>>(a);
+      else
+        ac_predict_kernel<B, false><<<blocks, kAcThreads, 0, s>>>(a);
       break;
     case kAcCorrector:
       ac_corrector_kernel<B><<<blocks, kAcThreads, 0, s>>>(a);

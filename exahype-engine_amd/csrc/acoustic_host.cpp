@@ -65,6 +65,7 @@ struct exa_acoustic {
   bool timing = false;
   std::vector<hipEvent_t> ev;
   int timed_steps = 0;
+  int pred_variant = 0;  // forward predictor: 0 x-line form, 1 node form
 };
 
 namespace {
@@ -201,6 +202,7 @@ exa::AcousticArgs args(exa_acoustic* ac, double* u) {
   a.fbar = ac->d_fbar;
   a.qbbar = ac->d_qbbar;
   a.gc = ac->d_gc;
+  a.pred_variant = ac->pred_variant;
   return a;
 }
 
@@ -652,6 +654,13 @@ int exa_acoustic_set_timing(exa_acoustic* ac, int enabled) {
   if (!ac) return argerr("exa_acoustic_set_timing: NULL acoustic");
   ac->timing = enabled != 0;
   ac->timed_steps = 0;
+  return EXA_OK;
+}
+
+int exa_acoustic_set_predictor(exa_acoustic* ac, int variant) {
+  if (!ac) return argerr("exa_acoustic_set_predictor: NULL acoustic");
+  if (variant != 0 && variant != 1) return argerr("exa_acoustic_set_predictor: variant must be 0 (x-line) or 1 (node)");
+  ac->pred_variant = variant;
   return EXA_OK;
 }
 

@@ -157,6 +157,7 @@ struct AcousticArgs {
   double* fbar = nullptr;
   double* qbbar = nullptr;
   double* gc = nullptr;
+  int pred_variant = 0;  // forward predictor: 0 x-line form, 1 node form
 };
 enum { kAcPredictUpdate = 0, kAcPredictFaces, kAcCorrector, kAcFbar, kAcRiemannAdj, kAcPredictAdj, kAcReceivers };
 bool acoustic_order_supported(int order);

@@ -135,6 +135,7 @@ _sig("exa_acoustic_run", [_cp, _I, _I, _dp])
 _sig("exa_acoustic_misfit_gradient", [_cp, _I, _I, _dp, _dp, _dp])
 _sig("exa_acoustic_last_times", [_cp, _dp, _dp, _dp])
 _sig("exa_acoustic_set_timing", [_cp, _I])
+_sig("exa_acoustic_set_predictor", [_cp, _I])
 _sig("exa_acoustic_phase_times", [_cp, _dp, _dp, _ip])
 
 
@@ -638,6 +639,10 @@ class Acoustic:
 
     def set_timing(self, enabled):
         _check(_lib.exa_acoustic_set_timing(self.h, 1 if enabled else 0))
+
+    def set_predictor(self, variant):
+        """Forward predictor form: 0 x-line kernel (default), 1 node kernel."""
+        _check(_lib.exa_acoustic_set_predictor(self.h, int(variant)))
 
     def phase_times(self):
         """Per-kernel milliseconds summed over the last eager run: predictor, corrector, steps."""

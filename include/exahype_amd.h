@@ -374,6 +374,12 @@ int exa_acoustic_last_times(exa_acoustic* ac, double* ms_forward, double* ms_rec
  * last run's steps. */
 int exa_acoustic_set_timing(exa_acoustic* ac, int enabled);
 int exa_acoustic_phase_times(exa_acoustic* ac, double* ms_predictor, double* ms_corrector, int* nsteps);
+/* Forward predictor form: 0 = x-line kernel (one thread per x-line of a cell,
+ * x-derivatives from registers; the default), 1 = node kernel (one thread per
+ * node). Both compute the same Cauchy-Kowalevski series; results agree to
+ * rounding (summation order differs). No reference counterpart: a tuning
+ * switch of this engine. */
+int exa_acoustic_set_predictor(exa_acoustic* ac, int variant);
 
 #ifdef __cplusplus
 }

@@ -60,13 +60,15 @@ def gpu_acoustic(order, n, h, luh, dt, src, rec):
     return ac
 
 
-@pytest.mark.parametrize("order,n,steps", [(1, 4, 6), (2, 4, 8), (3, 3, 6), (4, 4, 8), (5, 3, 5)])
-def test_forward_matches_oracle(order, n, steps):
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("order,n,steps", [(1, 4, 6), (2, 4, 8), (3, 3, 6), (4, 4, 8), (4, 5, 3), (5, 3, 5)])
+def test_forward_matches_oracle(order, n, steps, variant):
     luh, c, h, dt, src, rec, _ = small_problem(n, order, seed=order, steps=steps)
     o = O.LinearOracle(3, order)
     ref = luh.copy()
     rec_ref, _ = o.run(ref, n, h, dt, steps, src, rec)
     ac = gpu_acoustic(order, n, h, luh, dt, src, rec)
+    ac.set_predictor(variant)
     r = ac.run(steps)
     out = ac.get_solution().reshape(ref.shape)
     assert rel_l2(out, ref) <= 1e-12, rel_l2(out, ref)
