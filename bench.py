@@ -178,6 +178,7 @@ def bench_acoustic(args, E, P):
 
     def make(c, dt=0.0):
         ac = E.Acoustic(ctx, n, 1.0 / n)
+        ac.set_predictor(args.acoustic_predictor)
         ac.set_material(rho=np.ones(n ** 3), c=c)
         ac.set_dt(dt)
         ac.set_source(np.array(P.ACOUSTIC_SOURCE), P.ACOUSTIC_AMP, P.ACOUSTIC_F0, P.ACOUSTIC_T0)
@@ -234,6 +235,7 @@ def bench_acoustic(args, E, P):
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "phases_ms_per_step": {"predictor": pred_ms, "corrector": corr_ms},
+        "predictor_form": ("x-line", "node")[args.acoustic_predictor],
         "gradient_s": grad_wall,
         "gradient_phases_ms": {"forward": ms_fwd, "recompute": ms_rec, "adjoint": ms_adj},
         "gradient_over_forward": grad_wall / wall,
@@ -342,6 +344,8 @@ def main():
     ap.add_argument("--picard-fp32", type=int, default=0,
                     help="fixed Picard count of the FP32 form (0: the largest count the FP64 form executed)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--acoustic-predictor", type=int, default=0, choices=(0, 1),
+                    help="acoustic forward predictor: 0 x-line kernel, 1 node kernel")
     ap.add_argument("--modes", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-n", type=int, default=None)

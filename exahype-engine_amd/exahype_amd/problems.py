@@ -191,6 +191,23 @@ def amr_leaf_points(order, n, level, coords):
     return (np.asarray(coords)[:, None, :] + loc[None]) * h[:, None, None]
 
 
+def node_points(n, order, dim):
+    """Node coordinates [n^D cells][B^D nodes][dim] of the uniform mesh on the
+    unit box, cells and nodes x fastest (x = centre + h (xi - 1/2))."""
+    x1, _ = _coords(n, order, dim, np, None)  # [n][B]
+    B = order + 1
+    if dim == 2:
+        sh = (n, n, B, B)
+        X = np.broadcast_to(x1[None, :, None, :], sh)
+        Y = np.broadcast_to(x1[:, None, :, None], sh)
+        return np.stack([X, Y], -1).reshape(n * n, B * B, 2)
+    sh = (n, n, n, B, B, B)
+    X = np.broadcast_to(x1[None, None, :, None, None, :], sh)
+    Y = np.broadcast_to(x1[None, :, None, None, :, None], sh)
+    Z = np.broadcast_to(x1[:, None, None, :, None, None], sh)
+    return np.stack([X, Y, Z], -1).reshape(n ** 3, B ** 3, 3)
+
+
 def euler_state(rho, u, v, w, p):
     return np.stack([rho, rho * u, rho * v, rho * w, p / (GAMMA - 1) + 0.5 * rho * (u * u + v * v + w * w)], -1)
 

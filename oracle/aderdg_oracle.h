@@ -36,6 +36,13 @@
  *                   exact L2 projection (restriction) at level differences 1, 2
  *   amr mesh step   free-stream preservation, conservation, reduction to the
  *                   uniform driver for an unrefined / fully refined mesh
+ *   boundary        wall: ghost = mirror state (normal momentum negated), zero
+ *                   mass/energy wall flux for consistent data, the wall's
+ *                   momentum flux from the closed-form Rusanov formula;
+ *                   Dirichlet: Gauss time average vs adaptive quadrature of
+ *                   the exact entropy wave; mesh: rest state in a walled box,
+ *                   mirror symmetry of a walled box, reduction to the periodic
+ *                   driver, and convergence to the exact entropy wave
  */
 #ifndef EXAHYPE_AMD_ORACLE_H
 #define EXAHYPE_AMD_ORACLE_H
@@ -143,6 +150,16 @@ int orc_amr_nleaves(int n, const unsigned char* refined);
 void orc_amr_leaves(int n, const unsigned char* refined, int* level, int* coords);
 int orc_amr_run(const orc_solver* s, int n, const unsigned char* refined, double* luh, int steps, double* dt_io,
                 int threads);
+
+/* ---- boundary conditions (aderdg_bc_oracle.c) ---------------------------- */
+enum { ORC_BC_PERIODIC = 0, ORC_BC_WALL = 1, ORC_BC_DIRICHLET_ENTROPY_WAVE = 2 };
+void orc_entropy_wave(const orc_solver* s, const double* x, double t, double* Q);
+void orc_boundary_values(const orc_solver* s, int kind, const double* x, double t, double dt, int faceIndex,
+                         int direction, const double* FIn, const double* QIn, double* FOut, double* QOut);
+void orc_boundary_conditions(const orc_solver* s, int kind, double* FIn, const double* QIn, const double* centre,
+                             const double* dx, double t, double dt, int direction, int orientation);
+int orc_run_bc(const orc_solver* s, double* luh, int n, double h, int steps, double* dt_io, double* t_io,
+               const int* bc, int threads);
 
 #ifdef __cplusplus
 }

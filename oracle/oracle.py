@@ -123,6 +123,11 @@ def _lib():
     lib.orc_amr_leaves.argtypes = [I, u8p, ip, ip]
     lib.orc_amr_run.argtypes = [sp, I, u8p, dp, I, dp, I]
     lib.orc_amr_run.restype = I
+    lib.orc_entropy_wave.argtypes = [sp, dp, D, dp]
+    lib.orc_boundary_values.argtypes = [sp, I, dp, D, D, I, I, dp, dp, dp, dp]
+    lib.orc_boundary_conditions.argtypes = [sp, I, dp, dp, dp, dp, D, D, I, I]
+    lib.orc_run_bc.argtypes = [sp, dp, I, D, I, dp, dp, ip, I]
+    lib.orc_run_bc.restype = I
     return lib
 
 
@@ -144,6 +149,9 @@ def _p(a):
 def _ip(a):
     assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+BC_PERIODIC, BC_WALL, BC_DIRICHLET_ENTROPY_WAVE = 0, 1, 2
 
 
 class Oracle:
@@ -273,6 +281,42 @@ class Oracle:
         if rc != 0:
             raise MemoryError("oracle run_uniform failed to allocate")
         return float(dt_io[0]), pic
+
+    # ---- boundary conditions (Euler hooks of the reference's Euler example) ----
+    def entropy_wave(self, x, t):
+        Q = np.zeros(self.nvar, dtype=np.float64)
+        lib().orc_entropy_wave(self.ps, _p(np.ascontiguousarray(x, dtype=np.float64)), t, _p(Q))
+        return Q
+
+    def boundary_values(self, kind, x, t, dt, face_index, direction, FIn, QIn):
+        FOut = np.zeros(self.nvar, dtype=np.float64)
+        QOut = np.zeros(self.nd, dtype=np.float64)
+        lib().orc_boundary_values(self.ps, kind, _p(np.ascontiguousarray(x, dtype=np.float64)), t, dt, face_index,
+                                  direction, _p(np.ascontiguousarray(FIn, dtype=np.float64)),
+                                  _p(np.ascontiguousarray(QIn, dtype=np.float64)), _p(FOut), _p(QOut))
+        return FOut, QOut
+
+    def boundary_conditions(self, kind, FIn, QIn, centre, dx, t, dt, direction, orientation):
+        """The glue boundaryConditions for one face: returns the new FIn."""
+        F = np.ascontiguousarray(FIn, dtype=np.float64).copy()
+        lib().orc_boundary_conditions(self.ps, kind, _p(F), _p(np.ascontiguousarray(QIn, dtype=np.float64)),
+                                      _p(np.ascontiguousarray(centre, dtype=np.float64)),
+                                      _p(np.ascontiguousarray(dx, dtype=np.float64)), t, dt, direction,
+                                      orientation)
+        return F
+
+    def run_bc(self, luh, n, h, steps, bc, dt=0.0, t=0.0, threads=0):
+        """Advance a uniform mesh with a boundary kind per domain face; returns (t, dt_next)."""
+        assert luh.dtype == np.float64 and luh.flags["C_CONTIGUOUS"]
+        dt_io = np.array([dt], dtype=np.float64)
+        t_io = np.array([t], dtype=np.float64)
+        rc = lib().orc_run_bc(self.ps, _p(luh), n, h, steps, _p(dt_io), _p(t_io),
+                              _ip(np.ascontiguousarray(bc, dtype=np.int32)), threads)
+        if rc == -2:
+            raise ValueError("a direction must be periodic on both faces or on neither")
+        if rc != 0:
+            raise MemoryError("oracle run_bc failed to allocate")
+        return float(t_io[0]), float(dt_io[0])
 
 
 def _u8(a):

@@ -1,0 +1,257 @@
+"""The boundary-condition oracle (oracle/aderdg_bc_oracle.c).
+
+Pins:
+* wall hook: the outside state is the mirror state (normal momentum negated,
+  everything else copied) and the outside flux is the Euler flux of it (the
+  flux itself is pinned by the golden testPDEFluxes);
+* wall face, closed form: for face data whose flux is the flux of the state,
+  the Rusanov flux through a reflecting wall carries no mass, no tangential
+  momentum and no energy, and its normal momentum flux is
+  rho u_n^2 + p + (2o - 1) s rho u_n with s = |u_n| + c of the face average --
+  the wall pushes back on inflow.  The sign of the last term fixes which side
+  of the Riemann problem the outside state is on;
+* Dirichlet hook: Gauss time average of the exact entropy wave vs adaptive
+  quadrature; the entropy wave itself by its invariants (constant pressure and
+  velocity, translation with v0, peak 0.8 at x0 + v0 t, 1/e decay at distance
+  width^D, radial symmetry);
+* mesh: a rest state in a walled box stays at rest; walled boxes keep the
+  mirror symmetry of mirror-symmetric data in every walled direction;
+  all-periodic boundaries reduce bit-for-bit to the periodic driver; with the
+  Dirichlet entropy wave on every face the solution converges to the exact
+  entropy wave under refinement.
+"""
+import numpy as np
+import pytest
+from scipy.integrate import quad
+
+import oracle as O
+from exahype_amd import problems as P
+
+GAMMA = 1.4
+
+
+def random_face(rng, D, order):
+    B = order + 1
+    NF = B ** (D - 1)
+    rho = rng.uniform(0.8, 1.2, NF)
+    vel = rng.uniform(-0.5, 0.5, (NF, 3))
+    if D == 2:
+        vel[:, 2] = 0.0
+    p = rng.uniform(0.8, 1.2, NF)
+    return P.euler_state(rho, vel[:, 0], vel[:, 1], vel[:, 2], p)
+
+
+def pressure(Q, D):
+    j2 = (Q[..., 1:1 + D] ** 2).sum(-1)
+    return (GAMMA - 1) * (Q[..., 4] - 0.5 * j2 / Q[..., 0])
+
+
+@pytest.mark.parametrize("D,order", [(2, 2), (2, 4), (3, 2), (3, 4)])
+def test_wall_hook_mirrors_the_state(D, order):
+    o = O.Oracle(D, order)
+    rng = np.random.default_rng(D * 10 + order)
+    Q = random_face(rng, D, order)[0]
+    for d in range(D):
+        F, G = o.boundary_values(O.BC_WALL, np.full(3, 0.3), 0.0, 0.01, 2 * d, d, np.zeros(5), Q)
+        m = Q.copy()
+        m[1 + d] = -m[1 + d]
+        assert np.array_equal(G, m)
+        assert np.array_equal(F, o.flux(m)[d])
+
+
+@pytest.mark.parametrize("D,order", [(2, 3), (3, 2), (3, 5)])
+def test_wall_face_closed_form(D, order):
+    o = O.Oracle(D, order)
+    rng = np.random.default_rng(7 + D + order)
+    w = O.tables(order)["w"]
+    wf = w if D == 2 else np.outer(w, w).ravel()
+    for d in range(D):
+        for ori in (0, 1):
+            Q = random_face(rng, D, order)
+            F = np.array([o.flux(q)[d] for q in Q])
+            out = o.boundary_conditions(O.BC_WALL, F, Q, np.full(3, 0.5), np.full(3, 0.1), 0.0, 0.01, d,
+                                        ori).reshape(Q.shape[0], 5)
+            qa = (wf[:, None] * Q).sum(0)
+            s = abs(qa[1 + d] / qa[0]) + np.sqrt(GAMMA * pressure(qa, D) / qa[0])
+            un = Q[:, 1 + d] / Q[:, 0]
+            expect = Q[:, 0] * un * un + pressure(Q, D) + (2 * ori - 1) * s * Q[:, 0] * un
+            for v in range(5):
+                if v == 1 + d:
+                    np.testing.assert_allclose(out[:, v], expect, rtol=1e-13, atol=1e-13)
+                else:
+                    assert np.abs(out[:, v]).max() <= 1e-14, (d, ori, v)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_entropy_wave_invariants(D):
+    o = O.Oracle(D, 3)
+    rng = np.random.default_rng(D)
+    x0 = np.full(3, 0.5)
+    v0 = np.array([0.5, 0.0, 0.0])
+    for t in (0.0, 0.13, 0.4):
+        peak = o.entropy_wave(x0 + v0 * t, t)
+        assert abs(peak[0] - 0.8) <= 1e-15
+        for _ in range(5):
+            x = rng.uniform(0, 1, 3)
+            if D == 2:
+                x[2] = 0.5
+            Q = o.entropy_wave(x, t)
+            assert abs(pressure(Q, 3) - 1.0) <= 1e-14
+            assert abs(Q[1] / Q[0] - 0.5) <= 1e-15 and Q[2] == 0.0 and Q[3] == 0.0
+            tau = rng.uniform(0, 0.3)
+            assert np.allclose(o.entropy_wave(x + v0 * tau, t + tau), Q, rtol=1e-14, atol=0)
+            # radial symmetry about the moving centre
+            c = x0 + v0 * t
+            r = x - c
+            r[:D] = r[:D][::-1] if D == 3 else r[:D]
+            assert abs(o.entropy_wave(c + r, t)[0] - Q[0]) <= 1e-15
+    width_d = 0.3 ** D
+    e = np.zeros(3)
+    e[1] = width_d
+    assert abs(o.entropy_wave(x0 + e, 0.0)[0] - (0.5 + 0.3 / np.e)) <= 1e-15
+
+
+@pytest.mark.parametrize("D,order", [(2, 3), (3, 5)])
+def test_dirichlet_hook_is_the_time_average(D, order):
+    o = O.Oracle(D, order)
+    x = np.array([0.9, 0.2, 0.7])
+    t, dt = 0.05, 0.02
+    for d in range(D):
+        F, G = o.boundary_values(O.BC_DIRICHLET_ENTROPY_WAVE, x, t, dt, 2 * d + 1, d, np.zeros(5), np.zeros(5))
+        for v in range(5):
+            q = quad(lambda tau: o.entropy_wave(x, tau)[v], t, t + dt, epsabs=1e-15, epsrel=1e-14)[0] / dt
+            f = quad(lambda tau: o.flux(o.entropy_wave(x, tau))[d][v], t, t + dt, epsabs=1e-15,
+                     epsrel=1e-14)[0] / dt
+            assert abs(G[v] - q) <= 1e-12 * max(1.0, abs(q)), (d, v)
+            assert abs(F[v] - f) <= 1e-12 * max(1.0, abs(f)), (d, v)
+
+
+def symmetric_state(pts, D):
+    """Data mirror-symmetric about x = 1/2 and y = 1/2 (u odd in x, v odd in y)."""
+    x, y = pts[..., 0], pts[..., 1]
+    z = pts[..., 2] if D == 3 else 0.0 * x
+    cx, cy = np.cos(2 * np.pi * x), np.cos(2 * np.pi * y)
+    rho = 1 + 0.1 * cx * cy + 0.3 * (x - 0.5) ** 2 + 0.05 * np.cos(2 * np.pi * z)
+    u = 0.3 * np.sin(2 * np.pi * x) * (1 + 0.2 * cy)
+    v = 0.2 * np.sin(2 * np.pi * y) * (1 + 0.3 * cx)
+    w = 0.1 * cx * np.sin(2 * np.pi * z) if D == 3 else 0.0 * x
+    p = 1 + 0.1 * np.cos(4 * np.pi * x) + 0.05 * cy
+    return np.ascontiguousarray(P.euler_state(rho, u, v, w, p))
+
+
+def mirror(luh, n, order, D, d):
+    B = order + 1
+    shape = (n,) * D + (B,) * D + (5,)
+    a = luh.reshape(shape)
+    ax = D - 1 - d  # cell axis of direction d (x fastest)
+    a = np.flip(np.flip(a, ax), D + ax).copy()
+    a[..., 1 + d] *= -1
+    return a.reshape(luh.shape)
+
+
+@pytest.mark.parametrize("D,order,n,steps,bc", [
+    (2, 3, 6, 6, [1, 1, 0, 0]),
+    (2, 4, 5, 4, [1, 1, 1, 1]),
+    (3, 2, 4, 3, [1, 1, 0, 0, 1, 1]),
+    (3, 3, 3, 2, [1] * 6),
+])
+def test_walled_box_keeps_mirror_symmetry_and_rest(D, order, n, steps, bc):
+    o = O.Oracle(D, order)
+    pts = P.node_points(n, order, D)
+    luh = symmetric_state(pts, D)
+    walled = [d for d in range(D) if bc[2 * d] == O.BC_WALL]
+    for d in walled:
+        assert np.abs(mirror(luh, n, order, D, d) - luh).max() <= 1e-15
+    ref = luh.copy()
+    o.run_bc(luh, n, 1.0 / n, steps, bc)
+    assert np.abs(luh - ref).max() > 1e-3  # the state moved
+    for d in walled:
+        assert np.abs(mirror(luh, n, order, D, d) - luh).max() <= 1e-13, d
+    rest = np.zeros_like(luh)
+    rest[..., 0] = 1.3
+    rest[..., 4] = 2.0 / (GAMMA - 1)
+    r0 = rest.copy()
+    o.run_bc(rest, n, 1.0 / n, steps, bc)
+    assert np.abs(rest - r0).max() <= 1e-14
+
+
+@pytest.mark.parametrize("D,order,n", [(2, 3, 5), (3, 2, 3)])
+def test_all_periodic_is_the_periodic_driver(D, order, n):
+    o = O.Oracle(D, order)
+    a = symmetric_state(P.node_points(n, order, D), D)
+    b = a.copy()
+    t, dt_a = o.run_bc(a, n, 1.0 / n, 3, [O.BC_PERIODIC] * (2 * D))
+    dt_b, _ = o.run_uniform(b, n, 1.0 / n, 3)
+    assert np.array_equal(a, b) and dt_a == dt_b
+    with pytest.raises(ValueError):
+        o.run_bc(a, n, 1.0 / n, 1, [O.BC_PERIODIC, O.BC_WALL] + [O.BC_PERIODIC] * (2 * D - 2))
+
+
+def entropy_wave_state(o, n, order, D, t):
+    pts = P.node_points(n, order, D)
+    x = np.zeros(pts.shape[:-1] + (3,))
+    x[..., :D] = pts
+    x[..., D:] = 0.5
+    return np.ascontiguousarray(np.array([[o.entropy_wave(p, t) for p in cell] for cell in x]))
+
+
+def face_points(n, order, D, cell, f):
+    """Coordinates of the face nodes of face f of a cell, in the face-node
+    order of the extrapolated arrays (x-faces (z, y), y-faces (z, x), z-faces
+    (y, x)), read off the volume node coordinates of the line through each
+    face node."""
+    B = order + 1
+    d, o = f // 2, f % 2
+    vol = P.node_points(n, order, D)[cell].reshape((B,) * D + (D,))  # [(z)][y][x][D]
+    ax = D - 1 - d
+    line = np.take(vol, 0, axis=ax)  # the remaining axes keep their slow-to-fast order
+    pts = line.reshape(-1, D).copy()
+    centre = vol.reshape(-1, D).mean(0)  # nodes are symmetric about the centre
+    pts[:, d] = centre[d] + (o - 0.5) / n
+    x = np.full((pts.shape[0], 3), 0.5)
+    x[:, :D] = pts
+    return x
+
+
+@pytest.mark.parametrize("D,order,n", [(2, 3, 4), (3, 2, 3)])
+def test_dirichlet_face_uses_the_face_node_coordinates(D, order, n):
+    """boundaryConditions = hook at the face nodes, then the Riemann solve
+    with the outside state on the side opposite the cell."""
+    o = O.Oracle(D, order)
+    rng = np.random.default_rng(3)
+    B = order + 1
+    NF = B ** (D - 1)
+    t, dt = 0.7, 0.01
+    for f in range(2 * D):
+        d, ori = f // 2, f % 2
+        cell = int(rng.integers(n ** D))
+        centre = np.full(3, 0.5)
+        centre[:D] = P.node_points(n, order, D)[cell].mean(0)
+        Q = random_face(rng, D, order)
+        F = rng.normal(size=(NF, 5))
+        out = o.boundary_conditions(O.BC_DIRICHLET_ENTROPY_WAVE, F, Q, centre, np.full(3, 1.0 / n), t, dt, d, ori)
+        ghost = [o.boundary_values(O.BC_DIRICHLET_ENTROPY_WAVE, x, t, dt, f, d, np.zeros(5), np.zeros(5))
+                 for x in face_points(n, order, D, cell, f)]
+        FG = np.array([g[0] for g in ghost])
+        QG = np.array([g[1] for g in ghost])
+        if ori == 0:
+            FL, FR = o.riemann(FG, F, QG, Q, d)
+        else:
+            FL, FR = o.riemann(F, FG, Q, QG, d)
+        np.testing.assert_allclose(out.reshape(NF, 5), FL.reshape(NF, 5), rtol=1e-14, atol=1e-14)
+
+
+def test_dirichlet_entropy_wave_converges():
+    """The wave (centre at x = 0.9 at t = 0.8) leaves through the x = 1 face."""
+    D, order = 2, 3
+    o = O.Oracle(D, order)
+    errs = []
+    for n in (8, 16):
+        t0 = 0.8
+        luh = entropy_wave_state(o, n, order, D, t0)
+        h = 1.0 / n
+        t, _ = o.run_bc(luh, n, h, int(round(0.1 / (0.02 * h))), [O.BC_DIRICHLET_ENTROPY_WAVE] * 4,
+                        dt=0.02 * h, t=t0)
+        ex = entropy_wave_state(o, n, order, D, t)
+        errs.append(np.sqrt(((luh - ex) ** 2).sum() / (ex ** 2).sum()))
+    assert errs[0] < 1e-3 and errs[1] < errs[0] / 3, errs
