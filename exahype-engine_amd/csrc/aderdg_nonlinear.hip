@@ -414,11 +414,15 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Face entry (x, y, d, w): left cell x (its face 2d+1), right cell y (its
+// face 2d); with ghost faces given, w & 2 (w & 1): the left (right) side is
+// ghost face x (y) instead.
 template <int D, int B, int V, int NP>
 __global__ __launch_bounds__(256) void riemann_kernel(long nfaces, const int4* __restrict__ faces,
                                                       const double* __restrict__ lQhbnd,
-                                                      double* __restrict__ lFhbnd,
-                                                      const DevTables* __restrict__ T, PdeParams P) {
+                                                      double* __restrict__ lFhbnd, const double* __restrict__ gq,
+                                                      double* __restrict__ gf, const DevTables* __restrict__ T,
+                                                      PdeParams P) {
   using S = Shape<D, B, V, NP>;
   constexpr int ND = S::ND, NF = S::NF;
   const int lane = threadIdx.x & 63;
@@ -426,10 +430,11 @@ __global__ __launch_bounds__(256) void riemann_kernel(long nfaces, const int4* _
   if (f >= nfaces) return;  // uniform per wavefront; no block barriers below
   const int4 fc = faces[f];
   const int d = fc.z;
-  const double* QL = lQhbnd + ((long)fc.x * 2 * D + 2 * d + 1) * NF * ND;
-  const double* QR = lQhbnd + ((long)fc.y * 2 * D + 2 * d) * NF * ND;
-  double* FL = lFhbnd + ((long)fc.x * 2 * D + 2 * d + 1) * NF * V;
-  double* FR = lFhbnd + ((long)fc.y * 2 * D + 2 * d) * NF * V;
+  const bool gl = gq && (fc.w & 2) != 0, gr = gq && (fc.w & 1) != 0;  // w is ignored without ghost faces
+  const double* QL = gl ? gq + (long)fc.x * NF * ND : lQhbnd + ((long)fc.x * 2 * D + 2 * d + 1) * NF * ND;
+  const double* QR = gr ? gq + (long)fc.y * NF * ND : lQhbnd + ((long)fc.y * 2 * D + 2 * d) * NF * ND;
+  double* FL = gl ? gf + (long)fc.x * NF * V : lFhbnd + ((long)fc.x * 2 * D + 2 * d + 1) * NF * V;
+  double* FR = gr ? gf + (long)fc.y * NF * V : lFhbnd + ((long)fc.y * 2 * D + 2 * d) * NF * V;
   const double* w = T->w;
   double aL[ND], aR[ND];
 #pragma unroll
@@ -612,11 +617,31 @@ __device__ __forceinline__ long periodic_neighbour(long cell, int n, int d, int 
   return (D == 3) ? (iz * n + iy) * n + ix : iy * n + ix;
 }
 
+// Ghost face of face f = 2d + o of a cell when that face lies on a
+// non-periodic domain face (bit f of bcmask), else -1.  Index f * n^(D-1) +
+// transverse cell index: (iz, iy) for d = 0, (iz, ix) for d = 1, (iy, ix) for
+// d = 2; in 2D iy for d = 0, ix for d = 1.
+template <int D>
+__device__ __forceinline__ long ghost_face(long cell, int n, int f, unsigned bcmask) {
+  if (!((bcmask >> f) & 1u)) return -1;
+  const long ix = cell % n, iy = (cell / n) % n, iz = (D == 3) ? cell / (long(n) * n) : 0;
+  const int d = f >> 1;
+  const long id = (d == 0) ? ix : (d == 1 ? iy : iz);
+  if (id != ((f & 1) ? n - 1 : 0)) return -1;
+  long tr;
+  if (D == 3)
+    tr = (d == 0) ? iz * n + iy : (d == 1 ? iz * n + ix : iy * n + ix);
+  else
+    tr = (d == 0) ? iy : ix;
+  return long(f) * ((D == 3) ? long(n) * n : long(n)) + tr;
+}
+
 template <int D, int B, int V, int NP>
 __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riemann_kernel(
     long ncells, int n, double* __restrict__ luh, const double* __restrict__ lQhbnd,
     const double* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
-    double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P, double cflpnpm) {
+    double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P, double cflpnpm,
+    const double* __restrict__ gq, const double* __restrict__ gf, unsigned bcmask) {
   using S = Shape<D, B, V, NP>;
   using FS = FusedShape<D, B, V, NP>;
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, THREADS = S::THREADS;
@@ -634,10 +659,12 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
   // phase 1
   if (valid) {
     for (int task = s; task < NTASK; task += NS) {
-      const int f = task / NF, fn = task - f * NF, d = f >> 1, o = f & 1;
-      const long nb = periodic_neighbour<D>(cell, n, d, o ? 1 : -1);
+      const int f = task / NF, fn = task - f * NF, o = f & 1;
+      const long g = ghost_face<D>(cell, n, f, bcmask);
       const double* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
-      const double* qn = lQhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * ND;
+      const double* qn = g >= 0 ? gq + (g * NF + fn) * ND
+                                : lQhbnd + ((periodic_neighbour<D>(cell, n, f >> 1, o ? 1 : -1) * 2 * D + (f ^ 1)) * NF +
+                                            fn) * ND;
       const double* QL = o ? qo : qn;
       const double* QR = o ? qn : qo;
       const double wj = (D == 3) ? w[fn / B] * w[fn % B] : w[fn];
@@ -664,11 +691,12 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
   if (valid) {
     for (int task = s; task < NTASK; task += NS) {
       const int f = task / NF, fn = task - f * NF, d = f >> 1, o = f & 1;
-      const long nb = periodic_neighbour<D>(cell, n, d, o ? 1 : -1);
+      const long g = ghost_face<D>(cell, n, f, bcmask);
+      const long nb = g >= 0 ? 0 : periodic_neighbour<D>(cell, n, d, o ? 1 : -1);
       const double* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
-      const double* qn = lQhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * ND;
+      const double* qn = g >= 0 ? gq + (g * NF + fn) * ND : lQhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * ND;
       const double* fo = lFhbnd + ((cell * 2 * D + f) * NF + fn) * V;
-      const double* fnb = lFhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * V;
+      const double* fnb = g >= 0 ? gf + (g * NF + fn) * V : lFhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * V;
       const double* QL = o ? qo : qn;
       const double* QR = o ? qn : qo;
       const double* FL = o ? fo : fnb;
@@ -808,7 +836,7 @@ struct Launch {
     const int waves = (o.riemann_waves >= 1 && o.riemann_waves <= 16) ? o.riemann_waves : 4;
     const long blocks = (nf + waves - 1) / waves;
     hipLaunchKernelGGL((riemann_kernel<D, B, V, NP>), dim3(blocks), dim3(64 * waves), 0, st, nf,
-                       reinterpret_cast<const int4*>(faces), qb, fb, k.tab, k.pde);
+                       reinterpret_cast<const int4*>(faces), qb, fb, o.ghost_q, o.ghost_f, k.tab, k.pde);
     return static_cast<int>(hipGetLastError());
   }
   static int corrector(const KernelCfg& k, hipStream_t st, long n, double* luh, double* lduh, const double* fb,
@@ -825,11 +853,13 @@ struct Launch {
   }
   static int corrector_riemann(const KernelCfg& k, hipStream_t st, long n, int nside, double* luh,
                                const double* qb, const double* fb, const double* h, const double* dt,
-                               double* dt_next) {
+                               double* dt_next, const LaunchOpts& o) {
     if constexpr (FusedShape<D, B, V, NP>::ok) {
+      if (n <= 0) return 0;
       const long blocks = (n + S::CPB - 1) / S::CPB;
       hipLaunchKernelGGL((corrector_riemann_kernel<D, B, V, NP>), dim3(blocks), dim3(S::THREADS), 0, st, n, nside,
-                         luh, qb, fb, h, dt, dt_next, k.tab, k.pde, k.cfl * k.pnpm);
+                         luh, qb, fb, h, dt, dt_next, k.tab, k.pde, k.cfl * k.pnpm, o.ghost_q, o.ghost_f,
+                         o.bcmask);
       return static_cast<int>(hipGetLastError());
     } else {
       return -1;
@@ -857,7 +887,7 @@ struct Entry {
   int (*corrector)(const KernelCfg&, hipStream_t, long, double*, double*, const double*, const double*,
                    const double*, double*, int, const LaunchOpts&);
   int (*corrector_riemann)(const KernelCfg&, hipStream_t, long, int, double*, const double*, const double*,
-                           const double*, const double*, double*);
+                           const double*, const double*, double*, const LaunchOpts&);
   int (*face_integral)(const KernelCfg&, hipStream_t, double*, const double*, int, int, double);
   int (*solution_update)(const KernelCfg&, hipStream_t, double*, const double*, const double*, const double*);
 };
@@ -928,16 +958,18 @@ int launch_corrector(const KernelCfg& k, hipStream_t s, long n, double* luh, dou
 
 int launch_corrector_riemann(const KernelCfg& k, hipStream_t s, long n, int nside, double* luh,
                              const double* qb, const double* fb, const double* h, const double* dt,
-                             double* dt_next) {
+                             double* dt_next, const LaunchOpts& o) {
   const Entry* e = find_entry(k);
   if (!e) return -1;
   if (n <= 0) return 0;
-  return e->corrector_riemann(k, s, n, nside, luh, qb, fb, h, dt, dt_next);
+  return e->corrector_riemann(k, s, n, nside, luh, qb, fb, h, dt, dt_next, o);
 }
 
 bool corrector_riemann_supported(const KernelCfg& k) {
   const Entry* e = find_entry(k);
-  return e && e->corrector_riemann(k, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr) != -1;
+  // n = 0 launches nothing; -1 marks a shape whose fused kernel is not compiled
+  return e && e->corrector_riemann(k, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                   LaunchOpts()) != -1;
 }
 
 int launch_stable_dt(const KernelCfg& k, hipStream_t s, long n, const double* luh, const double* h,

@@ -37,6 +37,8 @@ struct exa_ctx {
   // point source of the linear path (per-cell predictor)
   bool has_src = false;
   double src_x[3] = {0, 0, 0}, src_amp = 0, src_f0 = 0, src_t0 = 0;
+  // boundaryValues hook of exa_boundaryConditions (the Euler example's default)
+  int bc_kind = EXA_BC_DIRICHLET_ENTROPY_WAVE;
 };
 
 namespace {
@@ -273,6 +275,19 @@ int exa_ctx_set_point_source(exa_ctx* c, const double* x, double amp, double f0,
   return EXA_OK;
 }
 
+int exa_ctx_set_boundary_values(exa_ctx* c, int kind) {
+  if (!c || (kind != EXA_BC_WALL && kind != EXA_BC_DIRICHLET_ENTROPY_WAVE)) {
+    set_error("exa_ctx_set_boundary_values: invalid argument (EXA_BC_WALL or EXA_BC_DIRICHLET_ENTROPY_WAVE)");
+    return EXA_ERR_ARG;
+  }
+  if (!exa::boundary_supported(c->k)) {
+    set_error("exa_ctx_set_boundary_values: boundary values are defined for the Euler equations only");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  c->bc_kind = kind;
+  return EXA_OK;
+}
+
 int exa_ctx_set_predictor_variant(exa_ctx* c, int variant) {
   if (!c || variant < 0 || variant > 3) {
     set_error("exa_ctx_set_predictor_variant: invalid argument");
@@ -495,6 +510,58 @@ int exa_riemannSolver(exa_ctx* c, double* FL, double* FR, const double* QL, cons
   if (rc) return rc;
   HIP_OK(hipMemcpyAsync(FL, d_fb + fl, NF * V * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy FL");
   HIP_OK(hipMemcpyAsync(FR, d_fb + fr, NF * V * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy FR");
+  HIP_OK(hipStreamSynchronize(c->stream), "sync");
+  return EXA_OK;
+}
+
+int exa_boundaryConditions(exa_ctx* c, double* fluxIn, const double* stateIn, const double* gradStateIn,
+                           const double* luh, const double* cellCentre, const double* cellSize, double t, double dt,
+                           int direction, int orientation) {
+  (void)gradStateIn;  // no viscous flux on this path
+  (void)luh;          // unused by the generic kernel as well
+  if (!c || !fluxIn || !stateIn || !cellCentre || !cellSize) {
+    set_error("exa_boundaryConditions: NULL argument");
+    return EXA_ERR_ARG;
+  }
+  if (direction < 0 || direction >= c->k.dim || (orientation != 0 && orientation != 1)) {
+    set_error("exa_boundaryConditions: direction or orientation out of range");
+    return EXA_ERR_ARG;
+  }
+  if (!exa::boundary_supported(c->k)) {
+    set_error("exa_boundaryConditions: boundary values are defined for the Euler equations only");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  const int NF = nface(c), ND = ndata(c), V = c->k.nvar, D = c->k.dim;
+  // pseudo cell 0 holds the inside face data on its face 2d + o; ghost face 0 the outside
+  const size_t qcell = size_t(2 * D) * NF * ND, fcell = size_t(2 * D) * NF * V;
+  const size_t face_at = (qcell + fcell + size_t(NF) * (ND + V) + 1) & ~size_t(1);  // int4 needs 16-byte alignment
+  int rc = ensure_scratch(c, (face_at + 2) * sizeof(double));
+  if (rc) return rc;
+  double* d_qb = c->d_scratch;
+  double* d_fb = d_qb + qcell;
+  double* d_gq = d_fb + fcell;
+  double* d_gf = d_gq + size_t(NF) * ND;
+  int* d_face = reinterpret_cast<int*>(d_qb + face_at);
+  const int d = direction, f = 2 * d + orientation;
+  // outside on the left of a left face, on the right of a right face
+  const int face[4] = {0, 0, d, orientation == 0 ? 2 : 1};
+  HIP_OK(hipMemcpyAsync(d_qb + size_t(f) * NF * ND, stateIn, NF * ND * sizeof(double), hipMemcpyHostToDevice,
+                        c->stream),
+         "copy stateIn");
+  HIP_OK(hipMemcpyAsync(d_fb + size_t(f) * NF * V, fluxIn, NF * V * sizeof(double), hipMemcpyHostToDevice, c->stream),
+         "copy fluxIn");
+  HIP_OK(hipMemcpyAsync(d_face, face, sizeof(face), hipMemcpyHostToDevice, c->stream), "copy face");
+  rc = launch_status(exa::launch_boundary_face(c->k, c->stream, c->bc_kind, cellCentre, cellSize, t, dt, d,
+                                               orientation, d_qb + size_t(f) * NF * ND, d_gq, d_gf),
+                     "boundary values");
+  if (rc) return rc;
+  exa::LaunchOpts o;
+  o.ghost_q = d_gq;
+  o.ghost_f = d_gf;
+  rc = launch_status(exa::launch_riemann(c->k, c->stream, 1, d_face, d_qb, d_fb, o), "riemann");
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(fluxIn, d_fb + size_t(f) * NF * V, NF * V * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+         "copy fluxIn");
   HIP_OK(hipStreamSynchronize(c->stream), "sync");
   return EXA_OK;
 }

@@ -50,11 +50,25 @@ struct KernelCfg {
 //   riemann_waves  faces (wavefronts) per Riemann workgroup, 1..16
 //   luh_update     predictor only: add the volume part of the update to this luh
 //                  (the predictor's own input) instead of writing lduh
+//   ghost_q/_f     Riemann kernels: outside face data of boundary faces
+//                  ([g][NF][ND], [g][NF][V]); a face-list entry with w & 2
+//                  (w & 1) takes its left (right) side from ghost face x (y)
+//   bcmask         fused corrector: bit f set = domain face f is not periodic;
+//                  its cells read ghost face f * n^(D-1) + transverse index
 struct LaunchOpts {
   const int* cinst = nullptr;
   long grid = 0;
   int riemann_waves = 4;
   double* luh_update = nullptr;
+  const double* ghost_q = nullptr;
+  double* ghost_f = nullptr;
+  unsigned bcmask = 0;
+};
+
+// Boundary kinds per domain face f = 2d + o (exa_boundary_kind).
+enum BcKind { kBcPeriodic = 0, kBcWall = 1, kBcDirichletEntropyWave = 2 };
+struct BcKinds {
+  int k[6];
 };
 
 // Launchers (return hipError_t as int; 0 on success, -1 for an unsupported
@@ -74,7 +88,7 @@ int launch_stable_dt(const KernelCfg& k, hipStream_t s, long ncells, const doubl
 // part of the update.  -1 when the shape's LDS need exceeds the kernel budget.
 int launch_corrector_riemann(const KernelCfg& k, hipStream_t s, long ncells, int nside, double* luh,
                              const double* lQhbnd, const double* lFhbnd, const double* h, const double* dt,
-                             double* dt_next);
+                             double* dt_next, const LaunchOpts& o = LaunchOpts());
 bool corrector_riemann_supported(const KernelCfg& k);
 // Single-cell helper kernels for the compatibility entry points.
 int launch_face_integral(const KernelCfg& k, hipStream_t s, double* lduh, const double* lFface, int direction,
@@ -89,9 +103,20 @@ int launch_predictor3d_lines(const KernelCfg& k, hipStream_t s, long ncells, con
 // Time bookkeeping of the device step per instance, state = t[ninst], dt[ninst],
 // dt_next[ninst]: t += dt; dt = dt_next; dt_next = DBL_MAX.
 int launch_advance_time(hipStream_t s, double* state, int ninst = 1);
+
+// Boundary conditions (Euler): outside face data of every boundary face of a
+// uniform n^D mesh (ghost layout of LaunchOpts::ghost_q) at the step's t, dt
+// read from state[0], state[1]; or of one face of one cell.
+bool boundary_supported(const KernelCfg& k);
+int launch_boundary_mesh(const KernelCfg& k, hipStream_t s, int n, double h, const BcKinds& bk, const double* qb,
+                         double* gq, double* gf, const double* state);
+int launch_boundary_face(const KernelCfg& k, hipStream_t s, int kind, const double* centre, const double* dx,
+                         double t, double dt, int d, int o, const double* qin, double* gq, double* gf);
 // Initial condition EXA_IC_DENSITY_WAVE on a uniform n^D mesh (initial.hip).
 int launch_density_wave(hipStream_t s, long ncells, int n, int D, int B, int ND, double h, const double* d_xi,
                         const double* d_prm, double* luh);
+int launch_entropy_wave(hipStream_t s, long ncells, int n, int D, int B, int ND, double h, const double* d_xi,
+                        double t, double* luh);
 int launch_fill(hipStream_t s, double* p, long n, double value);
 
 // ---- per-cell linear (acoustic) kernels (linear_cell.hip) ------------------

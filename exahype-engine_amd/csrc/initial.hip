@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "engine.h"
+#include "pde.h"
 
 namespace exa {
 namespace {
@@ -52,7 +53,40 @@ __global__ void density_wave_kernel(long ncells, int n, int D, int B, int ND, do
   for (int k = 5; k < ND; k++) Q[k] = 0.0;
 }
 
+// EXA_IC_ENTROPY_WAVE: the exact entropy wave at time t at every node
+template <int D>
+__global__ void entropy_wave_kernel(long ncells, int n, int B, int ND, double h, const double* __restrict__ xi,
+                                    double t, double* __restrict__ luh) {
+  const long NS = (D == 3) ? long(B) * B * B : long(B) * B;
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= ncells * NS) return;
+  const long cell = gid / NS;
+  const int s = static_cast<int>(gid - cell * NS);
+  const int jx = s % B, jy = (s / B) % B, jz = (D == 3) ? s / (B * B) : 0;
+  const long ix = cell % n, iy = (cell / n) % n, iz = (D == 3) ? cell / (long(n) * n) : 0;
+  double x[3];
+  x[0] = (ix + 0.5) * h + h * (xi[jx] - 0.5);
+  x[1] = (iy + 0.5) * h + h * (xi[jy] - 0.5);
+  x[2] = (D == 3) ? (iz + 0.5) * h + h * (xi[jz] - 0.5) : 0.5;
+  double* Q = luh + gid * ND;
+  euler_entropy_wave<D>(x, t, Q);
+  for (int k = 5; k < ND; k++) Q[k] = 0.0;
+}
+
 }  // namespace
+
+int launch_entropy_wave(hipStream_t s, long ncells, int n, int D, int B, int ND, double h, const double* d_xi,
+                        double t, double* luh) {
+  const long total = ncells * ((D == 3) ? long(B) * B * B : long(B) * B);
+  if (total <= 0) return 0;
+  if (D == 3)
+    hipLaunchKernelGGL(entropy_wave_kernel<3>, dim3((total + 255) / 256), dim3(256), 0, s, ncells, n, B, ND, h, d_xi,
+                       t, luh);
+  else
+    hipLaunchKernelGGL(entropy_wave_kernel<2>, dim3((total + 255) / 256), dim3(256), 0, s, ncells, n, B, ND, h, d_xi,
+                       t, luh);
+  return static_cast<int>(hipGetLastError());
+}
 
 int launch_density_wave(hipStream_t s, long ncells, int n, int D, int B, int ND, double h, const double* d_xi,
                         const double* d_prm, double* luh) {

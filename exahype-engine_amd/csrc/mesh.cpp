@@ -8,7 +8,10 @@
 //   4. dt_{n+1} = min over cells stableTimeStepSize (startNewTimeStep :1031-1049, reduce :458-462)
 // The first dt comes from the initial solution (updateTimeStepSize, :1051-1060).
 // Periodic neighbours are this driver's own rule: the reference ADER-DG path
-// has no periodic boundary.
+// has no periodic boundary.  Non-periodic domain faces take the reference's
+// boundary path (applyBoundaryConditions, :1836-1900): after the predictor a
+// boundary kernel writes the outside face data into ghost faces, which the
+// Riemann kernels read in place of a neighbour.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -43,6 +46,10 @@ struct exa_mesh {
   // fused step: the predictor adds the volume update to luh and one kernel
   // solves each cell's face Riemann problems and applies the face update
   bool fused = false;
+  // boundary kinds per domain face; ghost faces [2D][n^(D-1)][NF][ND | V]
+  exa::BcKinds bc{};
+  unsigned bcmask = 0;
+  double *d_gq = nullptr, *d_gf = nullptr;
 };
 
 namespace {
@@ -67,6 +74,22 @@ int status(int rc, const char* what) {
   return EXA_OK;
 }
 
+int enqueue_boundary(exa_mesh* m) {
+  if (!m->bcmask) return EXA_OK;
+  return status(exa::launch_boundary_mesh(m->k, m->stream, m->n, m->h, m->bc, m->d_qb, m->d_gq, m->d_gf, m->d_state),
+                "boundary conditions");
+}
+
+exa::LaunchOpts ghost_opts(const exa_mesh* m) {
+  exa::LaunchOpts o;
+  if (m->bcmask) {
+    o.ghost_q = m->d_gq;
+    o.ghost_f = m->d_gf;
+    o.bcmask = m->bcmask;
+  }
+  return o;
+}
+
 // Enqueue one global time step on m->stream.
 int enqueue_step(exa_mesh* m, hipEvent_t* ev) {
   double* st = m->d_state;
@@ -80,9 +103,11 @@ int enqueue_step(exa_mesh* m, hipEvent_t* ev) {
                     "predictor");
     if (rc) return rc;
     if (with_events) HIP_OK(hipEventRecord(ev[1], m->stream), "event");
+    rc = enqueue_boundary(m);
+    if (rc) return rc;
     if (with_events) HIP_OK(hipEventRecord(ev[2], m->stream), "event");
     rc = status(exa::launch_corrector_riemann(m->k, m->stream, m->ncells, m->n, m->d_luh, m->d_qb, m->d_fb, m->d_h,
-                                              st + 1, st + 2),
+                                              st + 1, st + 2, ghost_opts(m)),
                 "corrector");
     if (rc) return rc;
     rc = status(exa::launch_advance_time(m->stream, st), "advance time");
@@ -95,7 +120,10 @@ int enqueue_step(exa_mesh* m, hipEvent_t* ev) {
                   "predictor");
   if (rc) return rc;
   if (with_events) HIP_OK(hipEventRecord(ev[1], m->stream), "event");
-  rc = status(exa::launch_riemann(m->k, m->stream, m->nfaces, m->d_faces, m->d_qb, m->d_fb), "riemann");
+  rc = enqueue_boundary(m);
+  if (rc) return rc;
+  rc = status(exa::launch_riemann(m->k, m->stream, m->nfaces, m->d_faces, m->d_qb, m->d_fb, ghost_opts(m)),
+              "riemann");
   if (rc) return rc;
   if (with_events) HIP_OK(hipEventRecord(ev[2], m->stream), "event");
   rc = status(exa::launch_corrector(m->k, m->stream, m->ncells, m->d_luh, m->d_lduh, m->d_fb, m->d_h, st + 1,
@@ -112,13 +140,54 @@ void free_mesh(exa_mesh* m) {
   if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
   if (m->graph) (void)hipGraphDestroy(m->graph);
   for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
-  double* bufs[] = {m->d_luh, m->d_lduh, m->d_qb, m->d_fb, m->d_h, m->d_state};
+  double* bufs[] = {m->d_luh, m->d_lduh, m->d_qb, m->d_fb, m->d_h, m->d_state, m->d_gq, m->d_gf};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (m->d_faces) (void)hipFree(m->d_faces);
   if (m->d_picard) (void)hipFree(m->d_picard);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
+}
+
+// Face list: by direction, then by left cell (x fastest).  Periodic
+// directions wrap; in the others the last cell's right face pairs with its
+// ghost face (w = 1) and the first cell's left face with its ghost (w = 2).
+std::vector<int> build_faces(const exa_mesh* m) {
+  const int D = m->k.dim, n = m->n;
+  const long per = (D == 3) ? long(n) * n : long(n);
+  std::vector<int> faces;
+  faces.reserve(size_t(4) * D * (n + 1) * per);
+  auto push = [&faces](long a, long b, int d, int w) {
+    faces.push_back(static_cast<int>(a));
+    faces.push_back(static_cast<int>(b));
+    faces.push_back(d);
+    faces.push_back(w);
+  };
+  for (int d = 0; d < D; d++) {
+    const bool periodic = m->bc.k[2 * d] == exa::kBcPeriodic;
+    for (long c = 0; c < m->ncells; c++) {
+      long ix = c % n, iy = (c / n) % n, iz = (D == 3) ? c / (long(n) * n) : 0;
+      const long id = (d == 0) ? ix : (d == 1 ? iy : iz);
+      const long tr = (D == 3) ? ((d == 0) ? iz * n + iy : (d == 1 ? iz * n + ix : iy * n + ix)) : ((d == 0) ? iy : ix);
+      if (!periodic && id == 0) push(long(2 * d) * per + tr, c, d, 2);
+      if (!periodic && id == n - 1) {
+        push(c, long(2 * d + 1) * per + tr, d, 1);
+        continue;
+      }
+      if (d == 0) ix = (ix + 1) % n;
+      if (d == 1) iy = (iy + 1) % n;
+      if (d == 2) iz = (iz + 1) % n;
+      push(c, (D == 3) ? (iz * n + iy) * n + ix : iy * n + ix, d, 0);
+    }
+  }
+  return faces;
+}
+
+void drop_graph(exa_mesh* m) {
+  if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->graph_exec = nullptr;
+  m->graph = nullptr;
 }
 
 }  // namespace
@@ -150,28 +219,14 @@ int exa_mesh_create_uniform(exa_mesh** out, exa_ctx* ctx, int n, double h) {
   if (e == hipSuccess) e = hipMalloc(&m->d_fb, sizeof(double) * m->ncells * 2 * D * m->NF * V);
   if (e == hipSuccess) e = hipMalloc(&m->d_h, sizeof(double) * m->ncells);
   if (e == hipSuccess) e = hipMalloc(&m->d_state, sizeof(double) * 4);
-  if (e == hipSuccess) e = hipMalloc(&m->d_faces, sizeof(int) * 4 * m->nfaces);
+  // room for the face list of any boundary choice: (n + 1) n^(D-1) entries per direction
+  if (e == hipSuccess) e = hipMalloc(&m->d_faces, sizeof(int) * 4 * D * (n + 1) * (m->ncells / n));
   if (e == hipSuccess) e = hipMalloc(&m->d_picard, sizeof(int) * m->ncells);
   if (e != hipSuccess) {
     free_mesh(m);
     return fail(e, "exa_mesh_create_uniform");
   }
-  // faces ordered by direction, then by left cell (x fastest)
-  std::vector<int> faces(4 * m->nfaces);
-  long f = 0;
-  for (int d = 0; d < D; d++)
-    for (long c = 0; c < m->ncells; c++) {
-      long ix = c % n, iy = (c / n) % n, iz = (D == 3) ? c / (long(n) * n) : 0;
-      if (d == 0) ix = (ix + 1) % n;
-      if (d == 1) iy = (iy + 1) % n;
-      if (d == 2) iz = (iz + 1) % n;
-      const long r = (D == 3) ? (iz * n + iy) * n + ix : iy * n + ix;
-      faces[4 * f + 0] = static_cast<int>(c);
-      faces[4 * f + 1] = static_cast<int>(r);
-      faces[4 * f + 2] = d;
-      faces[4 * f + 3] = 0;
-      f++;
-    }
+  const std::vector<int> faces = build_faces(m);  // all periodic
   std::vector<double> hs(m->ncells, h);
   const double st[4] = {0.0, 0.0, DBL_MAX, 0.0};
   e = hipMemcpy(m->d_faces, faces.data(), sizeof(int) * faces.size(), hipMemcpyHostToDevice);
@@ -223,13 +278,27 @@ int exa_mesh_get_solution(exa_mesh* m, double* luh) {
 double* exa_mesh_device_solution(exa_mesh* m) { return m ? m->d_luh : nullptr; }
 
 int exa_mesh_adjust_solution(exa_mesh* m, int ic, const double* params, int nparams) {
-  if (!m || !params || nparams < 7) {
+  if (!m || !params || nparams < 1) {
     exa::set_error("exa_mesh_adjust_solution: invalid argument");
     return EXA_ERR_ARG;
   }
-  if (ic != EXA_IC_DENSITY_WAVE || m->k.pde.id != exa::kPdeEuler || m->k.nvar != 5) {
+  if ((ic != EXA_IC_DENSITY_WAVE && ic != EXA_IC_ENTROPY_WAVE) || m->k.pde.id != exa::kPdeEuler || m->k.nvar != 5) {
     exa::set_error("exa_mesh_adjust_solution: unknown initial condition for this PDE");
     return EXA_ERR_UNSUPPORTED;
+  }
+  const double* d_xi = reinterpret_cast<const double*>(reinterpret_cast<const char*>(m->k.tab) +
+                                                        offsetof(exa::DevTables, x));
+  if (ic == EXA_IC_ENTROPY_WAVE) {
+    const int rc = status(exa::launch_entropy_wave(m->stream, m->ncells, m->n, m->k.dim, m->k.order + 1, m->ND, m->h,
+                                                   d_xi, params[0], m->d_luh),
+                          "entropy wave");
+    if (rc) return rc;
+    HIP_OK(hipStreamSynchronize(m->stream), "sync");
+    return EXA_OK;
+  }
+  if (nparams < 7) {
+    exa::set_error("exa_mesh_adjust_solution: the density wave takes at least 7 params");
+    return EXA_ERR_ARG;
   }
   const int M = static_cast<int>(params[5]);
   if (M < 0 || nparams < 7 + 8 * M) {
@@ -239,8 +308,6 @@ int exa_mesh_adjust_solution(exa_mesh* m, int ic, const double* params, int npar
   double* d_prm = nullptr;
   HIP_OK(hipMalloc(&d_prm, sizeof(double) * nparams), "hipMalloc(params)");
   hipError_t e = hipMemcpyAsync(d_prm, params, sizeof(double) * nparams, hipMemcpyHostToDevice, m->stream);
-  const double* d_xi = reinterpret_cast<const double*>(reinterpret_cast<const char*>(m->k.tab) +
-                                                        offsetof(exa::DevTables, x));
   int rc = EXA_OK;
   if (e == hipSuccess) {
     rc = status(exa::launch_density_wave(m->stream, m->ncells, m->n, m->k.dim, m->k.order + 1, m->ND, m->h, d_xi,
@@ -345,10 +412,54 @@ int exa_mesh_set_fused(exa_mesh* m, int enabled) {
     return EXA_ERR_UNSUPPORTED;
   }
   m->fused = enabled != 0;
-  if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
-  if (m->graph) (void)hipGraphDestroy(m->graph);
-  m->graph_exec = nullptr;
-  m->graph = nullptr;
+  drop_graph(m);
+  return EXA_OK;
+}
+
+int exa_mesh_set_boundary(exa_mesh* m, const int* kinds) {
+  if (!m || !kinds) {
+    exa::set_error("exa_mesh_set_boundary: NULL argument");
+    return EXA_ERR_ARG;
+  }
+  const int D = m->k.dim;
+  exa::BcKinds bc{};
+  unsigned mask = 0;
+  for (int f = 0; f < 2 * D; f++) {
+    if (kinds[f] < EXA_BC_PERIODIC || kinds[f] > EXA_BC_DIRICHLET_ENTROPY_WAVE) {
+      exa::set_error("exa_mesh_set_boundary: unknown boundary kind");
+      return EXA_ERR_ARG;
+    }
+    bc.k[f] = kinds[f];
+    if (kinds[f] != EXA_BC_PERIODIC) mask |= 1u << f;
+  }
+  for (int d = 0; d < D; d++)
+    if ((bc.k[2 * d] == EXA_BC_PERIODIC) != (bc.k[2 * d + 1] == EXA_BC_PERIODIC)) {
+      exa::set_error("exa_mesh_set_boundary: a direction is periodic on both faces or on neither");
+      return EXA_ERR_ARG;
+    }
+  if (mask && !exa::boundary_supported(m->k)) {
+    exa::set_error("exa_mesh_set_boundary: boundary conditions are defined for the Euler equations only");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
+  if (mask && !m->d_gq) {
+    const size_t nghost = size_t(2 * D) * (m->ncells / m->n) * m->NF;
+    HIP_OK(hipMalloc(&m->d_gq, sizeof(double) * nghost * m->ND), "hipMalloc(ghost faces)");
+    HIP_OK(hipMalloc(&m->d_gf, sizeof(double) * nghost * m->k.nvar), "hipMalloc(ghost fluxes)");
+  }
+  m->bc = bc;
+  m->bcmask = mask;
+  const std::vector<int> faces = build_faces(m);
+  m->nfaces = static_cast<long>(faces.size() / 4);
+  HIP_OK(hipMemcpy(m->d_faces, faces.data(), sizeof(int) * faces.size(), hipMemcpyHostToDevice), "upload faces");
+  drop_graph(m);
+  return EXA_OK;
+}
+
+int exa_mesh_set_time(exa_mesh* m, double t) {
+  if (!m) return EXA_ERR_ARG;
+  HIP_OK(hipMemcpyAsync(m->d_state, &t, sizeof(double), hipMemcpyHostToDevice, m->stream), "set_time");
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
   return EXA_OK;
 }
 

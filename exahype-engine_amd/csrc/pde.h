@@ -66,4 +66,24 @@ __device__ __forceinline__ double pde_max_abs_eig(const PdeParams& P, const doub
   return fmax(fmax(l0, l1), l4);
 }
 
+// Exact entropy wave of the Euler example (ApplicationExamples/Euler/Euler/
+// EulerSolver_ADERDG.cpp:266-286): a density bump advected with v0 = (1/2, 0, 0)
+// at pressure 1, rho = 1/2 + 3/10 exp(-|x - x0 - v0 t| / 0.3^D), x0 = (1/2, 1/2, 1/2).
+template <int D>
+__device__ __forceinline__ void euler_entropy_wave(const double* x, double t, double* Q) {
+  const double width = 0.3, v0x = 0.5;
+  double r2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    const double c = x[d] - 0.5 - (d == 0 ? v0x * t : 0.0);
+    r2 += c * c;
+  }
+  const double distance = sqrt(r2);
+  Q[0] = 0.5 + 0.3 * exp(-distance / pow(width, D));
+  Q[1] = Q[0] * v0x;
+  Q[2] = 0.0;
+  Q[3] = 0.0;
+  Q[4] = 1.0 / (1.4 - 1) + 0.5 * Q[0] * (v0x * v0x);
+}
+
 }  // namespace exa

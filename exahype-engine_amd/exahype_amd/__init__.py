@@ -76,7 +76,13 @@ _sig("exa_mesh_set_solution", [_cp, _dp])
 _sig("exa_mesh_get_solution", [_cp, _dp])
 _sig("exa_mesh_device_solution", [_cp], ctypes.c_void_p)
 _sig("exa_mesh_adjust_solution", [_cp, ctypes.c_int, _dp, ctypes.c_int])
-IC_DENSITY_WAVE = 1
+IC_DENSITY_WAVE, IC_ENTROPY_WAVE = 1, 2
+BC_PERIODIC, BC_WALL, BC_DIRICHLET_ENTROPY_WAVE = 0, 1, 2
+_sig("exa_ctx_set_boundary_values", [_cp, ctypes.c_int])
+_sig("exa_boundaryConditions", [_cp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                ctypes.c_int])
+_sig("exa_mesh_set_boundary", [_cp, _ip])
+_sig("exa_mesh_set_time", [_cp, ctypes.c_double])
 _sig("exa_mesh_init_dt", [_cp, ctypes.c_double])
 _sig("exa_mesh_run", [_cp, ctypes.c_int, ctypes.c_int])
 _sig("exa_mesh_sync", [_cp])
@@ -238,6 +244,19 @@ class Context:
                                       _p(np.ascontiguousarray(QR, dtype=np.float64)), t, dt, _p(dxa), direction, 0, 0))
         return FL, FR
 
+    def set_boundary_values(self, kind):
+        """boundaryValues hook of boundaryConditions: BC_WALL or BC_DIRICHLET_ENTROPY_WAVE."""
+        _check(_lib.exa_ctx_set_boundary_values(self.h, kind))
+
+    def boundaryConditions(self, fluxIn, stateIn, centre, dx, t, dt, direction, orientation):
+        """The reference's boundaryConditions for one face of one cell: returns the new fluxIn."""
+        F = np.array(fluxIn, dtype=np.float64)
+        _check(_lib.exa_boundaryConditions(self.h, _p(F), _p(np.ascontiguousarray(stateIn, dtype=np.float64)), None,
+                                           None, _p(np.ascontiguousarray(centre, dtype=np.float64)),
+                                           _p(np.ascontiguousarray(dx, dtype=np.float64)), t, dt, direction,
+                                           orientation))
+        return F
+
     def faceIntegral(self, out, lFhbnd_face, direction, orientation, dx, dt=0.0, addToUpdate=True,
                      subfaceIndex=None, levelDelta=0):
         out = np.array(out, dtype=np.float64)
@@ -356,6 +375,15 @@ class Mesh:
 
     def init_dt(self, dt=0.0):
         _check(_lib.exa_mesh_init_dt(self.h, dt))
+
+    def set_boundary(self, kinds):
+        """Boundary kind per domain face [2*d + o] (BC_*)."""
+        k = np.ascontiguousarray(kinds, dtype=np.int32)
+        assert k.size == 2 * self.ctx.dim
+        _check(_lib.exa_mesh_set_boundary(self.h, k.ctypes.data_as(_ip)))
+
+    def set_time(self, t):
+        _check(_lib.exa_mesh_set_time(self.h, t))
 
     def run(self, nsteps, use_graph=False):
         _check(_lib.exa_mesh_run(self.h, nsteps, int(bool(use_graph))))

@@ -131,6 +131,33 @@ int exa_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* ctx, double* lduh, double
 int exa_riemannSolver(exa_ctx* ctx, double* FL, double* FR, const double* QL, const double* QR, double t,
                       double dt, const double* dx, int direction, int isBoundaryFace, int faceIndex);
 
+/* Boundary kinds of a domain face (Euler).  The two non-periodic kinds are the
+ * boundaryValues hooks of the reference's Euler example
+ * (ApplicationExamples/Euler/Euler/EulerSolver_ADERDG.cpp:525-554):
+ * EXA_BC_WALL reflecting wall (outside state = inside state with the normal
+ * momentum negated, outside flux = its Euler flux; the example's choice for
+ * SodShockTube / SphericalExplosion / RarefactionWave);
+ * EXA_BC_DIRICHLET_ENTROPY_WAVE outside state and flux = Gauss time average
+ * over [t, t + dt] of the exact entropy wave (:266-286; rho = 1/2 + 3/10
+ * exp(-|x - x0 - v0 t| / 0.3^D), x0 = (1/2, 1/2, 1/2), v0 = (1/2, 0, 0),
+ * p = 1; the example's default). */
+enum exa_boundary_kind { EXA_BC_PERIODIC = 0, EXA_BC_WALL = 1, EXA_BC_DIRICHLET_ENTROPY_WAVE = 2 };
+
+/* The boundaryValues hook used by exa_boundaryConditions (default
+ * EXA_BC_DIRICHLET_ENTROPY_WAVE, the example's default). */
+int exa_ctx_set_boundary_values(exa_ctx* ctx, int kind);
+
+/* boundaryConditions (…Implementation.template:307-350 -> generic/c/3d/boundaryConditions.cpph:29-122,
+ * caller ADERDGSolver.cpp:1836-1900).  fluxIn [B^(D-1)][V] (in: the cell's
+ * extrapolated flux on face 2*direction+orientation; out: the Riemann flux),
+ * stateIn [B^(D-1)][nData] the extrapolated state; the outside data come from
+ * the context's boundaryValues hook at the face nodes; the Riemann solve puts
+ * them on the left of a left face (orientation 0) and on the right of a right
+ * face.  gradStateIn and luh are unused (no viscous flux) and may be NULL. */
+int exa_boundaryConditions(exa_ctx* ctx, double* fluxIn, const double* stateIn, const double* gradStateIn,
+                           const double* luh, const double* cellCentre, const double* cellSize, double t, double dt,
+                           int direction, int orientation);
+
 /* faceIntegral (…Implementation.template:247-285 -> generic/c/3d/faceIntegralNonlinear.cpph:21-103).
  * levelDelta == 0: addToUpdate != 0: out is lduh and the face integral is
  * accumulated into it; addToUpdate == 0: out is luh and out += dt/w *
@@ -202,8 +229,11 @@ double* exa_mesh_device_solution(exa_mesh* mesh);
  * EXA_IC_DENSITY_WAVE (Euler): params = {a0, u0, v0, w0, p0, M, amp,
  *   M x (kx, ky, kz, phase) for rho, M x (kx, ky, kz, phase) for u};
  *   rho = 1 + a0 sin(2 pi (x+y[+z])) + amp sum cos(2 pi k.x + phase),
- *   u = u0 + amp sum cos(...), v = v0, w = w0 (0 in 2D), p = p0. */
-enum exa_initial_condition { EXA_IC_DENSITY_WAVE = 1 };
+ *   u = u0 + amp sum cos(...), v = v0, w = w0 (0 in 2D), p = p0.
+ * EXA_IC_ENTROPY_WAVE (Euler): params = {t}; the exact entropy wave of
+ *   EXA_BC_DIRICHLET_ENTROPY_WAVE at time t (the example's initial condition
+ *   at t = 0, EulerSolver_ADERDG.cpp:266-286). */
+enum exa_initial_condition { EXA_IC_DENSITY_WAVE = 1, EXA_IC_ENTROPY_WAVE = 2 };
 int exa_mesh_adjust_solution(exa_mesh* mesh, int ic, const double* params, int nparams);
 /* dt <= 0: compute the first dt from the current solution (stableTimeStepSize). */
 int exa_mesh_init_dt(exa_mesh* mesh, double dt);
@@ -221,6 +251,17 @@ int exa_mesh_picard(exa_mesh* mesh, int* counts);
  * phases (predictor into lduh, Riemann per face, face integrals + update, dt)
  * as separate kernels.  Results agree to rounding. */
 int exa_mesh_set_fused(exa_mesh* mesh, int enabled);
+/* Boundary kind per domain face kinds[2*d + o] (o = 0 at x_d = 0, o = 1 at
+ * x_d = n h), EXA_BC_*; default all periodic.  A direction is periodic on both
+ * faces or on neither.  Non-periodic faces follow the reference's
+ * applyBoundaryConditions (ADERDGSolver.cpp:1836-1900) at the step's t and dt:
+ * after the predictor, the boundaryValues hook writes each boundary face's
+ * outside data into a ghost face, and the Riemann solve uses it as the
+ * neighbour.  Euler only. */
+int exa_mesh_set_boundary(exa_mesh* mesh, const int* kinds);
+/* Set the mesh time t (the Dirichlet data depend on it); exa_mesh_init_dt
+ * resets it to 0. */
+int exa_mesh_set_time(exa_mesh* mesh, double t);
 /* The mesh's HIP stream (hipStream_t as void*). */
 void* exa_mesh_stream(exa_mesh* mesh);
 /* Per-phase timing of the last exa_mesh_run with timing enabled: milliseconds
