@@ -351,7 +351,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--sweep-variants", action="store_true",
-                    help="only time the predictor kernel variants (node-column, line, line+MFMA) and exit")
+                    help="only time the predictor kernel variants (node-column, line, line+MFMA, tile+MFMA) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -387,8 +387,8 @@ def main():
     log(f"[bench] 3D Euler N={args.order} {args.n}^3 cells, steps={args.steps} warmup={args.warmup}")
     if args.sweep_variants:
         for prec, name, pic in ((E.PREC_FP64, "fp64", 0), (E.PREC_FP32, "fp32", 3)):
-            for variant, vname in ((1, "node-column"), (2, "line"), (3, "line+mfma")):
-                if prec == E.PREC_FP64 and variant == 3:
+            for variant, vname in ((1, "node-column"), (2, "line"), (3, "line+mfma"), (4, "tile+mfma")):
+                if prec == E.PREC_FP64 and variant >= 3:
                     continue
                 r = run_form(E, P, args.n, args.order, args.steps, args.warmup, prec, pic, args.seed, args.modes, variant)
                 log(f"[sweep] {name} {vname:12s} predictor {r['predictor_ms']:.2f} ms/step "

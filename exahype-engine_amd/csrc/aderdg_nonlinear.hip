@@ -936,6 +936,7 @@ int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* lu
       return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, fp32, o.cinst, o.luh_update);
     if (v == 2)
       return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, false, o.cinst, o.luh_update);
+    if (v == 4 && fp32) return launch_predictor3d_tile(k, s, n, luh, h, dt, lduh, qb, fb, pic, o.cinst, o.luh_update);
   }
   return e->predictor(k, s, n, luh, h, dt, lduh, qb, fb, pic, o);
 }

@@ -289,7 +289,7 @@ int exa_ctx_set_boundary_values(exa_ctx* c, int kind) {
 }
 
 int exa_ctx_set_predictor_variant(exa_ctx* c, int variant) {
-  if (!c || variant < 0 || variant > 3) {
+  if (!c || variant < 0 || variant > 4) {
     set_error("exa_ctx_set_predictor_variant: invalid argument");
     return EXA_ERR_ARG;
   }

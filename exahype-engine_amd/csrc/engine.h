@@ -39,7 +39,8 @@ struct KernelCfg {
   double cfl, pnpm;
   PdeParams pde;
   const DevTables* tab;  // device pointer
-  int pred_variant;      // 0 auto, 1 node-column, 2 line (VALU), 3 line (FP32 matrix cores)
+  int pred_variant;      // 0 auto, 1 node-column, 2 line (VALU), 3 line (FP32 matrix cores),
+                         // 4 tile (FP32, full-rate matrix cores)
 };
 
 // Launch options of the batched kernels.
@@ -100,6 +101,11 @@ bool lines_supported(const KernelCfg& k);
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
                              const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard,
                              bool mfma, const int* cinst = nullptr, double* luh_update = nullptr);
+// FP32 tile form of the same predictor (3D Euler, order 5): space-time iterate
+// in LDS, contractions on 16x16x4 matrix-core steps; -1 for FP64.
+int launch_predictor3d_tile(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
+                            const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard,
+                            const int* cinst = nullptr, double* luh_update = nullptr);
 // Time bookkeeping of the device step per instance, state = t[ninst], dt[ninst],
 // dt_next[ninst]: t += dt; dt = dt_next; dt_next = DBL_MAX.
 int launch_advance_time(hipStream_t s, double* state, int ninst = 1);

@@ -125,6 +125,123 @@ __device__ __forceinline__ void euler_flux_dir(const R* Q, R irho, R p, int d, R
   F[4] = irho * (Q[4] + p) * qd;
 }
 
+// Post-Picard phase shared by the line and tile kernels (one cell per
+// workgroup): q = previous iterate (lane (t, z, y) < B^3 holds its x-line),
+// tl = final iterate [t][z][y][x][v].  Time averages lQhi = sum_t w_t lQi
+// (final iterate) and lFhi_d = sum_t w_t F_d(previous iterate) (the reference
+// averages the flux of the last iteration, :441-515), extrapolation
+// (:517-687) and the volume integral (volumeIntegralNonlinear.cpph:133-223);
+// buf is the workgroup's LDS buffer (>= 4 B^3 V doubles), tl may lie in it.
+template <int B, typename R>
+__device__ __forceinline__ void finish_cell(R (&q)[B][LShape<B>::V], R* tl, unsigned char* buf, int l, long cell,
+                                            double dt, double h, int iters, const DevTables* __restrict__ T,
+                                            double* __restrict__ lduh, double* __restrict__ lQhbnd,
+                                            double* __restrict__ lFhbnd, int* __restrict__ picard,
+                                            double* luh_upd) {
+  using S = LShape<B>;
+  constexpr int V = S::V, NS = S::NS, NF = S::NF;
+  const bool lane_on = l < NS;
+  const int t = l / (B * B), z = (l / B) % B, y = l % B;
+  const int own = ((t * B + z) * B + y) * B * V;
+  const int tbase = ((z * B + y) * B + t) * V;
+  const double* w = T->w;
+  // ---- time averages: lQhi = sum_t w_t lQi (final iterate),
+  //      lFhi_d = sum_t w_t F_d(previous iterate) ----
+  // component 0 (lQhi) is reduced straight from the tile, which holds the
+  // final iterate on every x-line; components 1..3 overwrite the tile with
+  // F_d of the previous iterate.
+  R acc[4][V];  // reduced values of point (z, y, x = t)
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    launder(q);
+    __syncthreads();
+    if (lane_on && c > 0) {
+#pragma unroll
+      for (int x = 0; x < B; x++) {
+        R F[V];
+        R ir, pr;
+        euler_prim<R>(q[x], ir, pr);
+        euler_flux_dir<R>(q[x], ir, pr, c - 1, F);
+#pragma unroll
+        for (int v = 0; v < V; v++) tl[own + x * V + v] = F[v];
+      }
+    }
+    __syncthreads();
+    if (lane_on) {
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        R a = R(0);
+#pragma unroll
+        for (int n = 0; n < B; n++) a += tl[tbase + n * NS * V + v] * static_cast<R>(w[n]);
+        acc[c][v] = a;
+      }
+    }
+  }
+  __syncthreads();
+  double* Qh = reinterpret_cast<double*>(buf);  // [NS][V]
+  double* Fh = Qh + NS * V;                     // [3][NS][V]
+  const int s = (z * B + y) * B + t;
+  if (lane_on) {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      Qh[s * V + v] = static_cast<double>(acc[0][v]);
+#pragma unroll
+      for (int d = 0; d < 3; d++) Fh[(d * NS + s) * V + v] = static_cast<double>(acc[d + 1][v]);
+    }
+  }
+  __syncthreads();
+  if (lane_on) {
+    // volume integral at point s = (z, y, x = t)
+    const int sz = z, sy = y, sx = t;
+    const double invDx = 1.0 / h;
+    double du[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) du[v] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const double tw = (d == 0) ? w[sz] * w[sy] : (d == 1 ? w[sz] * w[sx] : w[sy] * w[sx]);
+      const double upd = tw * invDx;
+      const int c = (d == 0) ? sx : (d == 1 ? sy : sz);
+      const int stride = (d == 0) ? 1 : (d == 1 ? B : B * B);
+      const int base = s - c * stride;
+#pragma unroll
+      for (int v = 0; v < V; v++)
+#pragma unroll
+        for (int m = 0; m < B; m++)
+          du[v] += T->Kxi[c * B + m] * Fh[(d * NS + base + m * stride) * V + v] * upd;
+    }
+    if (luh_upd) {
+      // volume part of the solution update in place (solutionUpdate.cpph:28-67)
+      const double upd = dt / (w[sz] * w[sy] * w[sx]);
+#pragma unroll
+      for (int v = 0; v < V; v++) luh_upd[(cell * NS + s) * V + v] += du[v] * upd;
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; v++) lduh[(cell * NS + s) * V + v] = du[v];
+    }
+    // extrapolation to the six faces; lane l produces outputs l, l + NS, ...
+    constexpr int NQ = 6 * NF * V;
+    for (int o = l; o < NQ; o += NS) {
+      const int v = o % V;
+      const int fn = (o / V) % NF;
+      const int f = o / (V * NF);
+      const int d = f >> 1;
+      const double* FC = (f & 1) ? T->F1 : T->F0;
+      const int a = fn / B, b = fn % B;
+      double qa = 0.0, fa = 0.0;
+#pragma unroll
+      for (int m = 0; m < B; m++) {
+        const int node = (d == 0) ? (a * B + b) * B + m : (d == 1 ? (a * B + m) * B + b : (m * B + a) * B + b);
+        qa += Qh[node * V + v] * FC[m];
+        fa += Fh[(d * NS + node) * V + v] * FC[m];
+      }
+      lQhbnd[cell * NQ + o] = qa;
+      lFhbnd[cell * NQ + o] = fa;
+    }
+    if (l == 0 && picard) picard[cell] = iters;
+  }
+}
+
 template <int B, typename R, bool MF>
 __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel(
     long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
@@ -357,107 +474,234 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
   }
 
   // ---- after the Picard loop: q = previous iterate, the tile holds the final one ----
-    // ---- time averages: lQhi = sum_t w_t lQi (final iterate),
-    //      lFhi_d = sum_t w_t F_d(previous iterate) ----
-    // component 0 (lQhi) is reduced straight from the tile, which holds the
-    // final iterate on every x-line; components 1..3 overwrite the tile with
-    // F_d of the previous iterate.
-    R acc[4][V];  // reduced values of point (z, y, x = t)
+  finish_cell<B, R>(q, tl, buf, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd);
+}
+
+
+// ---------------------------------------------------------------------------
+// FP32 tile form on full-rate matrix cores (v_mfma_f32_16x16x4_f32).
+//
+// The cell's space-time iterate lQi[t][z][y][x][v] and the flux-divergence
+// accumulator G[t][z][y][x][v] live in two FP32 LDS tiles.  Every contraction
+//   out[i][c] = sum_n op[i][n] X[n][c]      (op = dudx along x, y or z, P along t)
+// runs over pairs of 16-column sets stacked block-diagonally:
+//   rows i' = 8 set + i (i < B; rows B..7 of each set are zero),
+//   k'      = B set + n (K = 2B, ceil(2B / 4) MFMA steps),
+//   A[i'][k'] = op[i][n] when the two sets agree, else 0;  B[k'][j] = X[n][column j of that set].
+// Lane l supplies A[l % 16][4s + l / 16] and B[4s + l / 16][l % 16] in step s and
+// receives D[4 (l / 16) + r][l % 16], r < 4.  In a flux direction the columns
+// are the B^3 lines along d (points (line, n)); the B values a lane supplies
+// are the same points for all five variables, so the lane evaluates the Euler
+// flux there once.  In the time solve the columns are the B^3 V (point,
+// variable) pairs.  Per Picard iteration and cell: 3 * ceil(B^3 / 32) * 5 *
+// ceil(2B / 4) + ceil(B^3 V / 32) * ceil(2B / 4) MFMAs (417 at B = 6) -- the
+// line form issues 4x4x1 MFMAs at a quarter of the matrix-core rate.
+// Same mathematics and Picard rule as the line form (the FP32 sums are
+// ordered differently).
+// ---------------------------------------------------------------------------
+template <int B>
+struct TShape {
+  static constexpr int V = 5;
+  static constexpr int NS = B * B * B;         // lines per direction (t and two spatial indices)
+  static constexpr int TILE = NS * B * V;      // values per space-time tile
+  static constexpr int COLS = NS * V;          // columns of the time solve
+  static constexpr int NK = (2 * B + 3) / 4;   // MFMA steps per contraction
+  static constexpr int PAIRS = (NS + 31) / 32;
+  static constexpr int TPAIRS = (COLS + 31) / 32;
+  static constexpr int THREADS = 256;
+  static constexpr int NWAVE = THREADS / 64;
+  static constexpr int TPW = (TPAIRS + NWAVE - 1) / NWAVE;  // time-solve pairs per wave
+};
+
+// tile position of point n on line L of direction d (lines: t slowest, then the
+// two remaining spatial indices in z, y, x order)
+template <int B>
+__device__ __forceinline__ int line_point(int d, int L, int n) {
+  if (d == 0) return L * B + n;
+  const int lo = L % B, hi = L / B;  // hi = t * B + (z or y)
+  if (d == 1) return (hi * B + n) * B + lo;
+  const int tt = hi / B, mid = hi % B;  // d == 2: L = (t * B + y) * B + x
+  return ((tt * B + n) * B + mid) * B + lo;
+}
+
+template <int B>
+__global__ __launch_bounds__(256) void predictor3d_tile_kernel(
+    long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
+    const int* __restrict__ cinst, double* __restrict__ lduh, double* __restrict__ lQhbnd,
+    double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, int maxIter,
+    int adaptive, double* luh_upd) {
+  using S = TShape<B>;
+  constexpr int V = S::V, NS = S::NS, TILE = S::TILE, COLS = S::COLS, NK = S::NK;
+  static_assert(2 * B <= 4 * NK && B <= 8, "block-diagonal pair must fit 16 rows");
+  constexpr int TILE_BYTES = TILE * 4;
+  constexpr int AVG_BYTES = 4 * NS * V * 8;
+  constexpr int BUF_BYTES = 2 * TILE_BYTES > AVG_BYTES ? 2 * TILE_BYTES : AVG_BYTES;
+  __shared__ __align__(16) unsigned char buf[BUF_BYTES];
+  __shared__ double red[S::NWAVE];
+
+  const int l = threadIdx.x;
+  const int wave = l >> 6, lane = l & 63;
+  const int g = lane >> 4, j = lane & 15;
+  const long cell = blockIdx.x;
+  const double dt = dtp[cinst ? cinst[cell] : 0];
+  const double h = hcell[cell];
+  const float cdt = static_cast<float>(dt / h);
+
+  // per-step operand roles of this lane
+  int kset[NK], kn[NK];
+  bool kok[NK];
+  float aD[NK], aP[NK];
+  {
+    const int rset = j >> 3, ri = j & 7;
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      launder(q);
-      __syncthreads();
-      if (lane_on && c > 0) {
+    for (int s = 0; s < NK; s++) {
+      const int kp = 4 * s + g;
+      kok[s] = kp < 2 * B;
+      kset[s] = kok[s] ? kp / B : 0;
+      kn[s] = kok[s] ? kp % B : 0;
+      const bool on = kok[s] && rset == kset[s] && ri < B;
+      aD[s] = on ? T->dudxf[ri * B + kn[s]] : 0.f;
+      aP[s] = on ? T->Ptf[ri * B + kn[s]] : 0.f;
+    }
+  }
+
+  float* tA = reinterpret_cast<float*>(buf);
+  float* tB = tA + TILE;
+  // initial guess lQi(t) = luh at every time node
+  const double* lc = luh + cell * NS * V;
+  for (int i = l; i < TILE; i += S::THREADS) tA[i] = static_cast<float>(lc[i % COLS]);
+  // luh of the time-solve columns this lane writes (output set = g >> 1)
+  float u[S::TPW];
 #pragma unroll
-        for (int x = 0; x < B; x++) {
-          R F[V];
-          R ir, pr;
-          euler_prim<R>(q[x], ir, pr);
-          euler_flux_dir<R>(q[x], ir, pr, c - 1, F);
+  for (int k = 0; k < S::TPW; k++) {
+    const int c = 32 * (wave + S::NWAVE * k) + 16 * (g >> 1) + j;
+    u[k] = c < COLS ? static_cast<float>(lc[c]) : 0.f;
+  }
+  __syncthreads();
+
+  int iters = maxIter;
+  for (int iter = 0; iter < maxIter; iter++) {
+    // ---- G = sum_d D_d F_d(lQi): tA -> tB ----
 #pragma unroll
-          for (int v = 0; v < V; v++) tl[own + x * V + v] = F[v];
+    for (int d = 0; d < 3; d++) {
+      for (int p = wave; p < S::PAIRS; p += S::NWAVE) {
+        float F[NK][V];
+#pragma unroll
+        for (int s = 0; s < NK; s++) {
+          const int L = 32 * p + 16 * kset[s] + j;
+          if (kok[s] && L < NS) {
+            const float* Q = tA + line_point<B>(d, L, kn[s]) * V;
+            float Qv[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) Qv[v] = Q[v];
+            float ir, pr;
+            euler_prim<float>(Qv, ir, pr);
+            euler_flux_dir<float>(Qv, ir, pr, d, F[s]);
+          } else {
+#pragma unroll
+            for (int v = 0; v < V; v++) F[s][v] = 0.f;
+          }
         }
-      }
-      __syncthreads();
-      if (lane_on) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
-          R a = R(0);
+          exa_f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int n = 0; n < B; n++) a += tl[tbase + n * NS * V + v] * static_cast<R>(w[n]);
-          acc[c][v] = a;
+          for (int s = 0; s < NK; s++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aD[s], F[s][v], acc, 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int ip = 4 * g + r, os = ip >> 3, oi = ip & 7;
+            const int L = 32 * p + 16 * os + j;
+            if (oi < B && L < NS) {
+              const int idx = line_point<B>(d, L, oi) * V + v;
+              tB[idx] = (d == 0) ? acc[r] : tB[idx] + acc[r];
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // ---- time solve lQi' = luh - dt/h P G, in place in tB; residual vs tA ----
+    double sq = 0.0;
+#pragma unroll
+    for (int k = 0; k < S::TPW; k++) {
+      const int p = wave + S::NWAVE * k;
+      if (p < S::TPAIRS) {  // wave-uniform
+        float X[NK];
+#pragma unroll
+        for (int s = 0; s < NK; s++) {
+          const int c = 32 * p + 16 * kset[s] + j;
+          X[s] = (kok[s] && c < COLS) ? tB[kn[s] * COLS + c] : 0.f;
+        }
+        exa_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < NK; s++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[s], X[s], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int ip = 4 * g + r, os = ip >> 3, oi = ip & 7;
+          const int c = 32 * p + 16 * os + j;
+          if (oi < B && c < COLS) {
+            const int idx = oi * COLS + c;
+            const float nv = u[k] - cdt * acc[r];
+            const double dq = static_cast<double>(nv) - static_cast<double>(tA[idx]);
+            sq += dq * dq;
+            tB[idx] = nv;
+          }
         }
       }
     }
     __syncthreads();
-    double* Qh = reinterpret_cast<double*>(buf);  // [NS][V]
-    double* Fh = Qh + NS * V;                     // [3][NS][V]
-    const int s = (z * B + y) * B + t;
-    if (lane_on) {
+    bool last = (iter == maxIter - 1);
+    if (adaptive) {
 #pragma unroll
-      for (int v = 0; v < V; v++) {
-        Qh[s * V + v] = static_cast<double>(acc[0][v]);
+      for (int m = 32; m >= 1; m >>= 1) sq += __shfl_xor(sq, m, 64);
+      if (lane == 0) red[wave] = sq;
+      __syncthreads();
+      double res = 0.0;
 #pragma unroll
-        for (int d = 0; d < 3; d++) Fh[(d * NS + s) * V + v] = static_cast<double>(acc[d + 1][v]);
+      for (int i = 0; i < S::NWAVE; i++) res += red[i];
+      if (res < 1e-7 * 1e-7) {  // block-uniform decision (one cell per block)
+        iters = iter;
+        last = true;
       }
+      __syncthreads();  // red is rewritten next iteration
     }
-    __syncthreads();
-    if (lane_on) {
-      // volume integral at point s = (z, y, x = t)
-      const int sz = z, sy = y, sx = t;
-      const double invDx = 1.0 / h;
-      double du[V];
+    if (last) break;
+    float* tmp = tA;
+    tA = tB;
+    tB = tmp;
+  }
+
+  // ---- tB = final iterate, tA = previous iterate: the line form's post-Picard phase ----
+  constexpr int LV = LShape<B>::V;
+  float q[B][LV];
+  {
+    const bool lane_on = l < NS;
+    const int t = l / (B * B), z = (l / B) % B, y = l % B;
+    const int own = ((t * B + z) * B + y) * B * V;
 #pragma unroll
-      for (int v = 0; v < V; v++) du[v] = 0.0;
+    for (int x = 0; x < B; x++)
 #pragma unroll
-      for (int d = 0; d < 3; d++) {
-        const double tw = (d == 0) ? w[sz] * w[sy] : (d == 1 ? w[sz] * w[sx] : w[sy] * w[sx]);
-        const double upd = tw * invDx;
-        const int c = (d == 0) ? sx : (d == 1 ? sy : sz);
-        const int stride = (d == 0) ? 1 : (d == 1 ? B : B * B);
-        const int base = s - c * stride;
-#pragma unroll
-        for (int v = 0; v < V; v++)
-#pragma unroll
-          for (int m = 0; m < B; m++)
-            du[v] += T->Kxi[c * B + m] * Fh[(d * NS + base + m * stride) * V + v] * upd;
-      }
-      if (luh_upd) {
-        // volume part of the solution update in place (solutionUpdate.cpph:28-67)
-        const double upd = dt / (w[sz] * w[sy] * w[sx]);
-#pragma unroll
-        for (int v = 0; v < V; v++) luh_upd[(cell * NS + s) * V + v] += du[v] * upd;
-      } else {
-#pragma unroll
-        for (int v = 0; v < V; v++) lduh[(cell * NS + s) * V + v] = du[v];
-      }
-      // extrapolation to the six faces; lane l produces outputs l, l + NS, ...
-      constexpr int NQ = 6 * NF * V;
-      for (int o = l; o < NQ; o += NS) {
-        const int v = o % V;
-        const int fn = (o / V) % NF;
-        const int f = o / (V * NF);
-        const int d = f >> 1;
-        const double* FC = (f & 1) ? T->F1 : T->F0;
-        const int a = fn / B, b = fn % B;
-        double qa = 0.0, fa = 0.0;
-#pragma unroll
-        for (int m = 0; m < B; m++) {
-          const int node = (d == 0) ? (a * B + b) * B + m : (d == 1 ? (a * B + m) * B + b : (m * B + a) * B + b);
-          qa += Qh[node * V + v] * FC[m];
-          fa += Fh[(d * NS + node) * V + v] * FC[m];
-        }
-        lQhbnd[cell * NQ + o] = qa;
-        lFhbnd[cell * NQ + o] = fa;
-      }
-      if (l == 0 && picard) picard[cell] = iters;
-    }
+      for (int v = 0; v < V; v++) q[x][v] = lane_on ? tA[own + x * V + v] : 0.f;
+  }
+  finish_cell<B, float>(q, tB, buf, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd);
 }
 
 }  // namespace
 
 bool lines_supported(const KernelCfg& k) {
   return k.dim == 3 && k.nvar == 5 && k.npar == 0 && k.order + 1 == 6 && k.pde.id == kPdeEuler;
+}
+
+int launch_predictor3d_tile(const KernelCfg& k, hipStream_t st, long n, const double* luh, const double* h,
+                            const double* dt, double* lduh, double* qb, double* fb, int* pic, const int* cinst,
+                            double* luh_upd) {
+  constexpr int B = 6;
+  if (k.precision == 0) return -1;  // FP32 form only
+  const int adaptive = k.picard_max <= 0;
+  const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
+  hipLaunchKernelGGL((predictor3d_tile_kernel<B>), dim3(n), dim3(TShape<B>::THREADS), 0, st, n, luh, h, dt, cinst,
+                     lduh, qb, fb, pic, k.tab, maxIter, adaptive, luh_upd);
+  return static_cast<int>(hipGetLastError());
 }
 
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const double* luh, const double* h,

@@ -87,8 +87,10 @@ int exa_ctx_set_point_source(exa_ctx* ctx, const double* x, double amp, double f
 int exa_ctx_set_advection(exa_ctx* ctx, const double* a);
 
 /* Predictor kernel selection: 0 (default) picks the fastest kernel compiled
- * for the shape, 1 forces the generic node-column kernel (every shape).
- * Meshes copy the context's settings when they are created. */
+ * for the shape, 1 forces the generic node-column kernel (every shape); for
+ * 3D Euler order 5 also 2 = x-line kernel (VALU), 3 = x-line kernel on 4x4x1
+ * matrix-core steps (FP32), 4 = space-time tile kernel on 16x16x4 matrix-core
+ * steps (FP32).  Meshes copy the context's settings when they are created. */
 int exa_ctx_set_predictor_variant(exa_ctx* ctx, int variant);
 
 /* Operator tables of order N derived on the host (no device needed; each

@@ -317,10 +317,12 @@ def test_device_initial_condition_matches_host(dim, n, order):
 
 
 @pytest.mark.parametrize("precision,variant,tol", [(E.PREC_FP64, 2, 1e-13), (E.PREC_FP32, 2, 2e-6),
-                                                   (E.PREC_FP32, 3, 2e-6), (E.PREC_FP32, 0, 2e-6)])
+                                                   (E.PREC_FP32, 3, 2e-6), (E.PREC_FP32, 4, 2e-6),
+                                                   (E.PREC_FP32, 0, 2e-6)])
 def test_predictor_variants_agree(precision, variant, tol):
     """The line-decomposed 3D order-5 predictor (VALU and matrix-core forms)
-    agrees with the generic node-column kernel."""
+    and the FP32 space-time tile predictor agree with the generic node-column
+    kernel."""
     n, order, steps = 4, 5, 2
     luh = P.density_wave_3d(n, order)
     outs = []
