@@ -113,6 +113,10 @@ int launch_advance_time(hipStream_t s, double* state, int ninst = 1);
 // Boundary conditions (Euler): outside face data of every boundary face of a
 // uniform n^D mesh (ghost layout of LaunchOpts::ghost_q) at the step's t, dt
 // read from state[0], state[1]; or of one face of one cell.
+// Point probes: values [npts][nData] of the solution of a uniform n^D mesh at
+// points [npts][dim] inside the domain (tensor-product Lagrange interpolation).
+int launch_probe(const KernelCfg& k, hipStream_t s, int n, double h, const double* luh, int npts, const double* pts,
+                 double* out);
 bool boundary_supported(const KernelCfg& k);
 int launch_boundary_mesh(const KernelCfg& k, hipStream_t s, int n, double h, const BcKinds& bk, const double* qb,
                          double* gq, double* gf, const double* state);

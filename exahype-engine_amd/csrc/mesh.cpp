@@ -456,6 +456,37 @@ int exa_mesh_set_boundary(exa_mesh* m, const int* kinds) {
   return EXA_OK;
 }
 
+int exa_mesh_probe(exa_mesh* m, int npoints, const double* points, double* values) {
+  if (!m || npoints < 0 || (npoints > 0 && (!points || !values))) {
+    exa::set_error("exa_mesh_probe: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  if (npoints == 0) return EXA_OK;
+  const int D = m->k.dim;
+  const double L = m->n * m->h;
+  for (long i = 0; i < long(npoints) * D; i++)
+    if (!(points[i] >= 0.0 && points[i] < L)) {
+      exa::set_error("exa_mesh_probe: a point lies outside the mesh");
+      return EXA_ERR_ARG;
+    }
+  double *d_pts = nullptr, *d_out = nullptr;
+  const size_t npb = sizeof(double) * npoints * D, nob = sizeof(double) * npoints * m->ND;
+  hipError_t e = hipMalloc(&d_pts, npb);
+  if (e == hipSuccess) e = hipMalloc(&d_out, nob);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_pts, points, npb, hipMemcpyHostToDevice, m->stream);
+  int rc = EXA_OK;
+  if (e == hipSuccess) {
+    rc = status(exa::launch_probe(m->k, m->stream, m->n, m->h, m->d_luh, npoints, d_pts, d_out), "probe");
+    if (!rc) e = hipMemcpyAsync(values, d_out, nob, hipMemcpyDeviceToHost, m->stream);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  }
+  if (d_pts) (void)hipFree(d_pts);
+  if (d_out) (void)hipFree(d_out);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(e, "exa_mesh_probe");
+  return EXA_OK;
+}
+
 int exa_mesh_set_time(exa_mesh* m, double t) {
   if (!m) return EXA_ERR_ARG;
   HIP_OK(hipMemcpyAsync(m->d_state, &t, sizeof(double), hipMemcpyHostToDevice, m->stream), "set_time");

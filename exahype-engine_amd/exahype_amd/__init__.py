@@ -83,6 +83,7 @@ _sig("exa_boundaryConditions", [_cp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_doub
                                 ctypes.c_int])
 _sig("exa_mesh_set_boundary", [_cp, _ip])
 _sig("exa_mesh_set_time", [_cp, ctypes.c_double])
+_sig("exa_mesh_probe", [_cp, ctypes.c_int, _dp, _dp])
 _sig("exa_mesh_init_dt", [_cp, ctypes.c_double])
 _sig("exa_mesh_run", [_cp, ctypes.c_int, ctypes.c_int])
 _sig("exa_mesh_sync", [_cp])
@@ -384,6 +385,13 @@ class Mesh:
 
     def set_time(self, t):
         _check(_lib.exa_mesh_set_time(self.h, t))
+
+    def probe(self, points):
+        """Solution values [npoints][nData] at points [npoints][dim] (Lagrange interpolation)."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, self.ctx.dim)
+        out = np.zeros((pts.shape[0], self.ctx.nd))
+        _check(_lib.exa_mesh_probe(self.h, pts.shape[0], _p(pts), _p(out)))
+        return out
 
     def run(self, nsteps, use_graph=False):
         _check(_lib.exa_mesh_run(self.h, nsteps, int(bool(use_graph))))

@@ -264,6 +264,11 @@ int exa_mesh_set_boundary(exa_mesh* mesh, const int* kinds);
 /* Set the mesh time t (the Dirichlet data depend on it); exa_mesh_init_dt
  * resets it to 0. */
 int exa_mesh_set_time(exa_mesh* mesh, double t);
+/* Point probes (the reference's probe plotter, plotters/ADERDG2ProbeAscii.cpp:127-180 with
+ * kernels::legendre::interpolate, GaussLegendreBasis.cpp:21-49): values[npoints][nData] of the
+ * current solution at points[npoints][dim], each inside [0, n h)^dim, interpolated with the
+ * cell's Lagrange basis in the cell with offset <= x < offset + h.  Synchronous. */
+int exa_mesh_probe(exa_mesh* mesh, int npoints, const double* points, double* values);
 /* The mesh's HIP stream (hipStream_t as void*). */
 void* exa_mesh_stream(exa_mesh* mesh);
 /* Per-phase timing of the last exa_mesh_run with timing enabled: milliseconds

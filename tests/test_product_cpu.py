@@ -72,3 +72,30 @@ def test_invalid_arguments_return_error_codes():
     assert lib.exa_tables(-1, *([None] * 9)) == E.EXA_ERR_ARG
     with pytest.raises(E.ExaError):
         E.tables(40)
+
+
+def test_probe_ascii_format(tmp_path):
+    """The probe writer follows the reference's probe::ascii layout
+    (plotters/ADERDG2ProbeAscii.cpp: header, then plot time, time stamp, values)."""
+    from exahype_amd.plotters import ProbeAscii
+
+    class FakeMesh:
+        def __init__(self):
+            self.t = 0.0
+
+        def time(self):
+            return self.t, 0.01
+
+        def probe(self, pts):
+            return np.array([[1.0, 0.5, 1e-5, 0.0, 2.5 + self.t]])
+
+    m = FakeMesh()
+    w = ProbeAscii(str(tmp_path / "rho"), [0.3, 0.4, 0.5], 5)
+    w.plot(m)
+    m.t = 0.125
+    w.plot(m)
+    w.close()
+    lines = (tmp_path / "rho.probe").read_text().splitlines()
+    assert lines[0] == "# plot-time, real-time,Q0,Q1,Q2,Q3,Q4"
+    assert lines[1] == "0, 0, 1, 0.5, 1e-05, 0, 2.5"
+    assert lines[2] == "0.125, 0.125, 1, 0.5, 1e-05, 0, 2.625"
