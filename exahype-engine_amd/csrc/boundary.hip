@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void boundary_mesh_kernel(int n, int B, int ND
   const int f = static_cast<int>(g / per);
   const long tr = g - f * per;
   const int kind = bk.k[f];
-  if (kind == kBcPeriodic) return;
+  if (kind == kBcPeriodic || kind == kBcExchange) return;  // exchange faces: ghost data come from a halo message
   const int d = f >> 1, o = f & 1;
   const long id = o ? n - 1 : 0;
   long ix = 0, iy = 0, iz = 0;
@@ -136,6 +136,37 @@ __global__ __launch_bounds__(256) void boundary_mesh_kernel(int n, int B, int ND
   face_node_x<D>(centre, dx, d, o, fn, B, T->x, x);
   boundary_node<D>(kind, x, state[0], state[1], d, ND, B, qb + ((cell * 2 * D + f) * NF + fn) * ND,
                    gq + (g * NF + fn) * ND, gf + (g * NF + fn) * kV, T, P);
+}
+
+// cell on domain face f = 2d + o with transverse index tr (ghost-face order)
+template <int D>
+__device__ __forceinline__ long boundary_cell(int n, int f, long tr) {
+  const int d = f >> 1, o = f & 1;
+  const long id = o ? n - 1 : 0;
+  if (D == 3) {
+    const long a = tr / n, b = tr - (tr / n) * n;
+    if (d == 0) return (a * n + b) * n + id;
+    if (d == 1) return (a * n + id) * n + b;
+    return (id * n + a) * n + b;
+  }
+  return (d == 0) ? tr * n + id : id * n + tr;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void halo_pack_kernel(int n, int f, int NF, int ND, int V,
+                                                        const double* __restrict__ qb, const double* __restrict__ fb,
+                                                        double* __restrict__ msg_q, double* __restrict__ msg_f) {
+  const long per = (D == 3) ? long(n) * n : long(n);
+  const long nq = per * NF * ND, nf = per * NF * V;
+  const long gid = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid < nq) {
+    const long tr = gid / (long(NF) * ND), r = gid - tr * NF * ND;
+    msg_q[gid] = qb[(boundary_cell<D>(n, f, tr) * 2 * D + f) * NF * ND + r];
+  } else if (gid < nq + nf) {
+    const long i = gid - nq;
+    const long tr = i / (long(NF) * V), r = i - tr * NF * V;
+    msg_f[i] = fb[(boundary_cell<D>(n, f, tr) * 2 * D + f) * NF * V + r];
+  }
 }
 
 struct Geometry {
@@ -189,6 +220,20 @@ int launch_boundary_face(const KernelCfg& k, hipStream_t s, int kind, const doub
   else
     hipLaunchKernelGGL(boundary_face_kernel<2>, dim3(1), dim3(64), 0, s, kind, geo, t, dt, d, o, B, ND, qin, gq, gf,
                        k.tab, k.pde);
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_halo_pack(const KernelCfg& k, hipStream_t s, int n, int f, const double* qb, const double* fb,
+                     double* msg_q, double* msg_f) {
+  const int D = k.dim, B = k.order + 1, ND = k.nvar + k.npar, V = k.nvar;
+  const int NF = (D == 3) ? B * B : B;
+  const long per = (D == 3) ? long(n) * n : long(n);
+  const long total = per * NF * (ND + V);
+  const dim3 grid(static_cast<unsigned>((total + 255) / 256)), block(256);
+  if (D == 3)
+    hipLaunchKernelGGL(halo_pack_kernel<3>, grid, block, 0, s, n, f, NF, ND, V, qb, fb, msg_q, msg_f);
+  else
+    hipLaunchKernelGGL(halo_pack_kernel<2>, grid, block, 0, s, n, f, NF, ND, V, qb, fb, msg_q, msg_f);
   return static_cast<int>(hipGetLastError());
 }
 

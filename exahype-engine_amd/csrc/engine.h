@@ -67,7 +67,7 @@ struct LaunchOpts {
 };
 
 // Boundary kinds per domain face f = 2d + o (exa_boundary_kind).
-enum BcKind { kBcPeriodic = 0, kBcWall = 1, kBcDirichletEntropyWave = 2 };
+enum BcKind { kBcPeriodic = 0, kBcWall = 1, kBcDirichletEntropyWave = 2, kBcExchange = 3 };
 struct BcKinds {
   int k[6];
 };
@@ -122,6 +122,11 @@ int launch_boundary_mesh(const KernelCfg& k, hipStream_t s, int n, double h, con
                          double* gq, double* gf, const double* state);
 int launch_boundary_face(const KernelCfg& k, hipStream_t s, int kind, const double* centre, const double* dx,
                          double t, double dt, int d, int o, const double* qin, double* gq, double* gf);
+// Halo message of domain face f of a uniform n^D mesh: the extrapolated
+// predictor faces [n^(D-1)][NF][ND] and fluctuation faces [n^(D-1)][NF][V] of
+// the cells on that face, in transverse-index order (the ghost-face order).
+int launch_halo_pack(const KernelCfg& k, hipStream_t s, int n, int f, const double* qb, const double* fb,
+                     double* msg_q, double* msg_f);
 // Initial condition EXA_IC_DENSITY_WAVE on a uniform n^D mesh (initial.hip).
 int launch_density_wave(hipStream_t s, long ncells, int n, int D, int B, int ND, double h, const double* d_xi,
                         const double* d_prm, double* luh);

@@ -49,6 +49,7 @@ struct exa_mesh {
   // boundary kinds per domain face; ghost faces [2D][n^(D-1)][NF][ND | V]
   exa::BcKinds bc{};
   unsigned bcmask = 0;
+  bool hooks = false;  // some face takes its ghost data from a boundary-value hook
   double *d_gq = nullptr, *d_gf = nullptr;
 };
 
@@ -75,7 +76,7 @@ int status(int rc, const char* what) {
 }
 
 int enqueue_boundary(exa_mesh* m) {
-  if (!m->bcmask) return EXA_OK;
+  if (!m->hooks) return EXA_OK;
   return status(exa::launch_boundary_mesh(m->k, m->stream, m->n, m->h, m->bc, m->d_qb, m->d_gq, m->d_gf, m->d_state),
                 "boundary conditions");
 }
@@ -90,50 +91,53 @@ exa::LaunchOpts ghost_opts(const exa_mesh* m) {
   return o;
 }
 
-// Enqueue one global time step on m->stream.
-int enqueue_step(exa_mesh* m, hipEvent_t* ev) {
+// One global time step on m->stream in three parts, so that a caller can
+// exchange halo faces between predict and correct and reduce dt_next across
+// processes between correct and advance.  Events (timing): ev[0] before the
+// predictor, ev[1] after it, ev[2] after the Riemann solves (four-kernel form)
+// or before the fused corrector, ev[3] after the step.
+int enqueue_predict(exa_mesh* m, hipEvent_t* ev) {
   double* st = m->d_state;
-  const bool with_events = ev != nullptr;
-  if (with_events) HIP_OK(hipEventRecord(ev[0], m->stream), "event");
-  if (m->fused) {
-    exa::LaunchOpts o;
-    o.luh_update = m->d_luh;
-    int rc = status(exa::launch_predictor(m->k, m->stream, m->ncells, m->d_luh, m->d_h, st + 1, m->d_lduh, m->d_qb,
-                                          m->d_fb, m->d_picard, o),
-                    "predictor");
-    if (rc) return rc;
-    if (with_events) HIP_OK(hipEventRecord(ev[1], m->stream), "event");
-    rc = enqueue_boundary(m);
-    if (rc) return rc;
-    if (with_events) HIP_OK(hipEventRecord(ev[2], m->stream), "event");
-    rc = status(exa::launch_corrector_riemann(m->k, m->stream, m->ncells, m->n, m->d_luh, m->d_qb, m->d_fb, m->d_h,
-                                              st + 1, st + 2, ghost_opts(m)),
-                "corrector");
-    if (rc) return rc;
-    rc = status(exa::launch_advance_time(m->stream, st), "advance time");
-    if (rc) return rc;
-    if (with_events) HIP_OK(hipEventRecord(ev[3], m->stream), "event");
-    return EXA_OK;
-  }
+  if (ev) HIP_OK(hipEventRecord(ev[0], m->stream), "event");
+  exa::LaunchOpts o;
+  if (m->fused) o.luh_update = m->d_luh;
   int rc = status(exa::launch_predictor(m->k, m->stream, m->ncells, m->d_luh, m->d_h, st + 1, m->d_lduh, m->d_qb,
-                                        m->d_fb, m->d_picard),
+                                        m->d_fb, m->d_picard, o),
                   "predictor");
   if (rc) return rc;
-  if (with_events) HIP_OK(hipEventRecord(ev[1], m->stream), "event");
-  rc = enqueue_boundary(m);
+  if (ev) HIP_OK(hipEventRecord(ev[1], m->stream), "event");
+  return enqueue_boundary(m);
+}
+
+int enqueue_correct(exa_mesh* m, hipEvent_t* ev) {
+  double* st = m->d_state;
+  if (m->fused) {
+    if (ev) HIP_OK(hipEventRecord(ev[2], m->stream), "event");
+    return status(exa::launch_corrector_riemann(m->k, m->stream, m->ncells, m->n, m->d_luh, m->d_qb, m->d_fb, m->d_h,
+                                                st + 1, st + 2, ghost_opts(m)),
+                  "corrector");
+  }
+  int rc = status(exa::launch_riemann(m->k, m->stream, m->nfaces, m->d_faces, m->d_qb, m->d_fb, ghost_opts(m)),
+                  "riemann");
   if (rc) return rc;
-  rc = status(exa::launch_riemann(m->k, m->stream, m->nfaces, m->d_faces, m->d_qb, m->d_fb, ghost_opts(m)),
-              "riemann");
+  if (ev) HIP_OK(hipEventRecord(ev[2], m->stream), "event");
+  return status(exa::launch_corrector(m->k, m->stream, m->ncells, m->d_luh, m->d_lduh, m->d_fb, m->d_h, st + 1,
+                                      st + 2),
+                "corrector");
+}
+
+int enqueue_advance(exa_mesh* m, hipEvent_t* ev) {
+  const int rc = status(exa::launch_advance_time(m->stream, m->d_state), "advance time");
   if (rc) return rc;
-  if (with_events) HIP_OK(hipEventRecord(ev[2], m->stream), "event");
-  rc = status(exa::launch_corrector(m->k, m->stream, m->ncells, m->d_luh, m->d_lduh, m->d_fb, m->d_h, st + 1,
-                                    st + 2),
-              "corrector");
-  if (rc) return rc;
-  rc = status(exa::launch_advance_time(m->stream, st), "advance time");
-  if (rc) return rc;
-  if (with_events) HIP_OK(hipEventRecord(ev[3], m->stream), "event");
+  if (ev) HIP_OK(hipEventRecord(ev[3], m->stream), "event");
   return EXA_OK;
+}
+
+int enqueue_step(exa_mesh* m, hipEvent_t* ev) {
+  int rc = enqueue_predict(m, ev);
+  if (!rc) rc = enqueue_correct(m, ev);
+  if (!rc) rc = enqueue_advance(m, ev);
+  return rc;
 }
 
 void free_mesh(exa_mesh* m) {
@@ -425,7 +429,7 @@ int exa_mesh_set_boundary(exa_mesh* m, const int* kinds) {
   exa::BcKinds bc{};
   unsigned mask = 0;
   for (int f = 0; f < 2 * D; f++) {
-    if (kinds[f] < EXA_BC_PERIODIC || kinds[f] > EXA_BC_DIRICHLET_ENTROPY_WAVE) {
+    if (kinds[f] < EXA_BC_PERIODIC || kinds[f] > EXA_BC_EXCHANGE) {
       exa::set_error("exa_mesh_set_boundary: unknown boundary kind");
       return EXA_ERR_ARG;
     }
@@ -437,8 +441,10 @@ int exa_mesh_set_boundary(exa_mesh* m, const int* kinds) {
       exa::set_error("exa_mesh_set_boundary: a direction is periodic on both faces or on neither");
       return EXA_ERR_ARG;
     }
-  if (mask && !exa::boundary_supported(m->k)) {
-    exa::set_error("exa_mesh_set_boundary: boundary conditions are defined for the Euler equations only");
+  bool hooks = false;
+  for (int f = 0; f < 2 * D; f++) hooks = hooks || bc.k[f] == EXA_BC_WALL || bc.k[f] == EXA_BC_DIRICHLET_ENTROPY_WAVE;
+  if (hooks && !exa::boundary_supported(m->k)) {
+    exa::set_error("exa_mesh_set_boundary: the boundary-value hooks are defined for the Euler equations only");
     return EXA_ERR_UNSUPPORTED;
   }
   HIP_OK(hipStreamSynchronize(m->stream), "sync");
@@ -449,6 +455,7 @@ int exa_mesh_set_boundary(exa_mesh* m, const int* kinds) {
   }
   m->bc = bc;
   m->bcmask = mask;
+  m->hooks = hooks;
   const std::vector<int> faces = build_faces(m);
   m->nfaces = static_cast<long>(faces.size() / 4);
   HIP_OK(hipMemcpy(m->d_faces, faces.data(), sizeof(int) * faces.size(), hipMemcpyHostToDevice), "upload faces");
@@ -484,6 +491,71 @@ int exa_mesh_probe(exa_mesh* m, int npoints, const double* points, double* value
   if (d_out) (void)hipFree(d_out);
   if (rc) return rc;
   if (e != hipSuccess) return fail(e, "exa_mesh_probe");
+  return EXA_OK;
+}
+
+int exa_mesh_step_predict(exa_mesh* m) {
+  if (!m) return EXA_ERR_ARG;
+  return enqueue_predict(m, nullptr);
+}
+
+int exa_mesh_step_correct(exa_mesh* m) {
+  if (!m) return EXA_ERR_ARG;
+  return enqueue_correct(m, nullptr);
+}
+
+int exa_mesh_step_advance(exa_mesh* m) {
+  if (!m) return EXA_ERR_ARG;
+  return enqueue_advance(m, nullptr);
+}
+
+int exa_mesh_halo_size(const exa_mesh* m, long* n_q, long* n_f) {
+  if (!m) return EXA_ERR_ARG;
+  const long per = m->ncells / m->n;
+  if (n_q) *n_q = per * m->NF * m->ND;
+  if (n_f) *n_f = per * m->NF * m->k.nvar;
+  return EXA_OK;
+}
+
+int exa_mesh_halo_pack(exa_mesh* m, int face, double* d_msg_q, double* d_msg_f) {
+  if (!m || !d_msg_q || !d_msg_f || face < 0 || face >= 2 * m->k.dim) {
+    exa::set_error("exa_mesh_halo_pack: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  return status(exa::launch_halo_pack(m->k, m->stream, m->n, face, m->d_qb, m->d_fb, d_msg_q, d_msg_f), "halo pack");
+}
+
+int exa_mesh_halo_unpack(exa_mesh* m, int face, const double* d_msg_q, const double* d_msg_f) {
+  if (!m || !d_msg_q || !d_msg_f || face < 0 || face >= 2 * m->k.dim) {
+    exa::set_error("exa_mesh_halo_unpack: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  if (m->bc.k[face] != EXA_BC_EXCHANGE) {
+    exa::set_error("exa_mesh_halo_unpack: the face is not of kind EXA_BC_EXCHANGE");
+    return EXA_ERR_ARG;
+  }
+  long nq = 0, nf = 0;
+  exa_mesh_halo_size(m, &nq, &nf);
+  HIP_OK(hipMemcpyAsync(m->d_gq + size_t(face) * nq, d_msg_q, sizeof(double) * nq, hipMemcpyDeviceToDevice, m->stream),
+         "halo unpack");
+  HIP_OK(hipMemcpyAsync(m->d_gf + size_t(face) * nf, d_msg_f, sizeof(double) * nf, hipMemcpyDeviceToDevice, m->stream),
+         "halo unpack");
+  return EXA_OK;
+}
+
+double* exa_mesh_device_state(exa_mesh* m) { return m ? m->d_state : nullptr; }
+
+int exa_mesh_dt_next(exa_mesh* m, double* dt_next) {
+  if (!m || !dt_next) return EXA_ERR_ARG;
+  HIP_OK(hipMemcpyAsync(dt_next, m->d_state + 2, sizeof(double), hipMemcpyDeviceToHost, m->stream), "dt_next");
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
+  return EXA_OK;
+}
+
+int exa_mesh_set_dt_next(exa_mesh* m, double dt_next) {
+  if (!m || !(dt_next > 0)) return EXA_ERR_ARG;
+  HIP_OK(hipMemcpyAsync(m->d_state + 2, &dt_next, sizeof(double), hipMemcpyHostToDevice, m->stream), "dt_next");
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
   return EXA_OK;
 }
 

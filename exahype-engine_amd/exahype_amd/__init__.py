@@ -84,6 +84,16 @@ _sig("exa_boundaryConditions", [_cp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_doub
 _sig("exa_mesh_set_boundary", [_cp, _ip])
 _sig("exa_mesh_set_time", [_cp, ctypes.c_double])
 _sig("exa_mesh_probe", [_cp, ctypes.c_int, _dp, _dp])
+BC_EXCHANGE = 3
+_sig("exa_mesh_step_predict", [_cp])
+_sig("exa_mesh_step_correct", [_cp])
+_sig("exa_mesh_step_advance", [_cp])
+_sig("exa_mesh_halo_size", [_cp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)])
+_sig("exa_mesh_halo_pack", [_cp, ctypes.c_int, _vp, _vp])
+_sig("exa_mesh_halo_unpack", [_cp, ctypes.c_int, _vp, _vp])
+_sig("exa_mesh_device_state", [_cp], ctypes.c_void_p)
+_sig("exa_mesh_dt_next", [_cp, _dp])
+_sig("exa_mesh_set_dt_next", [_cp, ctypes.c_double])
 _sig("exa_mesh_init_dt", [_cp, ctypes.c_double])
 _sig("exa_mesh_run", [_cp, ctypes.c_int, ctypes.c_int])
 _sig("exa_mesh_sync", [_cp])
@@ -385,6 +395,36 @@ class Mesh:
 
     def set_time(self, t):
         _check(_lib.exa_mesh_set_time(self.h, t))
+
+    # ---- partitioned stepping with halo messages ----
+    def step_predict(self):
+        _check(_lib.exa_mesh_step_predict(self.h))
+
+    def step_correct(self):
+        _check(_lib.exa_mesh_step_correct(self.h))
+
+    def step_advance(self):
+        _check(_lib.exa_mesh_step_advance(self.h))
+
+    def halo_size(self):
+        a, b = ctypes.c_long(0), ctypes.c_long(0)
+        _check(_lib.exa_mesh_halo_size(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def halo_pack(self, face, d_msg_q, d_msg_f):
+        """Device pointers (ints) of the two message buffers."""
+        _check(_lib.exa_mesh_halo_pack(self.h, face, ctypes.c_void_p(d_msg_q), ctypes.c_void_p(d_msg_f)))
+
+    def halo_unpack(self, face, d_msg_q, d_msg_f):
+        _check(_lib.exa_mesh_halo_unpack(self.h, face, ctypes.c_void_p(d_msg_q), ctypes.c_void_p(d_msg_f)))
+
+    def dt_next(self):
+        v = ctypes.c_double(0)
+        _check(_lib.exa_mesh_dt_next(self.h, ctypes.byref(v)))
+        return v.value
+
+    def set_dt_next(self, v):
+        _check(_lib.exa_mesh_set_dt_next(self.h, v))
 
     def probe(self, points):
         """Solution values [npoints][nData] at points [npoints][dim] (Lagrange interpolation)."""

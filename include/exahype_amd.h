@@ -143,7 +143,12 @@ int exa_riemannSolver(exa_ctx* ctx, double* FL, double* FR, const double* QL, co
  * over [t, t + dt] of the exact entropy wave (:266-286; rho = 1/2 + 3/10
  * exp(-|x - x0 - v0 t| / 0.3^D), x0 = (1/2, 1/2, 1/2), v0 = (1/2, 0, 0),
  * p = 1; the example's default). */
-enum exa_boundary_kind { EXA_BC_PERIODIC = 0, EXA_BC_WALL = 1, EXA_BC_DIRICHLET_ENTROPY_WAVE = 2 };
+enum exa_boundary_kind {
+  EXA_BC_PERIODIC = 0,
+  EXA_BC_WALL = 1,
+  EXA_BC_DIRICHLET_ENTROPY_WAVE = 2,
+  EXA_BC_EXCHANGE = 3 /* mesh faces only: outside data from a halo message (exa_mesh_halo_unpack) */
+};
 
 /* The boundaryValues hook used by exa_boundaryConditions (default
  * EXA_BC_DIRICHLET_ENTROPY_WAVE, the example's default). */
@@ -264,6 +269,28 @@ int exa_mesh_set_boundary(exa_mesh* mesh, const int* kinds);
 /* Set the mesh time t (the Dirichlet data depend on it); exa_mesh_init_dt
  * resets it to 0. */
 int exa_mesh_set_time(exa_mesh* mesh, double t);
+/* Halo exchange of a partitioned domain (the reference's neighbour messages,
+ * ADERDGSolver.cpp:1946-2096: per face of a cell the extrapolated predictor
+ * lQhbnd [B^(D-1)][nData], then the fluctuation lFhbnd [B^(D-1)][V]).  A face of
+ * kind EXA_BC_EXCHANGE takes its outside data from a message: per step call
+ * exa_mesh_step_predict, pack the faces the neighbours need
+ * (exa_mesh_halo_pack: the lQhbnd message [n^(D-1) cells][B^(D-1)][nData] and
+ * the lFhbnd message [n^(D-1)][B^(D-1)][V] of domain face `face`, cells in
+ * transverse order (iz,iy) / (iz,ix) / (iy,ix), 2D iy / ix), send them,
+ * exa_mesh_halo_unpack the received ones (device pointers, same layout, the
+ * neighbour's opposite face), exa_mesh_step_correct, reduce dt_next (device
+ * state {t, dt, dt_next}) to the global minimum, exa_mesh_step_advance.  All
+ * enqueue on the mesh stream; exa_mesh_run does the three parts per step. */
+int exa_mesh_step_predict(exa_mesh* mesh);
+int exa_mesh_step_correct(exa_mesh* mesh);
+int exa_mesh_step_advance(exa_mesh* mesh);
+int exa_mesh_halo_size(const exa_mesh* mesh, long* n_q, long* n_f);
+int exa_mesh_halo_pack(exa_mesh* mesh, int face, double* d_msg_q, double* d_msg_f);
+int exa_mesh_halo_unpack(exa_mesh* mesh, int face, const double* d_msg_q, const double* d_msg_f);
+double* exa_mesh_device_state(exa_mesh* mesh);
+/* Host access to dt_next between correct and advance (synchronous). */
+int exa_mesh_dt_next(exa_mesh* mesh, double* dt_next);
+int exa_mesh_set_dt_next(exa_mesh* mesh, double dt_next);
 /* Point probes (the reference's probe plotter, plotters/ADERDG2ProbeAscii.cpp:127-180 with
  * kernels::legendre::interpolate, GaussLegendreBasis.cpp:21-49): values[npoints][nData] of the
  * current solution at points[npoints][dim], each inside [0, n h)^dim, interpolated with the

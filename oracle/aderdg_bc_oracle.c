@@ -127,17 +127,20 @@ void orc_boundary_conditions(const orc_solver* s, int kind, double* FIn, const d
   free(FOut);
 }
 
-/* Uniform n^D mesh on [0, n h]^D with a boundary kind per domain face
- * (bc[2d+o], ORC_BC_*): one global step per iteration in the order of
- * orc_run_uniform, with the boundary faces of the non-periodic directions
- * solved by orc_boundary_conditions at the step's (t, dt).  A direction is
- * periodic iff both of its faces are ORC_BC_PERIODIC. */
-int orc_run_bc(const orc_solver* s, double* luh, int n, double h, int steps, double* dt_io, double* t_io,
-               const int* bc, int threads) {
+/* Uniform box of nx x ny (x nz) cells of size h with a boundary kind per
+ * domain face (bc[2d+o], ORC_BC_*): one global step per iteration in the
+ * order of orc_run_uniform, with the boundary faces of the non-periodic
+ * directions solved by orc_boundary_conditions at the step's (t, dt).  A
+ * direction is periodic iff both of its faces are ORC_BC_PERIODIC.  Cells x
+ * fastest. */
+int orc_run_box(const orc_solver* s, double* luh, const int* ncell_d, double h, int steps, double* dt_io,
+                double* t_io, const int* bc, int threads) {
   const int B = s->b.order + 1, V = s->nvar, ND = V + s->npar, D = s->dim;
   const int B1 = (D == 3) ? B * B : B;
   const int sp = B1 * B;
-  const long ncell = (D == 3) ? (long)n * n * n : (long)n * n;
+  const long nx = ncell_d[0], ny = ncell_d[1], nz = (D == 3) ? ncell_d[2] : 1;
+  const long nd[3] = {nx, ny, nz};
+  const long ncell = nx * ny * nz;
   const double dx[3] = {h, h, h};
   for (int d = 0; d < D; d++)
     if ((bc[2 * d] == ORC_BC_PERIODIC) != (bc[2 * d + 1] == ORC_BC_PERIODIC)) return -2;
@@ -165,9 +168,10 @@ int orc_run_bc(const orc_solver* s, double* luh, int n, double h, int steps, dou
                        luh + c * sp * ND, dx, dt);
     for (int d = 0; d < D; d++) {
       const int periodic = bc[2 * d] == ORC_BC_PERIODIC;
+      const long n = nd[d];
 #pragma omp parallel for schedule(static)
       for (long c = 0; c < ncell; c++) {
-        long ix = c % n, iy = (c / n) % n, iz = (D == 3) ? c / ((long)n * n) : 0;
+        long ix = c % nx, iy = (c / nx) % ny, iz = (D == 3) ? c / (nx * ny) : 0;
         const long id = (d == 0) ? ix : (d == 1 ? iy : iz);
         double centre[3] = {(ix + 0.5) * h, (iy + 0.5) * h, (iz + 0.5) * h};
         if (!periodic && id == 0) /* the cell's left face is a domain face */
@@ -178,10 +182,10 @@ int orc_run_bc(const orc_solver* s, double* luh, int n, double h, int steps, dou
                                   lQhbnd + (c * 2 * D + 2 * d + 1) * B1 * ND, centre, dx, t, dt, d, 1);
           continue;
         }
-        if (d == 0) ix = (ix + 1) % n;
-        if (d == 1) iy = (iy + 1) % n;
-        if (d == 2) iz = (iz + 1) % n;
-        const long r = (D == 3) ? (iz * n + iy) * n + ix : iy * n + ix;
+        if (d == 0) ix = (ix + 1) % nx;
+        if (d == 1) iy = (iy + 1) % ny;
+        if (d == 2) iz = (iz + 1) % nz;
+        const long r = (D == 3) ? (iz * ny + iy) * nx + ix : iy * nx + ix;
         orc_riemann_nonlinear(s, lFhbnd + (c * 2 * D + 2 * d + 1) * B1 * V, lFhbnd + (r * 2 * D + 2 * d) * B1 * V,
                               lQhbnd + (c * 2 * D + 2 * d + 1) * B1 * ND, lQhbnd + (r * 2 * D + 2 * d) * B1 * ND, d);
       }
@@ -206,4 +210,11 @@ int orc_run_bc(const orc_solver* s, double* luh, int n, double h, int steps, dou
   free(lQhbnd);
   free(lFhbnd);
   return 0;
+}
+
+/* The n^D cube of orc_run_box. */
+int orc_run_bc(const orc_solver* s, double* luh, int n, double h, int steps, double* dt_io, double* t_io,
+               const int* bc, int threads) {
+  const int nd[3] = {n, n, n};
+  return orc_run_box(s, luh, nd, h, steps, dt_io, t_io, bc, threads);
 }

@@ -160,6 +160,8 @@ void orc_boundary_conditions(const orc_solver* s, int kind, double* FIn, const d
                              const double* dx, double t, double dt, int direction, int orientation);
 int orc_run_bc(const orc_solver* s, double* luh, int n, double h, int steps, double* dt_io, double* t_io,
                const int* bc, int threads);
+int orc_run_box(const orc_solver* s, double* luh, const int* ncell_d, double h, int steps, double* dt_io,
+                double* t_io, const int* bc, int threads);
 
 #ifdef __cplusplus
 }

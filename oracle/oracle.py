@@ -128,6 +128,8 @@ def _lib():
     lib.orc_boundary_conditions.argtypes = [sp, I, dp, dp, dp, dp, D, D, I, I]
     lib.orc_run_bc.argtypes = [sp, dp, I, D, I, dp, dp, ip, I]
     lib.orc_run_bc.restype = I
+    lib.orc_run_box.argtypes = [sp, dp, ip, D, I, dp, dp, ip, I]
+    lib.orc_run_box.restype = I
     return lib
 
 
@@ -316,6 +318,21 @@ class Oracle:
             raise ValueError("a direction must be periodic on both faces or on neither")
         if rc != 0:
             raise MemoryError("oracle run_bc failed to allocate")
+        return float(t_io[0]), float(dt_io[0])
+
+    def run_box(self, luh, ncells, h, steps, bc, dt=0.0, t=0.0, threads=0):
+        """run_bc on a box of ncells = (nx, ny[, nz]) cells (x fastest); returns (t, dt_next)."""
+        assert luh.dtype == np.float64 and luh.flags["C_CONTIGUOUS"]
+        nd = np.ones(3, dtype=np.int32)
+        nd[:len(ncells)] = ncells
+        dt_io = np.array([dt], dtype=np.float64)
+        t_io = np.array([t], dtype=np.float64)
+        rc = lib().orc_run_box(self.ps, _p(luh), _ip(nd), h, steps, _p(dt_io), _p(t_io),
+                               _ip(np.ascontiguousarray(bc, dtype=np.int32)), threads)
+        if rc == -2:
+            raise ValueError("a direction must be periodic on both faces or on neither")
+        if rc != 0:
+            raise MemoryError("oracle run_box failed to allocate")
         return float(t_io[0]), float(dt_io[0])
 
 

@@ -255,3 +255,19 @@ def test_dirichlet_entropy_wave_converges():
         ex = entropy_wave_state(o, n, order, D, t)
         errs.append(np.sqrt(((luh - ex) ** 2).sum() / (ex ** 2).sum()))
     assert errs[0] < 1e-3 and errs[1] < errs[0] / 3, errs
+
+
+@pytest.mark.parametrize("D,order,n", [(2, 3, 4), (3, 2, 3)])
+def test_box_with_repeated_data_is_the_cube(D, order, n):
+    """A periodic n x .. x 2n box holding two copies of a periodic state (period
+    n in the last direction) evolves as the n^D cube: pins the box indexing."""
+    o = O.Oracle(D, order)
+    a = symmetric_state(P.node_points(n, order, D), D)
+    B = order + 1
+    cube = a.reshape((n,) * D + (B ** D, 5))
+    box = np.ascontiguousarray(np.concatenate([cube, cube], axis=0).reshape(-1, B ** D, 5))
+    t1, dt1 = o.run_box(box, [n] * (D - 1) + [2 * n], 1.0 / n, 3, [O.BC_PERIODIC] * (2 * D))
+    t2, dt2 = o.run_bc(a, n, 1.0 / n, 3, [O.BC_PERIODIC] * (2 * D))
+    half = box.reshape((2 * n,) + (n,) * (D - 1) + (B ** D, 5))
+    assert np.array_equal(half[:n].reshape(a.shape), a) and np.array_equal(half[n:].reshape(a.shape), a)
+    assert t1 == t2 and dt1 == dt2
