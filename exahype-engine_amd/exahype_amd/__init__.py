@@ -273,7 +273,7 @@ class Context:
         out = np.array(out, dtype=np.float64)
         sub = None if subfaceIndex is None else np.ascontiguousarray(subfaceIndex, dtype=np.int32)
         _check(_lib.exa_faceIntegral(self.h, _p(out), _p(np.ascontiguousarray(lFhbnd_face, dtype=np.float64)),
-                                     direction, orientation, _ip(sub), levelDelta,
+                                     direction, orientation, _ipv(sub), levelDelta,
                                      _p(np.array(dx, dtype=np.float64)), dt, int(bool(addToUpdate))))
         return out
 
@@ -287,7 +287,7 @@ class Context:
         _check(_lib.exa_faceUnknownsProlongation(self.h, _p(qf), _p(ff),
                                                  _p(np.ascontiguousarray(lQhbndCoarse, dtype=np.float64)),
                                                  _p(np.ascontiguousarray(lFhbndCoarse, dtype=np.float64)),
-                                                 coarseGridLevel, fineGridLevel, _ip(sub)))
+                                                 coarseGridLevel, fineGridLevel, _ipv(sub)))
         return qf, ff
 
     def faceUnknownsRestriction(self, lQhbndCoarse, lFhbndCoarse, lQhbndFine, lFhbndFine, subfaceIndex,
@@ -298,21 +298,21 @@ class Context:
         _check(_lib.exa_faceUnknownsRestriction(self.h, _p(qc), _p(fc),
                                                 _p(np.ascontiguousarray(lQhbndFine, dtype=np.float64)),
                                                 _p(np.ascontiguousarray(lFhbndFine, dtype=np.float64)),
-                                                coarseGridLevel, fineGridLevel, _ip(sub)))
+                                                coarseGridLevel, fineGridLevel, _ipv(sub)))
         return qc, fc
 
     def volumeUnknownsProlongation(self, luhCoarse, subcellIndex, coarseGridLevel=0, fineGridLevel=1):
         out = np.zeros(self.nspace * self.nd)
         sub = np.ascontiguousarray(subcellIndex, dtype=np.int32)
         _check(_lib.exa_volumeUnknownsProlongation(self.h, _p(out), _p(np.ascontiguousarray(luhCoarse, dtype=np.float64)),
-                                                   coarseGridLevel, fineGridLevel, _ip(sub)))
+                                                   coarseGridLevel, fineGridLevel, _ipv(sub)))
         return out
 
     def volumeUnknownsRestriction(self, luhCoarse, luhFine, subcellIndex, coarseGridLevel=0, fineGridLevel=1):
         out = np.array(luhCoarse, dtype=np.float64)
         sub = np.ascontiguousarray(subcellIndex, dtype=np.int32)
         _check(_lib.exa_volumeUnknownsRestriction(self.h, _p(out), _p(np.ascontiguousarray(luhFine, dtype=np.float64)),
-                                                  coarseGridLevel, fineGridLevel, _ip(sub)))
+                                                  coarseGridLevel, fineGridLevel, _ipv(sub)))
         return out
 
     def surfaceIntegral(self, lduh, lFhbnd, dx):
@@ -386,6 +386,10 @@ class Mesh:
 
     def init_dt(self, dt=0.0):
         _check(_lib.exa_mesh_init_dt(self.h, dt))
+
+    def set_fused(self, enabled):
+        """Two-kernel fused step (default where compiled) or the four-kernel form."""
+        _check(_lib.exa_mesh_set_fused(self.h, int(bool(enabled))))
 
     def set_boundary(self, kinds):
         """Boundary kind per domain face [2*d + o] (BC_*)."""
