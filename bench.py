@@ -352,6 +352,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--sweep-variants", action="store_true",
                     help="only time the predictor kernel variants (node-column, line, line+MFMA, tile+MFMA) and exit")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (2^27 float64) three times: the reference "
+                         "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -361,6 +364,16 @@ def main():
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: launched with WORLD_SIZE > 1; one GPU only", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] copied 3 x 1 GiB")
+        return
 
     import exahype_amd as E
     from exahype_amd import problems as P

@@ -833,7 +833,8 @@ struct Launch {
   }
   static int riemann(const KernelCfg& k, hipStream_t st, long nf, const int* faces, const double* qb,
                      double* fb, const LaunchOpts& o) {
-    const int waves = (o.riemann_waves >= 1 && o.riemann_waves <= 16) ? o.riemann_waves : 4;
+    // riemann_kernel is compiled for at most 256 threads (4 waves) per workgroup
+    const int waves = (o.riemann_waves >= 1 && o.riemann_waves <= 4) ? o.riemann_waves : 4;
     const long blocks = (nf + waves - 1) / waves;
     hipLaunchKernelGGL((riemann_kernel<D, B, V, NP>), dim3(blocks), dim3(64 * waves), 0, st, nf,
                        reinterpret_cast<const int4*>(faces), qb, fb, o.ghost_q, o.ghost_f, k.tab, k.pde);

@@ -274,7 +274,7 @@ struct CellSet {
       }
     }
     if (!rc && nprolong) rc = status(exa::launch_face_prolong(k, stream, nprolong, d_prolong, d_qb, d_fb), "prolong");
-    for (int w : {1, 2, 4, 8}) {
+    for (int w : {1, 2, 4}) {  // the Riemann kernel's launch bound is 4 waves
       if (rc) break;
       exa::LaunchOpts o;
       o.riemann_waves = w;

@@ -48,7 +48,8 @@ struct KernelCfg {
 //                  cells of one instance must be contiguous
 //   grid           > 0: persistent launch with this many workgroups looping over the
 //                  cell tiles (predictor, corrector); 0: one workgroup per tile
-//   riemann_waves  faces (wavefronts) per Riemann workgroup, 1..16
+//   riemann_waves  faces (wavefronts) per Riemann workgroup, 1..4 (the kernel's
+//                  256-thread launch bound; other values fall back to 4)
 //   luh_update     predictor only: add the volume part of the update to this luh
 //                  (the predictor's own input) instead of writing lduh
 //   ghost_q/_f     Riemann kernels: outside face data of boundary faces
