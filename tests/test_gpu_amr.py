@@ -237,7 +237,9 @@ def test_full_batched_workload_conservation_and_oracle_step():
     def totals(q):
         return np.einsum("lnv,n,l->v", q.reshape(len(h), -1, 5), W, h ** 3)
 
-    np.testing.assert_allclose(totals(m.get_solution()), totals(u0), rtol=1e-12, atol=1e-14)
+    # exact in exact arithmetic; FP64 rounding of the coarse/fine flux scalings and of the totals
+    # themselves (~4.4e6 node values over 5 steps) leaves a few 1e-12 relative
+    np.testing.assert_allclose(totals(m.get_solution()), totals(u0), rtol=1e-11, atol=1e-14)
     for i in (0, 17, 40):
         nn = int(sizes[i])
         W2 = np.outer(w, w).ravel()
