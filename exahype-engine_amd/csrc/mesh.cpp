@@ -51,6 +51,7 @@ struct exa_mesh {
   unsigned bcmask = 0;
   bool hooks = false;  // some face takes its ghost data from a boundary-value hook
   double *d_gq = nullptr, *d_gf = nullptr;
+  double* d_halo = nullptr;  // staging of the host-side halo entry points
 };
 
 namespace {
@@ -144,7 +145,7 @@ void free_mesh(exa_mesh* m) {
   if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
   if (m->graph) (void)hipGraphDestroy(m->graph);
   for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
-  double* bufs[] = {m->d_luh, m->d_lduh, m->d_qb, m->d_fb, m->d_h, m->d_state, m->d_gq, m->d_gf};
+  double* bufs[] = {m->d_luh, m->d_lduh, m->d_qb, m->d_fb, m->d_h, m->d_state, m->d_gq, m->d_gf, m->d_halo};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (m->d_faces) (void)hipFree(m->d_faces);
@@ -540,6 +541,33 @@ int exa_mesh_halo_unpack(exa_mesh* m, int face, const double* d_msg_q, const dou
          "halo unpack");
   HIP_OK(hipMemcpyAsync(m->d_gf + size_t(face) * nf, d_msg_f, sizeof(double) * nf, hipMemcpyDeviceToDevice, m->stream),
          "halo unpack");
+  return EXA_OK;
+}
+
+int exa_mesh_halo_pack_host(exa_mesh* m, int face, double* msg_q, double* msg_f) {
+  if (!m || !msg_q || !msg_f) return EXA_ERR_ARG;
+  long nq = 0, nf = 0;
+  exa_mesh_halo_size(m, &nq, &nf);
+  if (!m->d_halo) HIP_OK(hipMalloc(&m->d_halo, sizeof(double) * (nq + nf)), "hipMalloc(halo staging)");
+  const int rc = exa_mesh_halo_pack(m, face, m->d_halo, m->d_halo + nq);
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(msg_q, m->d_halo, sizeof(double) * nq, hipMemcpyDeviceToHost, m->stream), "halo copy");
+  HIP_OK(hipMemcpyAsync(msg_f, m->d_halo + nq, sizeof(double) * nf, hipMemcpyDeviceToHost, m->stream), "halo copy");
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
+  return EXA_OK;
+}
+
+int exa_mesh_halo_unpack_host(exa_mesh* m, int face, const double* msg_q, const double* msg_f) {
+  if (!m || !msg_q || !msg_f) return EXA_ERR_ARG;
+  long nq = 0, nf = 0;
+  exa_mesh_halo_size(m, &nq, &nf);
+  if (!m->d_halo) HIP_OK(hipMalloc(&m->d_halo, sizeof(double) * (nq + nf)), "hipMalloc(halo staging)");
+  // the previous use of the staging buffer has completed: every halo call synchronises
+  HIP_OK(hipMemcpyAsync(m->d_halo, msg_q, sizeof(double) * nq, hipMemcpyHostToDevice, m->stream), "halo copy");
+  HIP_OK(hipMemcpyAsync(m->d_halo + nq, msg_f, sizeof(double) * nf, hipMemcpyHostToDevice, m->stream), "halo copy");
+  const int rc = exa_mesh_halo_unpack(m, face, m->d_halo, m->d_halo + nq);
+  if (rc) return rc;
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
   return EXA_OK;
 }
 

@@ -92,6 +92,8 @@ _sig("exa_mesh_halo_size", [_cp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(c
 _sig("exa_mesh_halo_pack", [_cp, ctypes.c_int, _vp, _vp])
 _sig("exa_mesh_halo_unpack", [_cp, ctypes.c_int, _vp, _vp])
 _sig("exa_mesh_device_state", [_cp], ctypes.c_void_p)
+_sig("exa_mesh_halo_pack_host", [_cp, ctypes.c_int, _dp, _dp])
+_sig("exa_mesh_halo_unpack_host", [_cp, ctypes.c_int, _dp, _dp])
 _sig("exa_mesh_dt_next", [_cp, _dp])
 _sig("exa_mesh_set_dt_next", [_cp, ctypes.c_double])
 _sig("exa_mesh_init_dt", [_cp, ctypes.c_double])
@@ -421,6 +423,17 @@ class Mesh:
 
     def halo_unpack(self, face, d_msg_q, d_msg_f):
         _check(_lib.exa_mesh_halo_unpack(self.h, face, ctypes.c_void_p(d_msg_q), ctypes.c_void_p(d_msg_f)))
+
+    def halo_pack_host(self, face):
+        """The two messages of a domain face as host arrays (lQhbnd faces, lFhbnd faces)."""
+        nq, nf = self.halo_size()
+        q, f = np.zeros(nq), np.zeros(nf)
+        _check(_lib.exa_mesh_halo_pack_host(self.h, face, _p(q), _p(f)))
+        return q, f
+
+    def halo_unpack_host(self, face, msg_q, msg_f):
+        _check(_lib.exa_mesh_halo_unpack_host(self.h, face, _p(np.ascontiguousarray(msg_q, dtype=np.float64)),
+                                              _p(np.ascontiguousarray(msg_f, dtype=np.float64))))
 
     def dt_next(self):
         v = ctypes.c_double(0)

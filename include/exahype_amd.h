@@ -287,6 +287,9 @@ int exa_mesh_step_advance(exa_mesh* mesh);
 int exa_mesh_halo_size(const exa_mesh* mesh, long* n_q, long* n_f);
 int exa_mesh_halo_pack(exa_mesh* mesh, int face, double* d_msg_q, double* d_msg_f);
 int exa_mesh_halo_unpack(exa_mesh* mesh, int face, const double* d_msg_q, const double* d_msg_f);
+/* Host-staged forms (host messages, as the reference's MPI messages; synchronous). */
+int exa_mesh_halo_pack_host(exa_mesh* mesh, int face, double* msg_q, double* msg_f);
+int exa_mesh_halo_unpack_host(exa_mesh* mesh, int face, const double* msg_q, const double* msg_f);
 double* exa_mesh_device_state(exa_mesh* mesh);
 /* Host access to dt_next between correct and advance (synchronous). */
 int exa_mesh_dt_next(exa_mesh* mesh, double* dt_next);
