@@ -352,6 +352,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--sweep-variants", action="store_true",
                     help="only time the predictor kernel variants (node-column, line, line+MFMA, tile+MFMA) and exit")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="with --sweep-variants: time only this predictor variant (1..4)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (2^27 float64) three times: the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -401,7 +403,7 @@ def main():
     if args.sweep_variants:
         for prec, name, pic in ((E.PREC_FP64, "fp64", 0), (E.PREC_FP32, "fp32", 3)):
             for variant, vname in ((1, "node-column"), (2, "line"), (3, "line+mfma"), (4, "tile+mfma")):
-                if prec == E.PREC_FP64 and variant >= 3:
+                if (prec == E.PREC_FP64 and variant >= 3) or (args.variant is not None and variant != args.variant):
                     continue
                 r = run_form(E, P, args.n, args.order, args.steps, args.warmup, prec, pic, args.seed, args.modes, variant)
                 log(f"[sweep] {name} {vname:12s} predictor {r['predictor_ms']:.2f} ms/step "

@@ -511,6 +511,7 @@ struct TShape {
   static constexpr int THREADS = 256;
   static constexpr int NWAVE = THREADS / 64;
   static constexpr int TPW = (TPAIRS + NWAVE - 1) / NWAVE;  // time-solve pairs per wave
+  static constexpr int PPW = (PAIRS + NWAVE - 1) / NWAVE;    // line pairs per wave and direction
 };
 
 // tile position of point n on line L of direction d (lines: t slowest, then the
@@ -525,7 +526,7 @@ __device__ __forceinline__ int line_point(int d, int L, int n) {
 }
 
 template <int B>
-__global__ __launch_bounds__(256) void predictor3d_tile_kernel(
+__global__ __launch_bounds__(256, 3) void predictor3d_tile_kernel(
     long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ lduh, double* __restrict__ lQhbnd,
     double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, int maxIter,
@@ -582,70 +583,103 @@ __global__ __launch_bounds__(256) void predictor3d_tile_kernel(
   int iters = maxIter;
   for (int iter = 0; iter < maxIter; iter++) {
     // ---- G = sum_d D_d F_d(lQi): tA -> tB ----
+    // The initial guess is constant in time, so in the first iteration G is
+    // evaluated on the t = 0 lines only and copied to the other time slices
+    // (the same operations on the same values: bitwise what the full
+    // evaluation gives).  A wave's line pairs are processed together so that
+    // its 5 * PPW accumulation chains interleave.
+    const int nlines = iter == 0 ? B * B : NS;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      for (int p = wave; p < S::PAIRS; p += S::NWAVE) {
-        float F[NK][V];
+      float F[S::PPW][NK][V];
+#pragma unroll
+      for (int q = 0; q < S::PPW; q++) {
+        const int p = wave + S::NWAVE * q;
 #pragma unroll
         for (int s = 0; s < NK; s++) {
           const int L = 32 * p + 16 * kset[s] + j;
-          if (kok[s] && L < NS) {
+          if (kok[s] && L < nlines) {
             const float* Q = tA + line_point<B>(d, L, kn[s]) * V;
             float Qv[V];
 #pragma unroll
             for (int v = 0; v < V; v++) Qv[v] = Q[v];
             float ir, pr;
             euler_prim<float>(Qv, ir, pr);
-            euler_flux_dir<float>(Qv, ir, pr, d, F[s]);
+            euler_flux_dir<float>(Qv, ir, pr, d, F[q][s]);
           } else {
 #pragma unroll
-            for (int v = 0; v < V; v++) F[s][v] = 0.f;
+            for (int v = 0; v < V; v++) F[q][s][v] = 0.f;
           }
         }
+      }
+      exa_f4 acc[S::PPW][V];
 #pragma unroll
-        for (int v = 0; v < V; v++) {
-          exa_f4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int q = 0; q < S::PPW; q++)
 #pragma unroll
-          for (int s = 0; s < NK; s++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aD[s], F[s][v], acc, 0, 0, 0);
+        for (int v = 0; v < V; v++) acc[q][v] = exa_f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int ip = 4 * g + r, os = ip >> 3, oi = ip & 7;
-            const int L = 32 * p + 16 * os + j;
-            if (oi < B && L < NS) {
-              const int idx = line_point<B>(d, L, oi) * V + v;
-              tB[idx] = (d == 0) ? acc[r] : tB[idx] + acc[r];
-            }
+      for (int s = 0; s < NK; s++)
+#pragma unroll
+        for (int q = 0; q < S::PPW; q++)
+          if (32 * (wave + S::NWAVE * q) < nlines)  // wave-uniform
+#pragma unroll
+            for (int v = 0; v < V; v++)
+              acc[q][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(aD[s], F[q][s][v], acc[q][v], 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < S::PPW; q++) {
+        const int p = wave + S::NWAVE * q;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int ip = 4 * g + r, os = ip >> 3, oi = ip & 7;
+          const int L = 32 * p + 16 * os + j;
+          if (oi < B && L < nlines) {
+            const int base = line_point<B>(d, L, oi) * V;
+#pragma unroll
+            for (int v = 0; v < V; v++) tB[base + v] = (d == 0) ? acc[q][v][r] : tB[base + v] + acc[q][v][r];
           }
         }
       }
       __syncthreads();
     }
+    if (iter == 0) {  // G(t) = G(t = 0)
+      for (int i = COLS + l; i < TILE; i += S::THREADS) tB[i] = tB[i % COLS];
+      __syncthreads();
+    }
     // ---- time solve lQi' = luh - dt/h P G, in place in tB; residual vs tA ----
+    // all loads of the wave's column pairs precede its stores (in place)
+    float X[S::TPW][NK];
+#pragma unroll
+    for (int k = 0; k < S::TPW; k++) {
+      const int p = wave + S::NWAVE * k;
+#pragma unroll
+      for (int s = 0; s < NK; s++) {
+        const int c = 32 * p + 16 * kset[s] + j;
+        X[k][s] = (kok[s] && c < COLS) ? tB[kn[s] * COLS + c] : 0.f;
+      }
+    }
+    exa_f4 tacc[S::TPW];
+#pragma unroll
+    for (int k = 0; k < S::TPW; k++) tacc[k] = exa_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < NK; s++)
+#pragma unroll
+      for (int k = 0; k < S::TPW; k++)
+        if (wave + S::NWAVE * k < S::TPAIRS)  // wave-uniform
+          tacc[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[s], X[k][s], tacc[k], 0, 0, 0);
     double sq = 0.0;
 #pragma unroll
     for (int k = 0; k < S::TPW; k++) {
       const int p = wave + S::NWAVE * k;
-      if (p < S::TPAIRS) {  // wave-uniform
-        float X[NK];
 #pragma unroll
-        for (int s = 0; s < NK; s++) {
-          const int c = 32 * p + 16 * kset[s] + j;
-          X[s] = (kok[s] && c < COLS) ? tB[kn[s] * COLS + c] : 0.f;
-        }
-        exa_f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < NK; s++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[s], X[s], acc, 0, 0, 0);
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int ip = 4 * g + r, os = ip >> 3, oi = ip & 7;
-          const int c = 32 * p + 16 * os + j;
-          if (oi < B && c < COLS) {
-            const int idx = oi * COLS + c;
-            const float nv = u[k] - cdt * acc[r];
-            const double dq = static_cast<double>(nv) - static_cast<double>(tA[idx]);
-            sq += dq * dq;
-            tB[idx] = nv;
-          }
+      for (int r = 0; r < 4; r++) {
+        const int ip = 4 * g + r, os = ip >> 3, oi = ip & 7;
+        const int c = 32 * p + 16 * os + j;
+        if (oi < B && c < COLS) {
+          const int idx = oi * COLS + c;
+          const float nv = u[k] - cdt * tacc[k][r];
+          const double dq = static_cast<double>(nv) - static_cast<double>(tA[idx]);
+          sq += dq * dq;
+          tB[idx] = nv;
         }
       }
     }
