@@ -401,7 +401,8 @@ def main():
 
     log(f"[bench] 3D Euler N={args.order} {args.n}^3 cells, steps={args.steps} warmup={args.warmup}")
     if args.sweep_variants:
-        for prec, name, pic in ((E.PREC_FP64, "fp64", 0), (E.PREC_FP32, "fp32", 3)):
+        k32 = args.picard_fp32 if args.picard_fp32 > 0 else 3
+        for prec, name, pic in ((E.PREC_FP64, "fp64", 0), (E.PREC_FP32, "fp32", k32)):
             for variant, vname in ((1, "node-column"), (2, "line"), (3, "line+mfma"), (4, "tile+mfma")):
                 if (prec == E.PREC_FP64 and variant >= 3) or (args.variant is not None and variant != args.variant):
                     continue
