@@ -213,6 +213,7 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
     h.F1[i] = c->tab.F1[i];
     h.wf[i] = static_cast<float>(c->tab.w[i]);
     h.F0f[i] = static_cast<float>(c->tab.F0[i]);
+    h.F1f[i] = static_cast<float>(c->tab.F1[i]);
   }
   for (int i = 0; i < B * B; i++) {
     h.Kxi[i] = c->tab.Kxi[i];

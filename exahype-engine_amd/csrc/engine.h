@@ -23,6 +23,7 @@ struct DevTables {
   // FP32 copies for the reduced-precision predictor
   float wf[kMaxBasis];
   float F0f[kMaxBasis];
+  float F1f[kMaxBasis];
   float Kxif[kMaxBasis * kMaxBasis];
   float iK1f[kMaxBasis * kMaxBasis];
   // derived operators of the line-decomposed predictor

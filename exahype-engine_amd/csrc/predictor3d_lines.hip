@@ -106,10 +106,21 @@ __device__ __forceinline__ const double* tab_P<double>(const DevTables* T) { ret
 template <>
 __device__ __forceinline__ const float* tab_P<float>(const DevTables* T) { return T->Ptf; }
 
+// 1/rho: IEEE division in FP64; the hardware reciprocal (1 ulp) in the FP32
+// form, whose accuracy target is 1e-6 relative.
+template <typename R>
+__device__ __forceinline__ R recip(R x) {
+  return R(1) / x;
+}
+template <>
+__device__ __forceinline__ float recip<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+
 // Euler pressure and 1/rho at one point (EulerSolver_ADERDG.cpp:78-86).
 template <typename R>
 __device__ __forceinline__ void euler_prim(const R* Q, R& irho, R& p) {
-  irho = R(1) / Q[0];
+  irho = recip<R>(Q[0]);
   const R j2 = Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3];
   p = (R(1.4) - R(1)) * (Q[4] - R(0.5) * irho * j2);
 }
@@ -242,6 +253,122 @@ __device__ __forceinline__ void finish_cell(R (&q)[B][LShape<B>::V], R* tl, unsi
   }
 }
 
+// Post-Picard phase of the FP32 kernels, one lane per spatial node s:
+// P = previous iterate, Q = final iterate, both FP32 tiles [t][s][v].
+//   lQhi[s] = sum_t w_t Q[t][s],  lFhi_d[s] = sum_t w_t F_d(P[t][s])
+// (the reference averages the flux of the last iteration, :441-515), then the
+// volume integral (volumeIntegralNonlinear.cpph:133-223) and the extrapolation
+// (:517-687) from the averages, in FP32; the update (or lduh) and the face
+// arrays are written in FP64.  The flux of P is evaluated once per space-time
+// point for all three directions.  avg: FP32 LDS scratch of 4 B^3 V values,
+// which may overlap P (it is written after a barrier).
+template <int B>
+__device__ __forceinline__ void finish_cell_points(const float* P, const float* Q, float* avg, int l, long cell,
+                                                   double dt, double h, int iters, const DevTables* __restrict__ T,
+                                                   double* __restrict__ lduh, double* __restrict__ lQhbnd,
+                                                   double* __restrict__ lFhbnd, int* __restrict__ picard,
+                                                   double* luh_upd, int nthreads) {
+  constexpr int V = 5, NS = B * B * B, NF = B * B, COLS = NS * V;
+  float qh[V], fh[3][V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    qh[v] = 0.f;
+    fh[0][v] = fh[1][v] = fh[2][v] = 0.f;
+  }
+  if (l < NS) {
+#pragma unroll
+    for (int t = 0; t < B; t++) {
+      const float wt = T->wf[t];
+      float q[V], p[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        q[v] = Q[t * COLS + l * V + v];
+        p[v] = P[t * COLS + l * V + v];
+      }
+#pragma unroll
+      for (int v = 0; v < V; v++) qh[v] += q[v] * wt;
+      float ir, pr;
+      euler_prim<float>(p, ir, pr);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        float F[V];
+        euler_flux_dir<float>(p, ir, pr, d, F);
+#pragma unroll
+        for (int v = 0; v < V; v++) fh[d][v] += F[v] * wt;
+      }
+    }
+  }
+  __syncthreads();  // P may be overwritten by avg below
+  float* aq = avg;             // [NS][V]
+  float* af = avg + NS * V;    // [3][NS][V]
+  if (l < NS) {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      aq[l * V + v] = qh[v];
+#pragma unroll
+      for (int d = 0; d < 3; d++) af[(d * NS + l) * V + v] = fh[d][v];
+    }
+  }
+  __syncthreads();
+  if (l < NS) {
+    // volume integral at node s = l = (z, y, x)
+    const int x = l % B, y = (l / B) % B, z = l / (B * B);
+    const float invDx = static_cast<float>(1.0 / h);
+    float du[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) du[v] = 0.f;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const float tw = (d == 0) ? T->wf[z] * T->wf[y] : (d == 1 ? T->wf[z] * T->wf[x] : T->wf[y] * T->wf[x]);
+      const int c = (d == 0) ? x : (d == 1 ? y : z);
+      const int stride = (d == 0) ? 1 : (d == 1 ? B : B * B);
+      const float* line = af + (d * NS + l - c * stride) * V;
+      float part[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) part[v] = 0.f;
+#pragma unroll
+      for (int m = 0; m < B; m++) {
+        const float k = T->Kxif[c * B + m];
+#pragma unroll
+        for (int v = 0; v < V; v++) part[v] += k * line[m * stride * V + v];
+      }
+#pragma unroll
+      for (int v = 0; v < V; v++) du[v] += part[v] * (tw * invDx);
+    }
+    if (luh_upd) {
+      const double upd = dt / (T->w[z] * T->w[y] * T->w[x]);
+#pragma unroll
+      for (int v = 0; v < V; v++) luh_upd[(cell * NS + l) * V + v] += static_cast<double>(du[v]) * upd;
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; v++) lduh[(cell * NS + l) * V + v] = static_cast<double>(du[v]);
+    }
+  }
+  // extrapolation: one (direction, face node, variable) triple gives both faces
+  constexpr int NTRI = 3 * NF * V;
+  for (int o = l; o < NTRI; o += nthreads) {
+    const int v = o % V, fn = (o / V) % NF, d = o / (V * NF);
+    const int a = fn / B, b = fn % B;
+    float q0 = 0.f, q1 = 0.f, f0 = 0.f, f1 = 0.f;
+#pragma unroll
+    for (int m = 0; m < B; m++) {
+      const int node = (d == 0) ? (a * B + b) * B + m : (d == 1 ? (a * B + m) * B + b : (m * B + a) * B + b);
+      const float qv = aq[node * V + v], fv = af[(d * NS + node) * V + v];
+      const float c0 = T->F0f[m], c1 = T->F1f[m];
+      q0 += qv * c0;
+      q1 += qv * c1;
+      f0 += fv * c0;
+      f1 += fv * c1;
+    }
+    const long base = cell * 6 * NF * V + fn * V + v;
+    lQhbnd[base + (2 * d) * NF * V] = q0;
+    lQhbnd[base + (2 * d + 1) * NF * V] = q1;
+    lFhbnd[base + (2 * d) * NF * V] = f0;
+    lFhbnd[base + (2 * d + 1) * NF * V] = f1;
+  }
+  if (l == 0 && picard) picard[cell] = iters;
+}
+
 template <int B, typename R, bool MF>
 __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel(
     long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
@@ -254,8 +381,10 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
   static_assert(!MF || (sizeof(R) == 4 && B <= 8), "matrix-core form: FP32, B <= 8");
   constexpr int NWAVE = S::THREADS / 64;
   constexpr int TILE_BYTES = TILE * int(sizeof(R));
-  constexpr int AVG_BYTES = 4 * NS * V * 8;  // lQhi + lFhi_{x,y,z} in FP64
-  constexpr int BUF_BYTES = TILE_BYTES > AVG_BYTES ? TILE_BYTES : AVG_BYTES;
+  constexpr int AVG_BYTES = 4 * NS * V * 8;  // FP64 form: lQhi + lFhi_{x,y,z} in FP64
+  // FP32 forms: a second tile for the previous iterate (point-parallel finish)
+  constexpr int NEED = sizeof(R) == 4 ? 2 * TILE_BYTES : TILE_BYTES;
+  constexpr int BUF_BYTES = NEED > AVG_BYTES ? NEED : AVG_BYTES;
   __shared__ __align__(16) unsigned char buf[BUF_BYTES];
   __shared__ double red[NWAVE];
   R* tl = reinterpret_cast<R*>(buf);
@@ -440,7 +569,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
     }
     __syncthreads();
     double sq = 0.0;
-    if (lane_on) {
+    if (adaptive && lane_on) {  // the residual decides only the adaptive Picard rule
 #pragma unroll
       for (int x = 0; x < B; x++)
 #pragma unroll
@@ -474,7 +603,20 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
   }
 
   // ---- after the Picard loop: q = previous iterate, the tile holds the final one ----
-  finish_cell<B, R>(q, tl, buf, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd);
+  if constexpr (sizeof(R) == 4) {
+    float* t2 = reinterpret_cast<float*>(buf) + TILE;
+    if (lane_on) {
+#pragma unroll
+      for (int x = 0; x < B; x++)
+#pragma unroll
+        for (int v = 0; v < V; v++) t2[own + x * V + v] = q[x][v];
+    }
+    __syncthreads();
+    finish_cell_points<B>(t2, tl, t2, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd,
+                          S::THREADS);
+  } else {
+    finish_cell<B, R>(q, tl, buf, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd);
+  }
 }
 
 
@@ -677,8 +819,10 @@ __global__ __launch_bounds__(256, 3) void predictor3d_tile_kernel(
         if (oi < B && c < COLS) {
           const int idx = oi * COLS + c;
           const float nv = u[k] - cdt * tacc[k][r];
-          const double dq = static_cast<double>(nv) - static_cast<double>(tA[idx]);
-          sq += dq * dq;
+          if (adaptive) {  // the residual decides only the adaptive Picard rule
+            const double dq = static_cast<double>(nv) - static_cast<double>(tA[idx]);
+            sq += dq * dq;
+          }
           tB[idx] = nv;
         }
       }
@@ -705,19 +849,8 @@ __global__ __launch_bounds__(256, 3) void predictor3d_tile_kernel(
     tB = tmp;
   }
 
-  // ---- tB = final iterate, tA = previous iterate: the line form's post-Picard phase ----
-  constexpr int LV = LShape<B>::V;
-  float q[B][LV];
-  {
-    const bool lane_on = l < NS;
-    const int t = l / (B * B), z = (l / B) % B, y = l % B;
-    const int own = ((t * B + z) * B + y) * B * V;
-#pragma unroll
-    for (int x = 0; x < B; x++)
-#pragma unroll
-      for (int v = 0; v < V; v++) q[x][v] = lane_on ? tA[own + x * V + v] : 0.f;
-  }
-  finish_cell<B, float>(q, tB, buf, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd);
+  // ---- tB = final iterate, tA = previous iterate ----
+  finish_cell_points<B>(tA, tB, tA, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd, S::THREADS);
 }
 
 }  // namespace
