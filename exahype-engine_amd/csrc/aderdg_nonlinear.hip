@@ -604,7 +604,9 @@ struct FusedShape {
   static constexpr int NTASK = 2 * D * S::NF;  // face nodes per cell
   static constexpr int RED = NTASK * 2 * S::ND;  // partial products per cell
   static constexpr int AVG = 2 * D * 2 * S::ND;  // averages per cell
-  static constexpr int LDS_BYTES = S::CPB * (RED + AVG) * 8;
+  static constexpr int NQ = S::NF >= 16 ? 4 : 1;  // partial sums of a face average (large faces)
+  static constexpr int PART = NQ > 1 ? AVG * NQ : 0;  // partial sums per cell
+  static constexpr int LDS_BYTES = S::CPB * (RED + AVG + PART) * 8;
   static constexpr bool ok = LDS_BYTES <= 60 * 1024;
 };
 
@@ -647,14 +649,15 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, THREADS = S::THREADS;
   constexpr int NTASK = FS::NTASK;
   __shared__ double sred[THREADS / 64];
-  __shared__ double sbuf[CPB * (FS::RED + FS::AVG)];
+  __shared__ double sbuf[CPB * (FS::RED + FS::AVG + FS::PART)];
   const int tid = threadIdx.x;
   const int cib = tid / NS;
   const int s = tid - cib * NS;
   const long cell = long(blockIdx.x) * CPB + cib;
   const bool valid = (cib < CPB) && (cell < ncells);
-  double* red = sbuf + (valid ? cib : 0) * (FS::RED + FS::AVG);  // [task][side][ND]; later fluxes [task][V]
-  double* avg = red + FS::RED;                                   // [face][side][ND]
+  double* red = sbuf + (valid ? cib : 0) * (FS::RED + FS::AVG + FS::PART);  // [task][side][ND]; later fluxes [task][V]
+  double* avg = red + FS::RED;                                              // [face][side][ND]
+  double* part = avg + FS::AVG;                                             // [face][side][ND][quarter]
   const double* w = T->w;
   // phase 1
   if (valid) {
@@ -676,16 +679,42 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
     }
   }
   __syncthreads();
-  // phase 2
+  // phase 2: face averages in a fixed order that depends only on the face
+  // (quarter sums over consecutive face nodes, then ((p0 + p1) + p2) + p3)
+  constexpr int CH = (NF + FS::NQ - 1) / FS::NQ;
+  if constexpr (FS::NQ == 1) {
+    if (valid) {
+      for (int q = s; q < FS::AVG; q += NS) {
+        const int f = q / (2 * ND), side = (q / ND) & 1, k = q % ND;
+        double acc = 0.0;
+        for (int fn = 0; fn < NF; fn++) acc += red[((f * NF + fn) * 2 + side) * ND + k];
+        avg[q] = acc;
+      }
+    }
+    __syncthreads();
+  } else {
   if (valid) {
-    for (int q = s; q < 2 * D * 2 * ND; q += NS) {
-      const int f = q / (2 * ND), side = (q / ND) & 1, k = q % ND;
+    for (int q = s; q < FS::PART; q += NS) {
+      const int item = q / FS::NQ, quarter = q - item * FS::NQ;
+      const int f = item / (2 * ND), side = (item / ND) & 1, k = item % ND;
+      const int lo = quarter * CH, hi = lo + CH < NF ? lo + CH : NF;
       double acc = 0.0;
-      for (int fn = 0; fn < NF; fn++) acc += red[((f * NF + fn) * 2 + side) * ND + k];
-      avg[(f * 2 + side) * ND + k] = acc;
+      for (int fn = lo; fn < hi; fn++) acc += red[((f * NF + fn) * 2 + side) * ND + k];
+      part[q] = acc;
     }
   }
   __syncthreads();
+  if (valid) {
+    for (int q = s; q < FS::AVG; q += NS) {
+      const double* pq = part + q * FS::NQ;
+      double acc = pq[0];
+#pragma unroll
+      for (int i = 1; i < FS::NQ; i++) acc += pq[i];
+      avg[q] = acc;  // q = (f * 2 + side) * ND + k
+    }
+  }
+  __syncthreads();
+  }
   // phase 3: fluxes into red[task][V] (red's partial products were consumed
   // in phase 2; phase 3 reads only avg and global memory)
   if (valid) {
