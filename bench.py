@@ -47,9 +47,9 @@ for p in (os.path.join(ROOT, "exahype-engine_amd"), os.path.join(ROOT, "oracle")
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per predictor launch at 81^3, order 5, measured with rocprofv3 FETCH_SIZE (x2, the
+# HBM bytes per FP32 predictor launch at 81^3, order 5, measured with rocprofv3 FETCH_SIZE (x2, the
 # factor a 1 GiB copy calibrates) + WRITE_SIZE (x1): profiles/euler3d_n5_81cubed_kernels.md.
-MEASURED_PREDICTOR_TRAFFIC = {(81, 5): 18.45e9}
+MEASURED_PREDICTOR_TRAFFIC = {(81, 5): 18.49e9}
 
 
 def log(*a):
@@ -462,6 +462,9 @@ def main():
         "fp64": {k: r64[k] for k in ("value", "ms_per_step", "predictor_ms", "riemann_ms", "corrector_ms",
                                      "picard_hist", "roofline")},
         "fp32_phases_ms": {k: r32[k] for k in ("predictor_ms", "riemann_ms", "corrector_ms")},
+        "step_form": "two kernels per step: predictor (+ volume part of the update), then one corrector "
+                     "kernel (Riemann problems of every cell's faces + face integrals + update + CFL); "
+                     "riemann_ms is the boundary-condition slot between them (periodic: empty)",
         "speedup_fp32_vs_fp64": r32["value"] / r64["value"],
         "rel_l2_err_fp32_vs_fp64": err,
         "max_rel_l2_err": max(err),
