@@ -296,21 +296,22 @@ def bench_batched(args, E, P):
             threads = os.cpu_count() or 1
         threads = max(1, min(16, threads))
         o2 = O.Oracle(2, order)
-        sample = [int(s) for s in sizes[:16]]
+        sample = [int(s) for s in sizes[:32]]
+        nst = 4
         t1 = time.perf_counter()
         for s_ in sample:
             luh = np.ascontiguousarray(P.density_wave_2d(s_, order).ravel())
-            o2.run_uniform(luh, s_, 1.0 / s_, 2, 0.0, threads=threads)
+            o2.run_uniform(luh, s_, 1.0 / s_, nst, 0.0, threads=threads)
         o3 = O.AmrOracle(3, order)
         r9 = P.amr_ball_refinement(9, radius=0.25)
         lv, co = O.AmrOracle.amr_leaves(9, r9)
         u = P.amr_density_wave(order, 9, lv, co)
-        o3.amr_run(u, 9, r9, 2, threads=threads)
+        o3.amr_run(u, 9, r9, nst, threads=threads)
         t = time.perf_counter() - t1
-        dofs = 2 * (sum(s_ * s_ for s_ in sample) * B ** 2 + len(lv) * B ** 3) * 5
+        dofs = nst * (sum(s_ * s_ for s_ in sample) * B ** 2 + len(lv) * B ** 3) * 5
         cpu = dict(value=dofs / t, unit="DoF-updates/s", cores=threads, kind="port",
-                   sample=f"oracle on the first 16 2D instances and a 9^3 two-level mesh with the same ball, "
-                          f"2 steps each ({t:.2f} s wall, {threads} threads)")
+                   sample=f"oracle on the first 32 2D instances and a 9^3 two-level mesh with the same ball, "
+                          f"{nst} steps each ({t:.2f} s wall, {threads} threads)")
         log(f"[bench] cpu baseline: {cpu['value']:.4e} DoF/s on {threads} threads")
     out = {
         "metric": "DoF-updates/s batched uneven work: 3D Euler order-4 two-level mesh + 64 2D Euler instances",
@@ -397,7 +398,7 @@ def main():
     args.n = args.n or 81
     args.order = args.order or 5
     args.cpu_n = args.cpu_n or 27
-    args.cpu_steps = args.cpu_steps or 2
+    args.cpu_steps = args.cpu_steps or 4
 
     log(f"[bench] 3D Euler N={args.order} {args.n}^3 cells, steps={args.steps} warmup={args.warmup}")
     if args.sweep_variants:
