@@ -473,10 +473,27 @@ __device__ __forceinline__ void atomic_min_pos_double(double* addr, double v) {
   atomicMin(reinterpret_cast<unsigned long long*>(addr), static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
+// CFL*PNPM / sum_d max_v |lambda_v(Q, d)| / dx (stableTimeStepSize.cpph:14-62).
+// For Euler the direction-independent 1/rho, p and c are computed once; the
+// per-direction maximum keeps pde_max_abs_eig's expressions (same bits).
 template <int D, int B, int V, int NP>
 __device__ __forceinline__ double node_cfl_dt(const double* __restrict__ Q, const PdeParams& P, double invDx,
                                               double cflpnpm) {
   double den = 0.0;
+  if (P.id == kPdeEuler) {
+    constexpr double gamma = 1.4;
+    const double irho = 1. / Q[0];
+    const double j2 = (D == 3) ? Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3] : Q[1] * Q[1] + Q[2] * Q[2];
+    const double p = (gamma - 1) * (Q[4] - 0.5 * irho * j2);
+    const double c = sqrt(gamma * p * irho);
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const double u_n = Q[d + 1] * irho;
+      const double l0 = fabs(u_n - c), l1 = fabs(u_n), l4 = fabs(u_n + c);
+      den += fmax(fmax(l0, l1), l4) * invDx;
+    }
+    return cflpnpm / den;
+  }
 #pragma unroll
   for (int d = 0; d < D; d++) den += pde_max_abs_eig<D>(P, Q, d) * invDx;
   return cflpnpm / den;
@@ -606,7 +623,8 @@ struct FusedShape {
   static constexpr int AVG = 2 * D * 2 * S::ND;  // averages per cell
   static constexpr int NQ = S::NF >= 16 ? 4 : 1;  // partial sums of a face average (large faces)
   static constexpr int PART = NQ > 1 ? AVG * NQ : 0;  // partial sums per cell
-  static constexpr int LDS_BYTES = S::CPB * (RED + AVG + PART) * 8;
+  static constexpr int SMAX = 2 * D;                   // Rusanov speed per face
+  static constexpr int LDS_BYTES = S::CPB * (RED + AVG + PART + SMAX) * 8;
   static constexpr bool ok = LDS_BYTES <= 60 * 1024;
 };
 
@@ -649,18 +667,56 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, THREADS = S::THREADS;
   constexpr int NTASK = FS::NTASK;
   __shared__ double sred[THREADS / 64];
-  __shared__ double sbuf[CPB * (FS::RED + FS::AVG + FS::PART)];
+  __shared__ double sbuf[CPB * (FS::RED + FS::AVG + FS::PART + FS::SMAX)];
   const int tid = threadIdx.x;
   const int cib = tid / NS;
   const int s = tid - cib * NS;
   const long cell = long(blockIdx.x) * CPB + cib;
   const bool valid = (cib < CPB) && (cell < ncells);
-  double* red = sbuf + (valid ? cib : 0) * (FS::RED + FS::AVG + FS::PART);  // [task][side][ND]; later fluxes [task][V]
-  double* avg = red + FS::RED;                                              // [face][side][ND]
-  double* part = avg + FS::AVG;                                             // [face][side][ND][quarter]
+  double* red = sbuf + (valid ? cib : 0) * (FS::RED + FS::AVG + FS::PART + FS::SMAX);  // [task][side][ND]; later [task][V]
+  double* avg = red + FS::RED;                                                         // [face][side][ND]
+  double* part = avg + FS::AVG;                                                        // [face][side][ND][quarter]
+  double* sm = part + FS::PART;                                                        // [face]
   const double* w = T->w;
-  // phase 1
+  // One face-node task per thread (3D order 5, 2D order >= 3): the thread
+  // loads both sides' states and fluxes once and keeps them for phase 3.
+  constexpr bool ONE = NTASK <= NS;
+  double rQL[ONE ? ND : 1], rQR[ONE ? ND : 1], rFL[ONE ? V : 1], rFR[ONE ? V : 1];
+  // the node's solution, loaded up front so that its latency overlaps the face phases
+  double Q[ND];
   if (valid) {
+#pragma unroll
+    for (int v = 0; v < ND; v++) Q[v] = luh[(cell * NS + s) * ND + v];
+  }
+  // phase 1
+  if constexpr (ONE) {
+    if (valid && s < NTASK) {
+      const int task = s;
+      const int f = task / NF, fn = task - f * NF, o = f & 1;
+      const long g = ghost_face<D>(cell, n, f, bcmask);
+      const long nb = g >= 0 ? 0 : periodic_neighbour<D>(cell, n, f >> 1, o ? 1 : -1);
+      const double* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
+      const double* qn = g >= 0 ? gq + (g * NF + fn) * ND : lQhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * ND;
+      const double* fo = lFhbnd + ((cell * 2 * D + f) * NF + fn) * V;
+      const double* fnb = g >= 0 ? gf + (g * NF + fn) * V : lFhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * V;
+#pragma unroll
+      for (int k = 0; k < ND; k++) {
+        rQL[k] = o ? qo[k] : qn[k];
+        rQR[k] = o ? qn[k] : qo[k];
+      }
+#pragma unroll
+      for (int k = 0; k < V; k++) {
+        rFL[k] = o ? fo[k] : fnb[k];
+        rFR[k] = o ? fnb[k] : fo[k];
+      }
+      const double wj = (D == 3) ? w[fn / B] * w[fn % B] : w[fn];
+#pragma unroll
+      for (int k = 0; k < ND; k++) {
+        red[(task * 2 + 0) * ND + k] = wj * rQL[k];
+        red[(task * 2 + 1) * ND + k] = wj * rQR[k];
+      }
+    }
+  } else if (valid) {
     for (int task = s; task < NTASK; task += NS) {
       const int f = task / NF, fn = task - f * NF, o = f & 1;
       const long g = ghost_face<D>(cell, n, f, bcmask);
@@ -715,9 +771,21 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
   }
   __syncthreads();
   }
+  // Rusanov speed of each face from its two averages (riemannSolverNonlinear.cpph:51-72)
+  if (valid && s < 2 * D)
+    sm[s] = fmax(pde_max_abs_eig<D>(P, avg + (s * 2 + 0) * ND, s >> 1),
+                 pde_max_abs_eig<D>(P, avg + (s * 2 + 1) * ND, s >> 1));
+  __syncthreads();
   // phase 3: fluxes into red[task][V] (red's partial products were consumed
-  // in phase 2; phase 3 reads only avg and global memory)
-  if (valid) {
+  // in phase 2; phase 3 reads only sm and the face data)
+  if constexpr (ONE) {
+    if (valid && s < NTASK) {
+      const int task = s;
+      const double smax = sm[task / NF];
+#pragma unroll
+      for (int k = 0; k < V; k++) red[task * V + k] = 0.5 * (rFR[k] + rFL[k]) - 0.5 * smax * (rQR[k] - rQL[k]);
+    }
+  } else if (valid) {
     for (int task = s; task < NTASK; task += NS) {
       const int f = task / NF, fn = task - f * NF, d = f >> 1, o = f & 1;
       const long g = ghost_face<D>(cell, n, f, bcmask);
@@ -730,8 +798,7 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
       const double* QR = o ? qn : qo;
       const double* FL = o ? fo : fnb;
       const double* FR = o ? fnb : fo;
-      const double smax = fmax(pde_max_abs_eig<D>(P, avg + (f * 2 + 0) * ND, d),
-                               pde_max_abs_eig<D>(P, avg + (f * 2 + 1) * ND, d));
+      const double smax = sm[f];
 #pragma unroll
       for (int k = 0; k < V; k++) red[task * V + k] = 0.5 * (FR[k] + FL[k]) - 0.5 * smax * (QR[k] - QL[k]);
     }
@@ -744,9 +811,6 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
     node_zyx<D, B>(s, z, y, x);
     const double h = hcell[cell];
     const double dt = *dtp;
-    double Q[ND];
-#pragma unroll
-    for (int v = 0; v < ND; v++) Q[v] = luh[(cell * NS + s) * ND + v];
     double du[V];
 #pragma unroll
     for (int v = 0; v < V; v++) du[v] = 0.0;
