@@ -388,7 +388,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
     double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, PdeParams P,
     int maxIter, int adaptive, double* luh_upd) {
   using S = LShape<B>;
-  constexpr int V = S::V, NS = S::NS, NF = S::NF, TILE = S::TILE;
+  constexpr int V = S::V, NS = S::NS, TILE = S::TILE;
   static_assert(S::CPB == 1, "one cell per workgroup");
   static_assert(!MF || (sizeof(R) == 4 && B <= 8), "matrix-core form: FP32, B <= 8");
   constexpr int NWAVE = S::THREADS / 64;
@@ -408,7 +408,6 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
 
   const double dt = dtp[cinst ? cinst[cell] : 0];
   const double h = hcell[cell];
-  const double* w = T->w;
   // operators (uniform across lanes; read through the scalar cache at use):
   //   Dm[i][j] = dudx[i][j] (derivative), Pm[i][j] = iK1[i][j] * w[j]
   const R* Dm = tab_dudx<R>(T);
