@@ -656,12 +656,12 @@ __device__ __forceinline__ long ghost_face(long cell, int n, int f, unsigned bcm
   return long(f) * ((D == 3) ? long(n) * n : long(n)) + tr;
 }
 
-template <int D, int B, int V, int NP>
+template <int D, int B, int V, int NP, typename TF>
 __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riemann_kernel(
-    long ncells, int n, double* __restrict__ luh, const double* __restrict__ lQhbnd,
-    const double* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
+    long ncells, int n, double* __restrict__ luh, const TF* __restrict__ lQhbnd,
+    const TF* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
     double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P, double cflpnpm,
-    const double* __restrict__ gq, const double* __restrict__ gf, unsigned bcmask) {
+    const TF* __restrict__ gq, const TF* __restrict__ gf, unsigned bcmask) {
   using S = Shape<D, B, V, NP>;
   using FS = FusedShape<D, B, V, NP>;
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, THREADS = S::THREADS;
@@ -695,10 +695,10 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
       const int f = task / NF, fn = task - f * NF, o = f & 1;
       const long g = ghost_face<D>(cell, n, f, bcmask);
       const long nb = g >= 0 ? 0 : periodic_neighbour<D>(cell, n, f >> 1, o ? 1 : -1);
-      const double* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
-      const double* qn = g >= 0 ? gq + (g * NF + fn) * ND : lQhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * ND;
-      const double* fo = lFhbnd + ((cell * 2 * D + f) * NF + fn) * V;
-      const double* fnb = g >= 0 ? gf + (g * NF + fn) * V : lFhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * V;
+      const TF* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
+      const TF* qn = g >= 0 ? gq + (g * NF + fn) * ND : lQhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * ND;
+      const TF* fo = lFhbnd + ((cell * 2 * D + f) * NF + fn) * V;
+      const TF* fnb = g >= 0 ? gf + (g * NF + fn) * V : lFhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * V;
 #pragma unroll
       for (int k = 0; k < ND; k++) {
         rQL[k] = o ? qo[k] : qn[k];
@@ -720,17 +720,17 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
     for (int task = s; task < NTASK; task += NS) {
       const int f = task / NF, fn = task - f * NF, o = f & 1;
       const long g = ghost_face<D>(cell, n, f, bcmask);
-      const double* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
-      const double* qn = g >= 0 ? gq + (g * NF + fn) * ND
-                                : lQhbnd + ((periodic_neighbour<D>(cell, n, f >> 1, o ? 1 : -1) * 2 * D + (f ^ 1)) * NF +
-                                            fn) * ND;
-      const double* QL = o ? qo : qn;
-      const double* QR = o ? qn : qo;
+      const TF* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
+      const TF* qn = g >= 0 ? gq + (g * NF + fn) * ND
+                            : lQhbnd + ((periodic_neighbour<D>(cell, n, f >> 1, o ? 1 : -1) * 2 * D + (f ^ 1)) * NF +
+                                        fn) * ND;
+      const TF* QL = o ? qo : qn;
+      const TF* QR = o ? qn : qo;
       const double wj = (D == 3) ? w[fn / B] * w[fn % B] : w[fn];
 #pragma unroll
       for (int k = 0; k < ND; k++) {
-        red[(task * 2 + 0) * ND + k] = wj * QL[k];
-        red[(task * 2 + 1) * ND + k] = wj * QR[k];
+        red[(task * 2 + 0) * ND + k] = wj * static_cast<double>(QL[k]);
+        red[(task * 2 + 1) * ND + k] = wj * static_cast<double>(QR[k]);
       }
     }
   }
@@ -790,17 +790,20 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
       const int f = task / NF, fn = task - f * NF, d = f >> 1, o = f & 1;
       const long g = ghost_face<D>(cell, n, f, bcmask);
       const long nb = g >= 0 ? 0 : periodic_neighbour<D>(cell, n, d, o ? 1 : -1);
-      const double* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
-      const double* qn = g >= 0 ? gq + (g * NF + fn) * ND : lQhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * ND;
-      const double* fo = lFhbnd + ((cell * 2 * D + f) * NF + fn) * V;
-      const double* fnb = g >= 0 ? gf + (g * NF + fn) * V : lFhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * V;
-      const double* QL = o ? qo : qn;
-      const double* QR = o ? qn : qo;
-      const double* FL = o ? fo : fnb;
-      const double* FR = o ? fnb : fo;
+      const TF* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
+      const TF* qn = g >= 0 ? gq + (g * NF + fn) * ND : lQhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * ND;
+      const TF* fo = lFhbnd + ((cell * 2 * D + f) * NF + fn) * V;
+      const TF* fnb = g >= 0 ? gf + (g * NF + fn) * V : lFhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * V;
+      const TF* QL = o ? qo : qn;
+      const TF* QR = o ? qn : qo;
+      const TF* FL = o ? fo : fnb;
+      const TF* FR = o ? fnb : fo;
       const double smax = sm[f];
 #pragma unroll
-      for (int k = 0; k < V; k++) red[task * V + k] = 0.5 * (FR[k] + FL[k]) - 0.5 * smax * (QR[k] - QL[k]);
+      for (int k = 0; k < V; k++) {
+        const double fr = FR[k], fl = FL[k], qr = QR[k], ql = QL[k];
+        red[task * V + k] = 0.5 * (fr + fl) - 0.5 * smax * (qr - ql);
+      }
     }
   }
   __syncthreads();
@@ -951,8 +954,19 @@ struct Launch {
     if constexpr (FusedShape<D, B, V, NP>::ok) {
       if (n <= 0) return 0;
       const long blocks = (n + S::CPB - 1) / S::CPB;
-      hipLaunchKernelGGL((corrector_riemann_kernel<D, B, V, NP>), dim3(blocks), dim3(S::THREADS), 0, st, n, nside,
-                         luh, qb, fb, h, dt, dt_next, k.tab, k.pde, k.cfl * k.pnpm, o.ghost_q, o.ghost_f,
+      if (o.faces_f32) {
+        if constexpr (D == 3 && B == 6 && V == 5 && NP == 0) {  // the FP32 line predictor's shape
+          hipLaunchKernelGGL((corrector_riemann_kernel<D, B, V, NP, float>), dim3(blocks), dim3(S::THREADS), 0, st, n,
+                             nside, luh, reinterpret_cast<const float*>(qb), reinterpret_cast<const float*>(fb), h, dt,
+                             dt_next, k.tab, k.pde, k.cfl * k.pnpm, reinterpret_cast<const float*>(o.ghost_q),
+                             reinterpret_cast<const float*>(o.ghost_f), o.bcmask);
+          return static_cast<int>(hipGetLastError());
+        } else {
+          return -1;
+        }
+      }
+      hipLaunchKernelGGL((corrector_riemann_kernel<D, B, V, NP, double>), dim3(blocks), dim3(S::THREADS), 0, st, n,
+                         nside, luh, qb, fb, h, dt, dt_next, k.tab, k.pde, k.cfl * k.pnpm, o.ghost_q, o.ghost_f,
                          o.bcmask);
       return static_cast<int>(hipGetLastError());
     } else {
@@ -1027,11 +1041,15 @@ int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* lu
     const int v = k.pred_variant;
     const bool fp32 = k.precision != 0;
     if ((v == 0 && fp32) || v == 3)
-      return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, fp32, o.cinst, o.luh_update);
+      return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, fp32, o.cinst, o.luh_update,
+                                      o.faces_f32);
     if (v == 2)
-      return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, false, o.cinst, o.luh_update);
-    if (v == 4 && fp32) return launch_predictor3d_tile(k, s, n, luh, h, dt, lduh, qb, fb, pic, o.cinst, o.luh_update);
+      return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, false, o.cinst, o.luh_update,
+                                      o.faces_f32);
+    if (v == 4 && fp32)
+      return launch_predictor3d_tile(k, s, n, luh, h, dt, lduh, qb, fb, pic, o.cinst, o.luh_update, o.faces_f32);
   }
+  if (o.faces_f32) return -1;  // the generic kernels write FP64 faces
   return e->predictor(k, s, n, luh, h, dt, lduh, qb, fb, pic, o);
 }
 

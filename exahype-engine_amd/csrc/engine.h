@@ -66,6 +66,9 @@ struct LaunchOpts {
   const double* ghost_q = nullptr;
   double* ghost_f = nullptr;
   unsigned bcmask = 0;
+  // FP32 predictors and the fused corrector: extrapolated faces stored as FP32
+  // (the FP32 forms compute them in FP32, so the storage is exact)
+  bool faces_f32 = false;
 };
 
 // Boundary kinds per domain face f = 2d + o (exa_boundary_kind).
@@ -102,12 +105,15 @@ int launch_solution_update(const KernelCfg& k, hipStream_t s, double* luh, const
 bool lines_supported(const KernelCfg& k);
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
                              const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard,
-                             bool mfma, const int* cinst = nullptr, double* luh_update = nullptr);
+                             bool mfma, const int* cinst = nullptr, double* luh_update = nullptr,
+                             bool f32faces = false);
 // FP32 tile form of the same predictor (3D Euler, order 5): space-time iterate
 // in LDS, contractions on 16x16x4 matrix-core steps; -1 for FP64.
 int launch_predictor3d_tile(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
                             const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard,
-                            const int* cinst = nullptr, double* luh_update = nullptr);
+                            const int* cinst = nullptr, double* luh_update = nullptr, bool f32faces = false);
+// The predictor selected by k writes FP32 faces on request (LaunchOpts::faces_f32).
+bool predictor_f32_faces_supported(const KernelCfg& k);
 // Time bookkeeping of the device step per instance, state = t[ninst], dt[ninst],
 // dt_next[ninst]: t += dt; dt = dt_next; dt_next = DBL_MAX.
 int launch_advance_time(hipStream_t s, double* state, int ninst = 1);

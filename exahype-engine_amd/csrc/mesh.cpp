@@ -82,6 +82,12 @@ int enqueue_boundary(exa_mesh* m) {
                 "boundary conditions");
 }
 
+// FP32 extrapolated faces (half the face traffic, exact for the FP32 forms):
+// fused step, all faces periodic, and an FP32 predictor that can write them.
+bool f32_faces(const exa_mesh* m) {
+  return m->fused && !m->bcmask && exa::predictor_f32_faces_supported(m->k);
+}
+
 exa::LaunchOpts ghost_opts(const exa_mesh* m) {
   exa::LaunchOpts o;
   if (m->bcmask) {
@@ -89,6 +95,7 @@ exa::LaunchOpts ghost_opts(const exa_mesh* m) {
     o.ghost_f = m->d_gf;
     o.bcmask = m->bcmask;
   }
+  o.faces_f32 = f32_faces(m);
   return o;
 }
 
@@ -102,6 +109,7 @@ int enqueue_predict(exa_mesh* m, hipEvent_t* ev) {
   if (ev) HIP_OK(hipEventRecord(ev[0], m->stream), "event");
   exa::LaunchOpts o;
   if (m->fused) o.luh_update = m->d_luh;
+  o.faces_f32 = f32_faces(m);
   int rc = status(exa::launch_predictor(m->k, m->stream, m->ncells, m->d_luh, m->d_h, st + 1, m->d_lduh, m->d_qb,
                                         m->d_fb, m->d_picard, o),
                   "predictor");

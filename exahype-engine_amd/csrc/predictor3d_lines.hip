@@ -279,7 +279,7 @@ __device__ __forceinline__ void finish_cell_points(const float* P, const float* 
                                                    double dt, double h, int iters, const DevTables* __restrict__ T,
                                                    double* __restrict__ lduh, double* __restrict__ lQhbnd,
                                                    double* __restrict__ lFhbnd, int* __restrict__ picard,
-                                                   double* luh_upd, int nthreads) {
+                                                   double* luh_upd, int nthreads, int f32faces) {
   constexpr int V = 5, NS = B * B * B, NF = B * B, COLS = NS * V;
   float qh[V], fh[3][V];
 #pragma unroll
@@ -373,10 +373,19 @@ __device__ __forceinline__ void finish_cell_points(const float* P, const float* 
       f1 += fv * c1;
     }
     const long base = cell * 6 * NF * V + fn * V + v;
-    lQhbnd[base + (2 * d) * NF * V] = q0;
-    lQhbnd[base + (2 * d + 1) * NF * V] = q1;
-    lFhbnd[base + (2 * d) * NF * V] = f0;
-    lFhbnd[base + (2 * d + 1) * NF * V] = f1;
+    if (f32faces) {  // the values are FP32 numbers: FP32 storage is exact
+      float* qf = reinterpret_cast<float*>(lQhbnd);
+      float* ff = reinterpret_cast<float*>(lFhbnd);
+      qf[base + (2 * d) * NF * V] = q0;
+      qf[base + (2 * d + 1) * NF * V] = q1;
+      ff[base + (2 * d) * NF * V] = f0;
+      ff[base + (2 * d + 1) * NF * V] = f1;
+    } else {
+      lQhbnd[base + (2 * d) * NF * V] = q0;
+      lQhbnd[base + (2 * d + 1) * NF * V] = q1;
+      lFhbnd[base + (2 * d) * NF * V] = f0;
+      lFhbnd[base + (2 * d + 1) * NF * V] = f1;
+    }
   }
   if (l == 0 && picard) picard[cell] = iters;
 }
@@ -386,7 +395,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
     long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ lduh, double* __restrict__ lQhbnd,
     double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, PdeParams P,
-    int maxIter, int adaptive, double* luh_upd) {
+    int maxIter, int adaptive, double* luh_upd, int f32faces) {
   using S = LShape<B>;
   constexpr int V = S::V, NS = S::NS, TILE = S::TILE;
   static_assert(S::CPB == 1, "one cell per workgroup");
@@ -622,7 +631,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
     }
     __syncthreads();
     finish_cell_points<B>(t2, tl, t2, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd,
-                          S::THREADS);
+                          S::THREADS, f32faces);
   } else {
     finish_cell<B, R>(q, tl, buf, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd);
   }
@@ -681,7 +690,7 @@ __global__ __launch_bounds__(256, 3) void predictor3d_tile_kernel(
     long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ lduh, double* __restrict__ lQhbnd,
     double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, int maxIter,
-    int adaptive, double* luh_upd) {
+    int adaptive, double* luh_upd, int f32faces) {
   using S = TShape<B>;
   constexpr int V = S::V, NS = S::NS, TILE = S::TILE, COLS = S::COLS, NK = S::NK;
   static_assert(2 * B <= 4 * NK && B <= 8, "block-diagonal pair must fit 16 rows");
@@ -859,7 +868,8 @@ __global__ __launch_bounds__(256, 3) void predictor3d_tile_kernel(
   }
 
   // ---- tB = final iterate, tA = previous iterate ----
-  finish_cell_points<B>(tA, tB, tA, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd, S::THREADS);
+  finish_cell_points<B>(tA, tB, tA, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd, S::THREADS,
+                        f32faces);
 }
 
 }  // namespace
@@ -870,36 +880,41 @@ bool lines_supported(const KernelCfg& k) {
 
 int launch_predictor3d_tile(const KernelCfg& k, hipStream_t st, long n, const double* luh, const double* h,
                             const double* dt, double* lduh, double* qb, double* fb, int* pic, const int* cinst,
-                            double* luh_upd) {
+                            double* luh_upd, bool f32faces) {
   constexpr int B = 6;
   if (k.precision == 0) return -1;  // FP32 form only
   const int adaptive = k.picard_max <= 0;
   const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
   hipLaunchKernelGGL((predictor3d_tile_kernel<B>), dim3(n), dim3(TShape<B>::THREADS), 0, st, n, luh, h, dt, cinst,
-                     lduh, qb, fb, pic, k.tab, maxIter, adaptive, luh_upd);
+                     lduh, qb, fb, pic, k.tab, maxIter, adaptive, luh_upd, f32faces ? 1 : 0);
   return static_cast<int>(hipGetLastError());
 }
 
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const double* luh, const double* h,
                              const double* dt, double* lduh, double* qb, double* fb, int* pic, bool mfma,
-                             const int* cinst, double* luh_upd) {
+                             const int* cinst, double* luh_upd, bool f32faces) {
   constexpr int B = 6;
   using S = LShape<B>;
   const long blocks = (n + S::CPB - 1) / S::CPB;
   const int adaptive = k.picard_max <= 0;
   const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
+  const int f32 = (f32faces && k.precision != 0) ? 1 : 0;  // FP32 faces exist only in the FP32 forms
   if (k.precision == 0) {
     hipLaunchKernelGGL((predictor3d_lines_kernel<B, double, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h, dt,
-                       cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd);
+                       cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, 0);
   } else {
     if (mfma)
       hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h,
-                         dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd);
+                         dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
     else
       hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh,
-                         h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd);
+                         h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
   }
   return static_cast<int>(hipGetLastError());
+}
+
+bool predictor_f32_faces_supported(const KernelCfg& k) {
+  return lines_supported(k) && k.precision != 0 && k.pred_variant != 1;
 }
 
 }  // namespace exa

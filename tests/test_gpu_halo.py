@@ -78,10 +78,14 @@ def coupled_run(ctx, n, luh_a, luh_b, steps, bc_other, fused):
     return out, t
 
 
-@pytest.mark.parametrize("D,order,n", [(3, 3, 4), (2, 4, 5), (3, 5, 3)])
-def test_same_halves_reproduce_the_periodic_mesh(D, order, n):
-    print(f"[halo same] D={D} order={order} n={n}", flush=True)
-    ctx = E.Context(D, order)
+@pytest.mark.parametrize("D,order,n,precision", [(3, 3, 4, E.PREC_FP64), (2, 4, 5, E.PREC_FP64),
+                                                 (3, 5, 3, E.PREC_FP64), (3, 5, 3, E.PREC_FP32)])
+def test_same_halves_reproduce_the_periodic_mesh(D, order, n, precision):
+    """In the FP32 form the periodic mesh stores its extrapolated faces in FP32
+    while exchange faces keep FP64 storage: equality pins that the FP32
+    storage is exact."""
+    print(f"[halo same] D={D} order={order} n={n} precision={precision}", flush=True)
+    ctx = E.Context(D, order, precision=precision, picard_max=0 if precision == E.PREC_FP64 else 3)
     luh = state(P.node_points(n, order, D), D, 1)
     for fused in (True, False):
         # the single periodic mesh in the same step form (the two forms agree only to rounding)
