@@ -517,7 +517,7 @@ __device__ __forceinline__ void corrector_tile(
     long blk, long ncells, double* __restrict__ luh, const double* __restrict__ lduh,
     const double* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P,
-    double cflpnpm, int do_update, double* sred) {
+    double cflpnpm, int do_update, double* sred, double* __restrict__ slots) {
   using S = Shape<D, B, V, NP>;
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, THREADS = S::THREADS;
   const int tid = threadIdx.x;
@@ -571,7 +571,8 @@ __device__ __forceinline__ void corrector_tile(
   const bool uniform = !cinst || cinst[first] == cinst[last];
   if (uniform) {
     const double bmin = block_min<THREADS>(mydt, sred);
-    if (tid == 0 && bmin < DBL_MAX) atomic_min_pos_double(dt_next + (cinst ? cinst[first] : 0), bmin);
+    double* dst = (slots && !cinst) ? slots + (blk % kDtSlots) * kDtSlotStride : dt_next + (cinst ? cinst[first] : 0);
+    if (tid == 0 && bmin < DBL_MAX) atomic_min_pos_double(dst, bmin);
   } else if (valid && mydt < DBL_MAX) {
     atomic_min_pos_double(dt_next + inst, mydt);
   }
@@ -582,10 +583,10 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_kerne
     long ncells, long ntiles, double* __restrict__ luh, const double* __restrict__ lduh,
     const double* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P,
-    double cflpnpm, int do_update) {
+    double cflpnpm, int do_update, double* __restrict__ slots) {
   __shared__ double sred[Shape<D, B, V, NP>::THREADS / 64];
   corrector_tile<D, B, V, NP>(blockIdx.x, ncells, luh, lduh, lFhbnd, hcell, dtp, cinst, dt_next, T, P, cflpnpm,
-                              do_update, sred);
+                              do_update, sred, slots);
 }
 
 template <int D, int B, int V, int NP>
@@ -593,11 +594,11 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_persi
     long ncells, long ntiles, double* __restrict__ luh, const double* __restrict__ lduh,
     const double* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P,
-    double cflpnpm, int do_update) {
+    double cflpnpm, int do_update, double* __restrict__ slots) {
   __shared__ double sred[Shape<D, B, V, NP>::THREADS / 64];
   for (long blk = blockIdx.x; blk < ntiles; blk += gridDim.x) {
     corrector_tile<D, B, V, NP>(blk, ncells, luh, lduh, lFhbnd, hcell, dtp, cinst, dt_next, T, P, cflpnpm,
-                                do_update, sred);
+                                do_update, sred, slots);
     __syncthreads();
   }
 }
@@ -656,12 +657,23 @@ __device__ __forceinline__ long ghost_face(long cell, int n, int f, unsigned bcm
   return long(f) * ((D == 3) ? long(n) * n : long(n)) + tr;
 }
 
+// Value k of face node fn in the face block at p (NF nodes, M values per node):
+// the reference layout [node][value] for FP64 faces, [value][node] for the
+// FP32 faces of the FP32 step (coalesced per value across a face's nodes).
+template <typename TF>
+__device__ __forceinline__ double face_val(const TF* p, int fn, int k, int NF, int M) {
+  if constexpr (sizeof(TF) == 4)
+    return static_cast<double>(p[k * NF + fn]);
+  else
+    return p[fn * M + k];
+}
+
 template <int D, int B, int V, int NP, typename TF>
 __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riemann_kernel(
     long ncells, int n, double* __restrict__ luh, const TF* __restrict__ lQhbnd,
     const TF* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
     double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P, double cflpnpm,
-    const TF* __restrict__ gq, const TF* __restrict__ gf, unsigned bcmask) {
+    const TF* __restrict__ gq, const TF* __restrict__ gf, unsigned bcmask, double* __restrict__ slots) {
   using S = Shape<D, B, V, NP>;
   using FS = FusedShape<D, B, V, NP>;
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, THREADS = S::THREADS;
@@ -695,19 +707,22 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
       const int f = task / NF, fn = task - f * NF, o = f & 1;
       const long g = ghost_face<D>(cell, n, f, bcmask);
       const long nb = g >= 0 ? 0 : periodic_neighbour<D>(cell, n, f >> 1, o ? 1 : -1);
-      const TF* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
-      const TF* qn = g >= 0 ? gq + (g * NF + fn) * ND : lQhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * ND;
-      const TF* fo = lFhbnd + ((cell * 2 * D + f) * NF + fn) * V;
-      const TF* fnb = g >= 0 ? gf + (g * NF + fn) * V : lFhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * V;
+      // face blocks (NF nodes each) of this cell and of the neighbour or ghost
+      const TF* qo = lQhbnd + (cell * 2 * D + f) * NF * ND;
+      const TF* qn = g >= 0 ? gq + g * NF * ND : lQhbnd + (nb * 2 * D + (f ^ 1)) * NF * ND;
+      const TF* fo = lFhbnd + (cell * 2 * D + f) * NF * V;
+      const TF* fnb = g >= 0 ? gf + g * NF * V : lFhbnd + (nb * 2 * D + (f ^ 1)) * NF * V;
 #pragma unroll
       for (int k = 0; k < ND; k++) {
-        rQL[k] = o ? qo[k] : qn[k];
-        rQR[k] = o ? qn[k] : qo[k];
+        const double a = face_val(qo, fn, k, NF, ND), b = face_val(qn, fn, k, NF, ND);
+        rQL[k] = o ? a : b;
+        rQR[k] = o ? b : a;
       }
 #pragma unroll
       for (int k = 0; k < V; k++) {
-        rFL[k] = o ? fo[k] : fnb[k];
-        rFR[k] = o ? fnb[k] : fo[k];
+        const double a = face_val(fo, fn, k, NF, V), b = face_val(fnb, fn, k, NF, V);
+        rFL[k] = o ? a : b;
+        rFR[k] = o ? b : a;
       }
       const double wj = (D == 3) ? w[fn / B] * w[fn % B] : w[fn];
 #pragma unroll
@@ -720,17 +735,16 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
     for (int task = s; task < NTASK; task += NS) {
       const int f = task / NF, fn = task - f * NF, o = f & 1;
       const long g = ghost_face<D>(cell, n, f, bcmask);
-      const TF* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
-      const TF* qn = g >= 0 ? gq + (g * NF + fn) * ND
-                            : lQhbnd + ((periodic_neighbour<D>(cell, n, f >> 1, o ? 1 : -1) * 2 * D + (f ^ 1)) * NF +
-                                        fn) * ND;
+      const TF* qo = lQhbnd + (cell * 2 * D + f) * NF * ND;
+      const TF* qn = g >= 0 ? gq + g * NF * ND
+                            : lQhbnd + (periodic_neighbour<D>(cell, n, f >> 1, o ? 1 : -1) * 2 * D + (f ^ 1)) * NF * ND;
       const TF* QL = o ? qo : qn;
       const TF* QR = o ? qn : qo;
       const double wj = (D == 3) ? w[fn / B] * w[fn % B] : w[fn];
 #pragma unroll
       for (int k = 0; k < ND; k++) {
-        red[(task * 2 + 0) * ND + k] = wj * static_cast<double>(QL[k]);
-        red[(task * 2 + 1) * ND + k] = wj * static_cast<double>(QR[k]);
+        red[(task * 2 + 0) * ND + k] = wj * face_val(QL, fn, k, NF, ND);
+        red[(task * 2 + 1) * ND + k] = wj * face_val(QR, fn, k, NF, ND);
       }
     }
   }
@@ -790,10 +804,10 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
       const int f = task / NF, fn = task - f * NF, d = f >> 1, o = f & 1;
       const long g = ghost_face<D>(cell, n, f, bcmask);
       const long nb = g >= 0 ? 0 : periodic_neighbour<D>(cell, n, d, o ? 1 : -1);
-      const TF* qo = lQhbnd + ((cell * 2 * D + f) * NF + fn) * ND;
-      const TF* qn = g >= 0 ? gq + (g * NF + fn) * ND : lQhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * ND;
-      const TF* fo = lFhbnd + ((cell * 2 * D + f) * NF + fn) * V;
-      const TF* fnb = g >= 0 ? gf + (g * NF + fn) * V : lFhbnd + ((nb * 2 * D + (f ^ 1)) * NF + fn) * V;
+      const TF* qo = lQhbnd + (cell * 2 * D + f) * NF * ND;
+      const TF* qn = g >= 0 ? gq + g * NF * ND : lQhbnd + (nb * 2 * D + (f ^ 1)) * NF * ND;
+      const TF* fo = lFhbnd + (cell * 2 * D + f) * NF * V;
+      const TF* fnb = g >= 0 ? gf + g * NF * V : lFhbnd + (nb * 2 * D + (f ^ 1)) * NF * V;
       const TF* QL = o ? qo : qn;
       const TF* QR = o ? qn : qo;
       const TF* FL = o ? fo : fnb;
@@ -801,7 +815,8 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
       const double smax = sm[f];
 #pragma unroll
       for (int k = 0; k < V; k++) {
-        const double fr = FR[k], fl = FL[k], qr = QR[k], ql = QL[k];
+        const double fr = face_val(FR, fn, k, NF, V), fl = face_val(FL, fn, k, NF, V);
+        const double qr = face_val(QR, fn, k, NF, ND), ql = face_val(QL, fn, k, NF, ND);
         red[task * V + k] = 0.5 * (fr + fl) - 0.5 * smax * (qr - ql);
       }
     }
@@ -842,7 +857,8 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
     mydt = node_cfl_dt<D, B, V, NP>(Q, P, 1.0 / h, cflpnpm);
   }
   const double bmin = block_min<THREADS>(mydt, sred);
-  if (tid == 0 && bmin < DBL_MAX) atomic_min_pos_double(dt_next, bmin);
+  double* dst = slots ? slots + (blockIdx.x % kDtSlots) * kDtSlotStride : dt_next;
+  if (tid == 0 && bmin < DBL_MAX) atomic_min_pos_double(dst, bmin);
 }
 
 // Single-cell helpers used by the per-cell compatibility entry points.
@@ -942,10 +958,10 @@ struct Launch {
     const long grid = grid_of(tiles, o);
     if (grid < tiles)
       hipLaunchKernelGGL((corrector_persistent_kernel<D, B, V, NP>), dim3(grid), dim3(S::THREADS), 0, st, n, tiles,
-                         luh, lduh, fb, h, dt, o.cinst, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update);
+                         luh, lduh, fb, h, dt, o.cinst, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update, o.dt_slots);
     else
       hipLaunchKernelGGL((corrector_kernel<D, B, V, NP>), dim3(grid), dim3(S::THREADS), 0, st, n, tiles, luh, lduh,
-                         fb, h, dt, o.cinst, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update);
+                         fb, h, dt, o.cinst, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update, o.dt_slots);
     return static_cast<int>(hipGetLastError());
   }
   static int corrector_riemann(const KernelCfg& k, hipStream_t st, long n, int nside, double* luh,
@@ -959,7 +975,7 @@ struct Launch {
           hipLaunchKernelGGL((corrector_riemann_kernel<D, B, V, NP, float>), dim3(blocks), dim3(S::THREADS), 0, st, n,
                              nside, luh, reinterpret_cast<const float*>(qb), reinterpret_cast<const float*>(fb), h, dt,
                              dt_next, k.tab, k.pde, k.cfl * k.pnpm, reinterpret_cast<const float*>(o.ghost_q),
-                             reinterpret_cast<const float*>(o.ghost_f), o.bcmask);
+                             reinterpret_cast<const float*>(o.ghost_f), o.bcmask, o.dt_slots);
           return static_cast<int>(hipGetLastError());
         } else {
           return -1;
@@ -967,7 +983,7 @@ struct Launch {
       }
       hipLaunchKernelGGL((corrector_riemann_kernel<D, B, V, NP, double>), dim3(blocks), dim3(S::THREADS), 0, st, n,
                          nside, luh, qb, fb, h, dt, dt_next, k.tab, k.pde, k.cfl * k.pnpm, o.ghost_q, o.ghost_f,
-                         o.bcmask);
+                         o.bcmask, o.dt_slots);
       return static_cast<int>(hipGetLastError());
     } else {
       return -1;
@@ -1105,6 +1121,33 @@ int launch_solution_update(const KernelCfg& k, hipStream_t s, double* luh, const
   const Entry* e = find_entry(k);
   if (!e) return -1;
   return e->solution_update(k, s, luh, luhOld, lduh, dt);
+}
+
+namespace {
+__global__ __launch_bounds__(256) void dt_reduce_kernel(double* slots, double* dt_next) {
+  __shared__ double sred[4];
+  double m = DBL_MAX;
+  for (int i = threadIdx.x; i < kDtSlots; i += 256) {
+    m = fmin(m, slots[i * kDtSlotStride]);
+    slots[i * kDtSlotStride] = DBL_MAX;  // ready for the next step
+  }
+  const double bmin = block_min<256>(m, sred);
+  if (threadIdx.x == 0 && bmin < DBL_MAX) atomic_min_pos_double(dt_next, bmin);
+}
+
+__global__ __launch_bounds__(256) void dt_slots_init_kernel(double* slots) {
+  for (int i = threadIdx.x; i < kDtSlots; i += 256) slots[i * kDtSlotStride] = DBL_MAX;
+}
+}  // namespace
+
+int launch_dt_reduce(hipStream_t s, double* slots, double* dt_next) {
+  hipLaunchKernelGGL(dt_reduce_kernel, dim3(1), dim3(256), 0, s, slots, dt_next);
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_dt_slots_init(hipStream_t s, double* slots) {
+  hipLaunchKernelGGL(dt_slots_init_kernel, dim3(1), dim3(256), 0, s, slots);
+  return static_cast<int>(hipGetLastError());
 }
 
 int launch_advance_time(hipStream_t s, double* state, int ninst) {

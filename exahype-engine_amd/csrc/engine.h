@@ -69,7 +69,20 @@ struct LaunchOpts {
   // FP32 predictors and the fused corrector: extrapolated faces stored as FP32
   // (the FP32 forms compute them in FP32, so the storage is exact)
   bool faces_f32 = false;
+  // correctors without cinst: workgroup minima of the stable dt go to one of
+  // kDtSlots slots (separate cache lines) instead of all to dt_next; a
+  // launch_dt_reduce then merges the slots into dt_next and resets them
+  double* dt_slots = nullptr;
 };
+
+// Slots of the spread dt reduction: one device-scope atomic per workgroup on a
+// single address serialises at that address (~10 ns each, 0.5 M workgroups at
+// 81^3); spread over 1024 cache lines the atomics proceed in parallel.
+constexpr int kDtSlots = 1024;
+constexpr int kDtSlotStride = 16;  // doubles: one 128-byte line per slot
+// slots [kDtSlots * kDtSlotStride] -> dt_next = min(dt_next, min slots); slots reset to DBL_MAX
+int launch_dt_reduce(hipStream_t s, double* slots, double* dt_next);
+int launch_dt_slots_init(hipStream_t s, double* slots);
 
 // Boundary kinds per domain face f = 2d + o (exa_boundary_kind).
 enum BcKind { kBcPeriodic = 0, kBcWall = 1, kBcDirichletEntropyWave = 2, kBcExchange = 3 };

@@ -52,6 +52,8 @@ struct exa_mesh {
   bool hooks = false;  // some face takes its ghost data from a boundary-value hook
   double *d_gq = nullptr, *d_gf = nullptr;
   double* d_halo = nullptr;  // staging of the host-side halo entry points
+  // per-workgroup stable-dt minima spread over kDtSlots cache lines (DBL_MAX between steps)
+  double* d_slots = nullptr;
 };
 
 namespace {
@@ -96,6 +98,7 @@ exa::LaunchOpts ghost_opts(const exa_mesh* m) {
     o.bcmask = m->bcmask;
   }
   o.faces_f32 = f32_faces(m);
+  o.dt_slots = m->d_slots;
   return o;
 }
 
@@ -120,19 +123,25 @@ int enqueue_predict(exa_mesh* m, hipEvent_t* ev) {
 
 int enqueue_correct(exa_mesh* m, hipEvent_t* ev) {
   double* st = m->d_state;
+  int rc;
   if (m->fused) {
     if (ev) HIP_OK(hipEventRecord(ev[2], m->stream), "event");
-    return status(exa::launch_corrector_riemann(m->k, m->stream, m->ncells, m->n, m->d_luh, m->d_qb, m->d_fb, m->d_h,
-                                                st + 1, st + 2, ghost_opts(m)),
-                  "corrector");
-  }
-  int rc = status(exa::launch_riemann(m->k, m->stream, m->nfaces, m->d_faces, m->d_qb, m->d_fb, ghost_opts(m)),
-                  "riemann");
-  if (rc) return rc;
-  if (ev) HIP_OK(hipEventRecord(ev[2], m->stream), "event");
-  return status(exa::launch_corrector(m->k, m->stream, m->ncells, m->d_luh, m->d_lduh, m->d_fb, m->d_h, st + 1,
-                                      st + 2),
+    rc = status(exa::launch_corrector_riemann(m->k, m->stream, m->ncells, m->n, m->d_luh, m->d_qb, m->d_fb, m->d_h,
+                                              st + 1, st + 2, ghost_opts(m)),
                 "corrector");
+  } else {
+    rc = status(exa::launch_riemann(m->k, m->stream, m->nfaces, m->d_faces, m->d_qb, m->d_fb, ghost_opts(m)),
+                "riemann");
+    if (rc) return rc;
+    if (ev) HIP_OK(hipEventRecord(ev[2], m->stream), "event");
+    exa::LaunchOpts o;
+    o.dt_slots = m->d_slots;
+    rc = status(exa::launch_corrector(m->k, m->stream, m->ncells, m->d_luh, m->d_lduh, m->d_fb, m->d_h, st + 1,
+                                      st + 2, o),
+                "corrector");
+  }
+  if (rc) return rc;
+  return status(exa::launch_dt_reduce(m->stream, m->d_slots, st + 2), "dt reduction");
 }
 
 int enqueue_advance(exa_mesh* m, hipEvent_t* ev) {
@@ -153,7 +162,7 @@ void free_mesh(exa_mesh* m) {
   if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
   if (m->graph) (void)hipGraphDestroy(m->graph);
   for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
-  double* bufs[] = {m->d_luh, m->d_lduh, m->d_qb, m->d_fb, m->d_h, m->d_state, m->d_gq, m->d_gf, m->d_halo};
+  double* bufs[] = {m->d_luh, m->d_lduh, m->d_qb, m->d_fb, m->d_h, m->d_state, m->d_gq, m->d_gf, m->d_halo, m->d_slots};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (m->d_faces) (void)hipFree(m->d_faces);
@@ -235,6 +244,7 @@ int exa_mesh_create_uniform(exa_mesh** out, exa_ctx* ctx, int n, double h) {
   // room for the face list of any boundary choice: (n + 1) n^(D-1) entries per direction
   if (e == hipSuccess) e = hipMalloc(&m->d_faces, sizeof(int) * 4 * D * (n + 1) * (m->ncells / n));
   if (e == hipSuccess) e = hipMalloc(&m->d_picard, sizeof(int) * m->ncells);
+  if (e == hipSuccess) e = hipMalloc(&m->d_slots, sizeof(double) * exa::kDtSlots * exa::kDtSlotStride);
   if (e != hipSuccess) {
     free_mesh(m);
     return fail(e, "exa_mesh_create_uniform");
@@ -246,6 +256,8 @@ int exa_mesh_create_uniform(exa_mesh** out, exa_ctx* ctx, int n, double h) {
   if (e == hipSuccess) e = hipMemcpy(m->d_h, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(m->d_state, st, sizeof(st), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(m->d_picard, 0, sizeof(int) * m->ncells);
+  if (e == hipSuccess) e = static_cast<hipError_t>(exa::launch_dt_slots_init(m->stream, m->d_slots));
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
   if (e != hipSuccess) {
     free_mesh(m);
     return fail(e, "exa_mesh_create_uniform(upload)");
@@ -342,8 +354,11 @@ int exa_mesh_init_dt(exa_mesh* m, double dt) {
   const double st[4] = {0.0, dt > 0 ? dt : DBL_MAX, DBL_MAX, 0.0};
   HIP_OK(hipMemcpyAsync(m->d_state, st, sizeof(st), hipMemcpyHostToDevice, m->stream), "init_dt");
   if (!(dt > 0)) {
-    const int rc = status(exa::launch_stable_dt(m->k, m->stream, m->ncells, m->d_luh, m->d_h, m->d_state + 1),
-                          "initial dt");
+    exa::LaunchOpts o;
+    o.dt_slots = m->d_slots;
+    int rc = status(exa::launch_stable_dt(m->k, m->stream, m->ncells, m->d_luh, m->d_h, m->d_state + 1, o),
+                    "initial dt");
+    if (!rc) rc = status(exa::launch_dt_reduce(m->stream, m->d_slots, m->d_state + 1), "dt reduction");
     if (rc) return rc;
   }
   HIP_OK(hipStreamSynchronize(m->stream), "sync");

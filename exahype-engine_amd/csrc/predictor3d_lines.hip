@@ -356,10 +356,15 @@ __device__ __forceinline__ void finish_cell_points(const float* P, const float* 
       for (int v = 0; v < V; v++) lduh[(cell * NS + l) * V + v] = static_cast<double>(du[v]);
     }
   }
-  // extrapolation: one (direction, face node, variable) triple gives both faces
+  // extrapolation: one (direction, face node, variable) triple gives both faces.
+  // FP64 faces: the reference layout [face][node][var] (var fastest across
+  // lanes); FP32 faces: [face][var][node] (node fastest), the mesh-internal
+  // layout of the FP32 step whose corrector loads one variable row per step.
   constexpr int NTRI = 3 * NF * V;
   for (int o = l; o < NTRI; o += nthreads) {
-    const int v = o % V, fn = (o / V) % NF, d = o / (V * NF);
+    const int v = f32faces ? (o / NF) % V : o % V;
+    const int fn = f32faces ? o % NF : (o / V) % NF;
+    const int d = o / (V * NF);
     const int a = fn / B, b = fn % B;
     float q0 = 0.f, q1 = 0.f, f0 = 0.f, f1 = 0.f;
 #pragma unroll
@@ -372,15 +377,16 @@ __device__ __forceinline__ void finish_cell_points(const float* P, const float* 
       f0 += fv * c0;
       f1 += fv * c1;
     }
-    const long base = cell * 6 * NF * V + fn * V + v;
     if (f32faces) {  // the values are FP32 numbers: FP32 storage is exact
+      const long fb = cell * 6 * NF * V + v * NF + fn;
       float* qf = reinterpret_cast<float*>(lQhbnd);
       float* ff = reinterpret_cast<float*>(lFhbnd);
-      qf[base + (2 * d) * NF * V] = q0;
-      qf[base + (2 * d + 1) * NF * V] = q1;
-      ff[base + (2 * d) * NF * V] = f0;
-      ff[base + (2 * d + 1) * NF * V] = f1;
+      qf[fb + (2 * d) * NF * V] = q0;
+      qf[fb + (2 * d + 1) * NF * V] = q1;
+      ff[fb + (2 * d) * NF * V] = f0;
+      ff[fb + (2 * d + 1) * NF * V] = f1;
     } else {
+      const long base = cell * 6 * NF * V + fn * V + v;
       lQhbnd[base + (2 * d) * NF * V] = q0;
       lQhbnd[base + (2 * d + 1) * NF * V] = q1;
       lFhbnd[base + (2 * d) * NF * V] = f0;
