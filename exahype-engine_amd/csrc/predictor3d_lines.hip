@@ -148,6 +148,37 @@ __device__ __forceinline__ void euler_flux_dir<float>(const float* Q, float irho
   F[4] = (Q[4] + p) * ud;
 }
 
+// lFhi_d = sum_t w_t F_d(q[t]) at one point from its time line q[t][v]
+// (in the order of the point-parallel averages below: t ascending, FP32),
+// stored at af[(d NS) V + pv + v], pv = point index * V.
+template <int B>
+__device__ __forceinline__ void time_line_flux_averages(const float (&q)[B][LShape<B>::V],
+                                                        const DevTables* __restrict__ T, float* af, int pv) {
+  constexpr int V = 5, NS = B * B * B;
+  float fh[3][V];
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int v = 0; v < V; v++) fh[d][v] = 0.f;
+#pragma unroll
+  for (int t = 0; t < B; t++) {
+    const float wt = T->wf[t];
+    float ir, pr;
+    euler_prim<float>(q[t], ir, pr);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      float F[V];
+      euler_flux_dir<float>(q[t], ir, pr, d, F);
+#pragma unroll
+      for (int v = 0; v < V; v++) fh[d][v] += F[v] * wt;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int v = 0; v < V; v++) af[d * NS * V + pv + v] = fh[d][v];
+}
+
 // Post-Picard phase shared by the line and tile kernels (one cell per
 // workgroup): q = previous iterate (lane (t, z, y) < B^3 holds its x-line),
 // tl = final iterate [t][z][y][x][v].  Time averages lQhi = sum_t w_t lQi
@@ -275,12 +306,19 @@ __device__ __forceinline__ void finish_cell(R (&q)[B][LShape<B>::V], R* tl, unsi
 // point for all three directions.  avg: FP32 LDS scratch of 4 B^3 V values,
 // which may overlap P (it is written after a barrier).
 template <int B>
+__device__ __forceinline__ void finish_from_averages(const float* aq, const float* af, int l, long cell, double dt,
+                                                     double h, int iters, const DevTables* __restrict__ T,
+                                                     double* __restrict__ lduh, double* __restrict__ lQhbnd,
+                                                     double* __restrict__ lFhbnd, int* __restrict__ picard,
+                                                     double* luh_upd, int nthreads, int f32faces);
+
+template <int B>
 __device__ __forceinline__ void finish_cell_points(const float* P, const float* Q, float* avg, int l, long cell,
                                                    double dt, double h, int iters, const DevTables* __restrict__ T,
                                                    double* __restrict__ lduh, double* __restrict__ lQhbnd,
                                                    double* __restrict__ lFhbnd, int* __restrict__ picard,
                                                    double* luh_upd, int nthreads, int f32faces) {
-  constexpr int V = 5, NS = B * B * B, NF = B * B, COLS = NS * V;
+  constexpr int V = 5, NS = B * B * B, COLS = NS * V;
   float qh[V], fh[3][V];
 #pragma unroll
   for (int v = 0; v < V; v++) {
@@ -322,6 +360,21 @@ __device__ __forceinline__ void finish_cell_points(const float* P, const float* 
     }
   }
   __syncthreads();
+  finish_from_averages<B>(aq, af, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd, nthreads,
+                          f32faces);
+}
+
+// Volume integral and extrapolation of the FP32 forms from the time averages
+// in LDS: aq = lQhi [NS][V], af = lFhi_d [3][NS][V] (node index (z, y, x), x
+// fastest).  Lane l < NS handles node l of the volume integral; the
+// extrapolation runs over (direction, face node, variable) triples.
+template <int B>
+__device__ __forceinline__ void finish_from_averages(const float* aq, const float* af, int l, long cell, double dt,
+                                                     double h, int iters, const DevTables* __restrict__ T,
+                                                     double* __restrict__ lduh, double* __restrict__ lQhbnd,
+                                                     double* __restrict__ lFhbnd, int* __restrict__ picard,
+                                                     double* luh_upd, int nthreads, int f32faces) {
+  constexpr int V = 5, NS = B * B * B, NF = B * B;
   if (l < NS) {
     // volume integral at node s = l = (z, y, x)
     const int x = l % B, y = (l / B) % B, z = l / (B * B);
@@ -396,7 +449,7 @@ __device__ __forceinline__ void finish_cell_points(const float* P, const float* 
   if (l == 0 && picard) picard[cell] = iters;
 }
 
-template <int B, typename R, bool MF>
+template <int B, typename R, bool MF, bool FIX>
 __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel(
     long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ lduh, double* __restrict__ lQhbnd,
@@ -406,12 +459,19 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
   constexpr int V = S::V, NS = S::NS, TILE = S::TILE;
   static_assert(S::CPB == 1, "one cell per workgroup");
   static_assert(!MF || (sizeof(R) == 4 && B <= 8), "matrix-core form: FP32, B <= 8");
+  static_assert(!FIX || MF, "fixed-count form: FP32 matrix-core kernel");
+  if constexpr (FIX) adaptive = 0;  // launched only for a fixed Picard count
   constexpr int NWAVE = S::THREADS / 64;
   constexpr int TILE_BYTES = TILE * int(sizeof(R));
   constexpr int AVG_BYTES = 4 * NS * V * 8;  // FP64 form: lQhi + lFhi_{x,y,z} in FP64
-  // FP32 forms: a second tile for the previous iterate (point-parallel finish)
-  constexpr int NEED = sizeof(R) == 4 ? 2 * TILE_BYTES : TILE_BYTES;
-  constexpr int BUF_BYTES = NEED > AVG_BYTES ? NEED : AVG_BYTES;
+  // FP32 adaptive forms: a second tile for the previous iterate (point-parallel
+  // finish).  Fixed Picard count (FIX): the time lanes form the averages
+  // themselves, lFhi_d in iteration maxIter - 2 (from the iterate the last
+  // iteration evaluates fluxes of) into af behind the tile, lQhi in the last
+  // iteration; the tile then holds lQhi.
+  constexpr int AF_BYTES = 3 * NS * V * 4;
+  constexpr int NEED = FIX ? TILE_BYTES + AF_BYTES : (sizeof(R) == 4 ? 2 * TILE_BYTES : TILE_BYTES);
+  constexpr int BUF_BYTES = (FIX || NEED > AVG_BYTES) ? NEED : AVG_BYTES;
   __shared__ __align__(16) unsigned char buf[BUF_BYTES];
   __shared__ double red[NWAVE];
   R* tl = reinterpret_cast<R*>(buf);
@@ -458,6 +518,22 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
   const int ybase = (t * B + z) * B * B * V + y * V;      // y-line (t, z, *, x = y)
   const int zbase = (t * B * B * B + y * B + z) * V;      // z-line (t, *, y, x = z)
   const int tbase = ((z * B + y) * B + t) * V;            // time line of (z, y, x = t)
+
+  // FIX: time averages at this lane's time-line point, lQhi in registers, lFhi_d in af
+  [[maybe_unused]] float qh[V];
+  [[maybe_unused]] float* af = reinterpret_cast<float*>(buf + TILE_BYTES);
+  if constexpr (FIX) {
+#pragma unroll
+    for (int v = 0; v < V; v++) qh[v] = 0.f;
+    if (maxIter == 1 && lane_on) {  // the last iteration evaluates fluxes of the initial guess
+      float qi[B][V];
+#pragma unroll
+      for (int i = 0; i < B; i++)
+#pragma unroll
+        for (int v = 0; v < V; v++) qi[i][v] = u[v];
+      time_line_flux_averages<B>(qi, T, af, tbase);
+    }
+  }
 
   int iters = maxIter;
   for (int iter = 0; iter < maxIter; iter++) {
@@ -564,15 +640,44 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
     __syncthreads();
     if constexpr (MF) {
       constexpr int ts = NS * V;
+      if constexpr (FIX) {
+        float qn[B][V];  // the new iterate on this lane's time line
 #pragma unroll
-      for (int v = 0; v < V; v++) {
-        float L[B], o[B];
+        for (int v = 0; v < V; v++) {
+          float L[B], o[B];
 #pragma unroll
-        for (int n = 0; n < B; n++) L[n] = lane_on ? tl[tbase + n * ts + v] : 0.f;
-        mfma_line<B>(aP, L, o);
+          for (int n = 0; n < B; n++) L[n] = lane_on ? tl[tbase + n * ts + v] : 0.f;
+          mfma_line<B>(aP, L, o);
+#pragma unroll
+          for (int i = 0; i < B; i++) qn[i][v] = u[v] - cdt * o[i];
+        }
         if (lane_on) {
+          if (iter < maxIter - 1) {
 #pragma unroll
-          for (int i = 0; i < B; i++) tl[tbase + i * ts + v] = u[v] - cdt * o[i];
+            for (int i = 0; i < B; i++)
+#pragma unroll
+              for (int v = 0; v < V; v++) tl[tbase + i * ts + v] = qn[i][v];
+            if (iter == maxIter - 2) time_line_flux_averages<B>(qn, T, af, tbase);
+          } else {
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+              const float wt = T->wf[i];
+#pragma unroll
+              for (int v = 0; v < V; v++) qh[v] += qn[i][v] * wt;
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          float L[B], o[B];
+#pragma unroll
+          for (int n = 0; n < B; n++) L[n] = lane_on ? tl[tbase + n * ts + v] : 0.f;
+          mfma_line<B>(aP, L, o);
+          if (lane_on) {
+#pragma unroll
+            for (int i = 0; i < B; i++) tl[tbase + i * ts + v] = u[v] - cdt * o[i];
+          }
         }
       }
     } else if (lane_on) {
@@ -627,7 +732,16 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
   }
 
   // ---- after the Picard loop: q = previous iterate, the tile holds the final one ----
-  if constexpr (sizeof(R) == 4) {
+  if constexpr (FIX) {
+    // every time line has been read (barrier at the end of the last time phase)
+    if (lane_on) {
+#pragma unroll
+      for (int v = 0; v < V; v++) tl[tbase + v] = qh[v];
+    }
+    __syncthreads();
+    finish_from_averages<B>(tl, af, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd, S::THREADS,
+                            f32faces);
+  } else if constexpr (sizeof(R) == 4) {
     float* t2 = reinterpret_cast<float*>(buf) + TILE;
     if (lane_on) {
 #pragma unroll
@@ -906,15 +1020,18 @@ int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const d
   const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
   const int f32 = (f32faces && k.precision != 0) ? 1 : 0;  // FP32 faces exist only in the FP32 forms
   if (k.precision == 0) {
-    hipLaunchKernelGGL((predictor3d_lines_kernel<B, double, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h, dt,
+    hipLaunchKernelGGL((predictor3d_lines_kernel<B, double, false, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h, dt,
                        cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, 0);
   } else {
-    if (mfma)
-      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h,
-                         dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+    if (mfma && !adaptive)
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true>), dim3(blocks), dim3(S::THREADS), 0, st, n,
+                         luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+    else if (mfma)
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, false>), dim3(blocks), dim3(S::THREADS), 0, st, n,
+                         luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
     else
-      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh,
-                         h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, false, false>), dim3(blocks), dim3(S::THREADS), 0, st,
+                         n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
   }
   return static_cast<int>(hipGetLastError());
 }

@@ -340,3 +340,28 @@ def test_predictor_variants_agree(precision, variant, tol):
     assert rel_l2(outs[0][0] - luh, outs[1][0] - luh) < tol * 1e3
     assert rel_l2(outs[0][0], outs[1][0]) < tol
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("picard", [1, 2, 4])
+def test_fp32_fixed_picard_counts_agree(picard):
+    """The FP32 matrix-core line predictor with a fixed Picard count (time
+    averages formed by the time lanes: fluxes of iteration picard - 2's output,
+    or of the initial guess for one iteration) against the FP32 node-column
+    kernel with the same count."""
+    n, order, steps = 3, 5, 2
+    luh = P.density_wave_3d(n, order)
+    outs = []
+    for variant in (3, 1):
+        ctx = E.Context(3, order, 5, 0, precision=E.PREC_FP32, picard_max=picard)
+        ctx.set_predictor_variant(variant)
+        mesh = E.Mesh(ctx, n, 1.0 / n)
+        mesh.set_solution(luh)
+        mesh.init_dt(0.0)
+        mesh.run(steps)
+        mesh.sync()
+        outs.append((mesh.get_solution(), mesh.picard()))
+        mesh.close()
+        ctx.close()
+    assert rel_l2(outs[0][0] - luh, outs[1][0] - luh) < 2e-3
+    assert rel_l2(outs[0][0], outs[1][0]) < 2e-6
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
