@@ -93,6 +93,31 @@ __device__ __forceinline__ void mfma_line(const float (&aop)[2][B], const float 
   for (int i = 0; i < B; i++) out[i] = i < 4 ? c0[i] : c1[i - 4];
 }
 
+// Largest tile index (in values) any lane of the line kernel reads in the y,
+// z and t phases, lanes l >= B^3 included (their (t, z, y) run past the cell).
+template <int B>
+constexpr int lines_max_read() {
+  constexpr int V = 5;
+  int mx = 0;
+  for (int l = 0; l < LShape<B>::THREADS; l++) {
+    const int t = l / (B * B), z = (l / B) % B, y = l % B;
+    const int yb = (t * B + z) * B * B * V + y * V, zb = (t * B * B * B + y * B + z) * V;
+    const int tb = ((z * B + y) * B + t) * V;
+    const int c[3] = {yb + (B - 1) * B * V, zb + (B - 1) * B * B * V, tb + (B - 1) * B * B * B * V};
+    for (int i = 0; i < 3; i++) mx = c[i] + V - 1 > mx ? c[i] + V - 1 : mx;
+  }
+  return mx;
+}
+
+// a[h][n] = col[h B + n]: one lane's operand columns from LDS (16-byte aligned)
+template <int B>
+__device__ __forceinline__ void load_op(const float* col, float (&a)[2][B]) {
+#pragma unroll
+  for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+    for (int n = 0; n < B; n++) a[hh][n] = col[hh * B + n];
+}
+
 template <typename R>
 __device__ __forceinline__ const R* tab_dudx(const DevTables* T);
 template <>
@@ -177,6 +202,15 @@ __device__ __forceinline__ void time_line_flux_averages(const float (&q)[B][LSha
   for (int d = 0; d < 3; d++)
 #pragma unroll
     for (int v = 0; v < V; v++) af[d * NS * V + pv + v] = fh[d][v];
+}
+
+// Component v of the FP32 x flux, the same operations as euler_flux_dir<float>(.., 0, ..).
+__device__ __forceinline__ float euler_flux_x_comp(const float* Q, float irho, float p, int v) {
+  const float ud = irho * Q[1];
+  if (v == 0) return Q[1];
+  if (v == 1) return Q[1] * ud + p;
+  if (v == 4) return (Q[4] + p) * ud;
+  return Q[v] * ud;
 }
 
 // Post-Picard phase shared by the line and tile kernels (one cell per
@@ -450,7 +484,9 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
 }
 
 template <int B, typename R, bool MF, bool FIX>
-__global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel(
+// Fixed-count form: four workgroups per CU (LDS 39 KB each; the register
+// budget of 128 costs a few spills, measured faster than three workgroups)
+__global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor3d_lines_kernel(
     long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ lduh, double* __restrict__ lQhbnd,
     double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, PdeParams P,
@@ -472,6 +508,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
   constexpr int AF_BYTES = 3 * NS * V * 4;
   constexpr int NEED = FIX ? TILE_BYTES + AF_BYTES : (sizeof(R) == 4 ? 2 * TILE_BYTES : TILE_BYTES);
   constexpr int BUF_BYTES = (FIX || NEED > AVG_BYTES) ? NEED : AVG_BYTES;
+  static_assert(!MF || (lines_max_read<B>() + 1) * 4 <= BUF_BYTES, "idle lanes' reads stay inside buf");
   __shared__ __align__(16) unsigned char buf[BUF_BYTES];
   __shared__ double red[NWAVE];
   R* tl = reinterpret_cast<R*>(buf);
@@ -488,19 +525,20 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
   const R* Dm = tab_dudx<R>(T);
   const R* Pm = tab_P<R>(T);
   const R cdt = static_cast<R>(dt / h);
-  // matrix-core operand columns: aD[h][n] = dudx[4h + l%4][n], aP[h][n] = P[4h + l%4][n]
-  float aD[2][B], aP[2][B];
+  // matrix-core operand columns, staged in LDS and loaded at each contraction
+  // phase (registers are the occupancy limit): opl[0][r][h][n] = dudx[4h + r][n],
+  // opl[1][r][h][n] = P[4h + r][n] (zero rows >= B); a lane uses r = l % 4
+  __shared__ __align__(16) float opl[2][4][2 * B];
   if constexpr (MF) {
-    const int r = l & 3;
-#pragma unroll
-    for (int hh = 0; hh < 2; hh++)
-#pragma unroll
-      for (int n = 0; n < B; n++) {
-        const int row = 4 * hh + r;
-        aD[hh][n] = row < B ? T->dudxf[row * B + n] : 0.f;
-        aP[hh][n] = row < B ? T->Ptf[row * B + n] : 0.f;
-      }
+    if (l < 16 * B) {
+      const int op = l / (8 * B), r = (l / (2 * B)) % 4, hh = (l % (2 * B)) / B, n = l % B;
+      const int row = 4 * hh + r;
+      const float* tab = op ? T->Ptf : T->dudxf;
+      opl[op][r][hh * B + n] = row < B ? tab[row * B + n] : 0.f;
+    }
+    __syncthreads();
   }
+  const int opr = l & 3;
 
   // time-line role: this lane solves in time for point (z, y, x = t)
   R u[V];
@@ -550,14 +588,14 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
 #pragma unroll
       for (int v = 0; v < V; v++) G[i][v] = R(0);
     if constexpr (MF) {
-      float F[B][V];
-#pragma unroll
-      for (int n = 0; n < B; n++) euler_flux_dir<R>(q[n], irl[n], prl[n], 0, F[n]);
+      float aD[2][B];
+      load_op<B>(opl[0][opr], aD);
+      // one flux component at a time: B operands live instead of B V
 #pragma unroll
       for (int v = 0; v < V; v++) {
         float L[B], o[B];
 #pragma unroll
-        for (int n = 0; n < B; n++) L[n] = F[n][v];
+        for (int n = 0; n < B; n++) L[n] = euler_flux_x_comp(q[n], irl[n], prl[n], v);
         mfma_line<B>(aD, L, o);
 #pragma unroll
         for (int i = 0; i < B; i++) G[i][v] = o[i];
@@ -591,11 +629,15 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
       if constexpr (MF) {
         const int lb = (d == 1) ? ybase : zbase;
         const int ls = (d == 1) ? B * V : B * B * V;
+        float aD[2][B];
+        load_op<B>(opl[0][opr], aD);
 #pragma unroll
         for (int v = 0; v < V; v++) {
           float L[B], o[B];
 #pragma unroll
-          for (int n = 0; n < B; n++) L[n] = lane_on ? tl[lb + n * ls + v] : 0.f;
+          // lanes >= NS read inside buf too (lines_max_read, checked at compile time)
+          // and discard: a matrix-core column depends only on its own lane's operand
+          for (int n = 0; n < B; n++) L[n] = tl[lb + n * ls + v];
           mfma_line<B>(aD, L, o);
           if (lane_on) {
 #pragma unroll
@@ -640,13 +682,15 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
     __syncthreads();
     if constexpr (MF) {
       constexpr int ts = NS * V;
+      float aP[2][B];
+      load_op<B>(opl[1][opr], aP);
       if constexpr (FIX) {
         float qn[B][V];  // the new iterate on this lane's time line
 #pragma unroll
         for (int v = 0; v < V; v++) {
           float L[B], o[B];
 #pragma unroll
-          for (int n = 0; n < B; n++) L[n] = lane_on ? tl[tbase + n * ts + v] : 0.f;
+          for (int n = 0; n < B; n++) L[n] = tl[tbase + n * ts + v];  // lanes >= NS: as above
           mfma_line<B>(aP, L, o);
 #pragma unroll
           for (int i = 0; i < B; i++) qn[i][v] = u[v] - cdt * o[i];
@@ -672,7 +716,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS)) void predictor3d_lines_kernel
         for (int v = 0; v < V; v++) {
           float L[B], o[B];
 #pragma unroll
-          for (int n = 0; n < B; n++) L[n] = lane_on ? tl[tbase + n * ts + v] : 0.f;
+          for (int n = 0; n < B; n++) L[n] = tl[tbase + n * ts + v];  // lanes >= NS: as above
           mfma_line<B>(aP, L, o);
           if (lane_on) {
 #pragma unroll
