@@ -95,15 +95,15 @@ __device__ __forceinline__ void mfma_line(const float (&aop)[2][B], const float 
 
 // Largest tile index (in values) any lane of the line kernel reads in the y,
 // z and t phases, lanes l >= B^3 included (their (t, z, y) run past the cell).
-template <int B>
+template <int B, int LS>
 constexpr int lines_max_read() {
   constexpr int V = 5;
   int mx = 0;
   for (int l = 0; l < LShape<B>::THREADS; l++) {
     const int t = l / (B * B), z = (l / B) % B, y = l % B;
-    const int yb = (t * B + z) * B * B * V + y * V, zb = (t * B * B * B + y * B + z) * V;
-    const int tb = ((z * B + y) * B + t) * V;
-    const int c[3] = {yb + (B - 1) * B * V, zb + (B - 1) * B * B * V, tb + (B - 1) * B * B * B * V};
+    const int yb = (t * B + z) * B * LS + y * V, zb = (t * B * B + y) * LS + z * V;
+    const int tb = (z * B + y) * LS + t * V;
+    const int c[3] = {yb + (B - 1) * LS, zb + (B - 1) * B * LS, tb + (B - 1) * B * B * LS};
     for (int i = 0; i < 3; i++) mx = c[i] + V - 1 > mx ? c[i] + V - 1 : mx;
   }
   return mx;
@@ -492,7 +492,11 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
     double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, PdeParams P,
     int maxIter, int adaptive, double* luh_upd, int f32faces) {
   using S = LShape<B>;
-  constexpr int V = S::V, NS = S::NS, TILE = S::TILE;
+  constexpr int V = S::V, NS = S::NS;
+  // x-line stride in the tile; the fixed-count form pads each line by one
+  // value so that the lanes' own-line accesses (stride LS) hit distinct banks
+  constexpr int LS = B * V + (FIX ? 1 : 0);
+  constexpr int TILE = B * B * B * LS;
   static_assert(S::CPB == 1, "one cell per workgroup");
   static_assert(!MF || (sizeof(R) == 4 && B <= 8), "matrix-core form: FP32, B <= 8");
   static_assert(!FIX || MF, "fixed-count form: FP32 matrix-core kernel");
@@ -508,7 +512,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
   constexpr int AF_BYTES = 3 * NS * V * 4;
   constexpr int NEED = FIX ? TILE_BYTES + AF_BYTES : (sizeof(R) == 4 ? 2 * TILE_BYTES : TILE_BYTES);
   constexpr int BUF_BYTES = (FIX || NEED > AVG_BYTES) ? NEED : AVG_BYTES;
-  static_assert(!MF || (lines_max_read<B>() + 1) * 4 <= BUF_BYTES, "idle lanes' reads stay inside buf");
+  static_assert(!MF || (lines_max_read<B, LS>() + 1) * 4 <= BUF_BYTES, "idle lanes' reads stay inside buf");
   __shared__ __align__(16) unsigned char buf[BUF_BYTES];
   __shared__ double red[NWAVE];
   R* tl = reinterpret_cast<R*>(buf);
@@ -552,24 +556,24 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
 #pragma unroll
     for (int v = 0; v < V; v++) u[v] = static_cast<R>(luh[base + (lane_on ? t : 0) * V + v]);
   }
-  const int own = ((t * B + z) * B + y) * B * V;          // this lane's x-line in the tile
-  const int ybase = (t * B + z) * B * B * V + y * V;      // y-line (t, z, *, x = y)
-  const int zbase = (t * B * B * B + y * B + z) * V;      // z-line (t, *, y, x = z)
-  const int tbase = ((z * B + y) * B + t) * V;            // time line of (z, y, x = t)
+  // point (t, z, y, x) of the tile at ((t B + z) B + y) LS + x V
+  const int own = ((t * B + z) * B + y) * LS;             // this lane's x-line in the tile
+  const int ybase = (t * B + z) * B * LS + y * V;         // y-line (t, z, *, x = y), stride LS
+  const int zbase = (t * B * B + y) * LS + z * V;         // z-line (t, *, y, x = z), stride B LS
+  const int tbase = (z * B + y) * LS + t * V;             // time line of (z, y, x = t), stride B^2 LS
+  [[maybe_unused]] const int pv = ((z * B + y) * B + t) * V;  // that point in [node][v] arrays
 
   // FIX: time averages at this lane's time-line point, lQhi in registers, lFhi_d in af
   [[maybe_unused]] float qh[V];
   [[maybe_unused]] float* af = reinterpret_cast<float*>(buf + TILE_BYTES);
   if constexpr (FIX) {
-#pragma unroll
-    for (int v = 0; v < V; v++) qh[v] = 0.f;
     if (maxIter == 1 && lane_on) {  // the last iteration evaluates fluxes of the initial guess
       float qi[B][V];
 #pragma unroll
       for (int i = 0; i < B; i++)
 #pragma unroll
         for (int v = 0; v < V; v++) qi[i][v] = u[v];
-      time_line_flux_averages<B>(qi, T, af, tbase);
+      time_line_flux_averages<B>(qi, T, af, pv);
     }
   }
 
@@ -628,7 +632,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
       __syncthreads();
       if constexpr (MF) {
         const int lb = (d == 1) ? ybase : zbase;
-        const int ls = (d == 1) ? B * V : B * B * V;
+        const int ls = (d == 1) ? LS : B * LS;
         float aD[2][B];
         load_op<B>(opl[0][opr], aD);
 #pragma unroll
@@ -646,7 +650,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
         }
       } else if (lane_on) {
         const int lb = (d == 1) ? ybase : zbase;
-        const int ls = (d == 1) ? B * V : B * B * V;
+        const int ls = (d == 1) ? LS : B * LS;
         // one variable at a time; each lane writes back onto its own line only
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -681,7 +685,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
     }
     __syncthreads();
     if constexpr (MF) {
-      constexpr int ts = NS * V;
+      constexpr int ts = B * B * LS;
       float aP[2][B];
       load_op<B>(opl[1][opr], aP);
       if constexpr (FIX) {
@@ -701,14 +705,17 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
             for (int i = 0; i < B; i++)
 #pragma unroll
               for (int v = 0; v < V; v++) tl[tbase + i * ts + v] = qn[i][v];
-            if (iter == maxIter - 2) time_line_flux_averages<B>(qn, T, af, tbase);
+            if (iter == maxIter - 2) time_line_flux_averages<B>(qn, T, af, pv);
           } else {
+            float sum[V] = {0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int i = 0; i < B; i++) {
               const float wt = T->wf[i];
 #pragma unroll
-              for (int v = 0; v < V; v++) qh[v] += qn[i][v] * wt;
+              for (int v = 0; v < V; v++) sum[v] += qn[i][v] * wt;
             }
+#pragma unroll
+            for (int v = 0; v < V; v++) qh[v] = sum[v];  // set in the last iteration only
           }
         }
       } else {
@@ -725,7 +732,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
         }
       }
     } else if (lane_on) {
-      constexpr int ts = NS * V;
+      constexpr int ts = B * B * LS;
 #pragma unroll
       for (int v = 0; v < V; v++) {
         R L[B];
@@ -780,7 +787,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
     // every time line has been read (barrier at the end of the last time phase)
     if (lane_on) {
 #pragma unroll
-      for (int v = 0; v < V; v++) tl[tbase + v] = qh[v];
+      for (int v = 0; v < V; v++) tl[pv + v] = qh[v];
     }
     __syncthreads();
     finish_from_averages<B>(tl, af, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd, S::THREADS,
