@@ -65,6 +65,13 @@ __device__ __forceinline__ double tabI<double>(const DevTables* T, int i) { retu
 template <>
 __device__ __forceinline__ float tabI<float>(const DevTables* T, int i) { return T->iK1f[i]; }
 
+template <typename R>
+__device__ __forceinline__ R tabD(const DevTables* T, int i);
+template <>
+__device__ __forceinline__ double tabD<double>(const DevTables* T, int i) { return T->dudx[i]; }
+template <>
+__device__ __forceinline__ float tabD<float>(const DevTables* T, int i) { return T->dudxf[i]; }
+
 // Coordinates (z, y, x) of spatial node s; z = 0 in 2D.
 template <int D, int B>
 __device__ __forceinline__ void node_zyx(int s, int& z, int& y, int& x) {
@@ -107,12 +114,12 @@ __device__ __forceinline__ int stride_of(int d) {
 // One tile (CPB cells) of the predictor; called by predictor_kernel for
 // tile blk.  Threads of invalid cells return at the end, after the last
 // barrier of the tile.
-template <int D, int B, int V, int NP, typename R>
+template <int D, int B, int V, int NP, typename R, bool CG>
 __device__ __forceinline__ void predictor_tile(
     long blk, long ncells, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
     double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
-    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd) {
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect) {
   using S = Shape<D, B, V, NP>;
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, NSP = S::NSP;
   constexpr int SLAB = NS * D * V;                       // one flux slab, per cell
@@ -171,6 +178,63 @@ __device__ __forceinline__ void predictor_tile(
     base[d] = s - cc[d] * stride_of<D, B>(d);
 #pragma unroll
     for (int n = 0; n < B; n++) Kc[d][n] = tabK<R>(T, n * B + cc[d]);
+  }
+
+  // CERK initial guess of the optimised kernels
+  // (KernelGenerator/kernelgenerator/templates/aderdg/fusedSPTVI_nonlinear_cpp.template:121-173,
+  // templates/subtemplates/RK_loop.template:12-40): K1 = sum_d dudx_d F_d(luh) / dx,
+  // lwh = luh - dt K1, K2 = the same of lwh, lQi(t) = luh - dt xi_t K1 - dt xi_t^2 (K2 - K1) / 2
+  if constexpr (CG) {
+    R* slab = reinterpret_cast<R*>(buf) + cslot * SLAB;
+    R K1[V], K2[V], lw[ND];
+#pragma unroll
+    for (int v = 0; v < ND; v++) lw[v] = q[0][v];
+#pragma unroll
+    for (int stage = 0; stage < 2; stage++) {
+      if (valid) {
+        R F[D][V];
+        pde_flux<D, R>(P, lw, F);
+#pragma unroll
+        for (int d = 0; d < D; d++)
+#pragma unroll
+          for (int v = 0; v < V; v++) slab[(s * D + d) * V + v] = F[d][v];
+      }
+      __syncthreads();
+      R K[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) K[v] = R(0);
+      if (valid) {
+#pragma unroll
+        for (int d = 0; d < D; d++)
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            R a = R(0);
+#pragma unroll
+            for (int n = 0; n < B; n++)
+              a += slab[((base[d] + n * stride_of<D, B>(d)) * D + d) * V + v] *
+                   (static_cast<R>(invDx) * tabD<R>(T, cc[d] * B + n));
+            K[v] += a;
+          }
+      }
+      __syncthreads();  // slab rewritten by the next stage / the Picard loop
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        if (stage == 0) {
+          K1[v] = K[v];
+          lw[v] = static_cast<R>(u[v]) - static_cast<R>(dt) * K1[v];
+        } else {
+          K2[v] = K[v];
+        }
+      }
+    }
+    const R rdt = static_cast<R>(dt);
+#pragma unroll
+    for (int t = 0; t < B; t++) {
+      const R xt = static_cast<R>(T->x[t]);
+#pragma unroll
+      for (int v = 0; v < V; v++)
+        q[t][v] = static_cast<R>(u[v]) - (rdt * xt * K1[v]) - (R(0.5) * rdt * xt * xt * (K2[v] - K1[v]));
+    }
   }
 
   R fh[D][V];  // time-averaged flux of the last executed iteration
@@ -262,7 +326,8 @@ __device__ __forceinline__ void predictor_tile(
       const double tol = 1e-7;
       if (active && res < tol * tol) {
         active = false;
-        iters = (D == 3) ? iter : maxIter + 2;  // reference return conventions
+        // reference return conventions; optimised kernels: min(iter + 1, MaxIterations)
+        iters = dialect ? (iter + 1 < maxIter ? iter + 1 : maxIter) : ((D == 3) ? iter : maxIter + 2);
       }
       if (!__syncthreads_or(active)) break;
     } else {
@@ -381,26 +446,26 @@ __device__ __forceinline__ void predictor_tile(
 }
 
 // Predictor + volume integral, one workgroup per tile of CPB cells.
-template <int D, int B, int V, int NP, typename R>
+template <int D, int B, int V, int NP, typename R, bool CG>
 __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_kernel(
     long ncells, long ntiles, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
     double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
-    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd) {
-  predictor_tile<D, B, V, NP, R>(blockIdx.x, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard, T, P,
-                                 maxIter, adaptive, luh_upd);
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect) {
+  predictor_tile<D, B, V, NP, R, CG>(blockIdx.x, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard, T, P,
+                                 maxIter, adaptive, luh_upd, dialect);
 }
 
 // Persistent form: a grid smaller than ntiles loops over the tiles.
-template <int D, int B, int V, int NP, typename R>
+template <int D, int B, int V, int NP, typename R, bool CG>
 __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_persistent_kernel(
     long ncells, long ntiles, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
     double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
-    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd) {
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect) {
   for (long blk = blockIdx.x; blk < ntiles; blk += gridDim.x) {
-    predictor_tile<D, B, V, NP, R>(blk, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard, T, P,
-                                   maxIter, adaptive, luh_upd);
+    predictor_tile<D, B, V, NP, R, CG>(blk, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard, T, P,
+                                   maxIter, adaptive, luh_upd, dialect);
     __syncthreads();  // LDS of this tile fully consumed before the next one
   }
 }
@@ -923,25 +988,38 @@ struct Launch {
                        const double* dt, double* lduh, double* qb, double* fb, int* pic, const LaunchOpts& o) {
     const long tiles = (n + S::CPB - 1) / S::CPB;
     const long grid = grid_of(tiles, o);
-    const int adaptive = k.picard_max <= 0;
-    const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
-    const bool persistent = grid < tiles;
-    if (k.precision == 0) {
-      if (persistent)
-        hipLaunchKernelGGL((predictor_persistent_kernel<D, B, V, NP, double>), dim3(grid), dim3(S::THREADS), 0, st,
-                           n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update);
-      else
-        hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, double>), dim3(grid), dim3(S::THREADS), 0, st, n, tiles,
-                           luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update);
-    } else {
-      if (persistent)
-        hipLaunchKernelGGL((predictor_persistent_kernel<D, B, V, NP, float>), dim3(grid), dim3(S::THREADS), 0, st,
-                           n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update);
-      else
-        hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, float>), dim3(grid), dim3(S::THREADS), 0, st, n, tiles,
-                           luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update);
-    }
+    // Picard rule: generic (adaptive or fixed count) or the optimised kernels'
+    // (fusedSPTVI_nonlinear_cpp.template:177, tolerance exit always on)
+    const int adaptive = k.dialect ? 1 : k.picard_max <= 0;
+    const int maxIter = k.dialect == 2 ? (B - 3 <= 1 ? 1 : B - 3)
+                        : k.dialect == 1 ? 2 * B + 1
+                        : (adaptive ? 2 * (k.order + 1) : k.picard_max);
+    const int dl = k.dialect;
+    // the CERK form is compiled without the persistent variant (one workgroup per tile)
+    const bool persistent = grid < tiles && dl != 2;
+    if (k.precision == 0)
+      launch_predictor_form<double>(st, dl == 2, persistent, grid, tiles, n, luh, h, dt, lduh, qb, fb, pic, k,
+                                    maxIter, adaptive, o, dl);
+    else
+      launch_predictor_form<float>(st, dl == 2, persistent, grid, tiles, n, luh, h, dt, lduh, qb, fb, pic, k,
+                                   maxIter, adaptive, o, dl);
     return static_cast<int>(hipGetLastError());
+  }
+  template <typename R>
+  static void launch_predictor_form(hipStream_t st, bool cerk, bool persistent, long grid, long tiles, long n,
+                                    const double* luh, const double* h, const double* dt, double* lduh, double* qb,
+                                    double* fb, int* pic, const KernelCfg& k, int maxIter, int adaptive,
+                                    const LaunchOpts& o, int dl) {
+    if (cerk)
+      hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, R, true>), dim3(tiles), dim3(S::THREADS), 0, st, n, tiles,
+                         luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update, dl);
+    else if (persistent)
+      hipLaunchKernelGGL((predictor_persistent_kernel<D, B, V, NP, R, false>), dim3(grid), dim3(S::THREADS), 0, st,
+                         n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive,
+                         o.luh_update, dl);
+    else
+      hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, R, false>), dim3(tiles), dim3(S::THREADS), 0, st, n, tiles,
+                         luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update, dl);
   }
   static int riemann(const KernelCfg& k, hipStream_t st, long nf, const int* faces, const double* qb,
                      double* fb, const LaunchOpts& o) {
@@ -1053,7 +1131,7 @@ int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* lu
   if (!e) return -1;
   if (n <= 0) return 0;
   // auto: FP32 -> line kernel on the matrix cores; FP64 -> node-column kernel
-  if (lines_supported(k)) {
+  if (lines_supported(k) && k.dialect == 0) {  // the optimised dialect runs on the node-column kernel
     const int v = k.pred_variant;
     const bool fp32 = k.precision != 0;
     if ((v == 0 && fp32) || v == 3)

@@ -39,6 +39,8 @@ struct exa_ctx {
   double src_x[3] = {0, 0, 0}, src_amp = 0, src_f0 = 0, src_t0 = 0;
   // boundaryValues hook of exa_boundaryConditions (the Euler example's default)
   int bc_kind = EXA_BC_DIRICHLET_ENTROPY_WAVE;
+  // vector width (doubles) of the optimised dialect's padded arrays
+  int vect = 1;
 };
 
 namespace {
@@ -84,6 +86,17 @@ int nface(const exa_ctx* c) {
 int ndata(const exa_ctx* c) { return c->k.nvar + c->k.npar; }
 
 bool is_linear(const exa_ctx* c) { return c->k.pde.id == EXA_PDE_ACOUSTIC; }
+
+// getSizeWithPadding (KernelGenerator/kernelgenerator/controller.py:190-192):
+// the next multiple of the vector width
+int padded(int n, int vect) { return vect * ((n + vect - 1) / vect); }
+
+// One converter of converter_cpp.template: nblk blocks of nrow rows, each row
+// holding n values; the generic rows are n apart, the optimised rows npad.
+void convert_rows(const double* in, double* out, long rows, int n, int ld_in, int ld_out) {
+  for (long r = 0; r < rows; r++)
+    for (int v = 0; v < n; v++) out[r * ld_out + v] = in[r * ld_in + v];
+}
 
 double lagrange(const double* x, int B, int k, double s) {
   double r = 1.0;
@@ -289,6 +302,30 @@ int exa_ctx_set_boundary_values(exa_ctx* c, int kind) {
   return EXA_OK;
 }
 
+int exa_ctx_set_dialect(exa_ctx* c, int dialect, int vect_size) {
+  if (!c || dialect < EXA_DIALECT_GENERIC || dialect > EXA_DIALECT_OPTIMISED_CERK ||
+      !(vect_size == 1 || vect_size == 2 || vect_size == 4 || vect_size == 8)) {
+    set_error("exa_ctx_set_dialect: invalid argument (dialect 0..2, vect_size 1, 2, 4 or 8)");
+    return EXA_ERR_ARG;
+  }
+  if (is_linear(c) && dialect != EXA_DIALECT_GENERIC) {
+    set_error("exa_ctx_set_dialect: the optimised dialect covers the nonlinear kernels");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  c->k.dialect = dialect;
+  c->vect = vect_size;
+  return EXA_OK;
+}
+
+int exa_ctx_padded_size(const exa_ctx* c, int which, int* size) {
+  if (!c || !size || (which != 0 && which != 1)) {
+    set_error("exa_ctx_padded_size: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  *size = padded(which ? ndata(c) : c->k.nvar, c->vect);
+  return EXA_OK;
+}
+
 int exa_ctx_set_predictor_variant(exa_ctx* c, int variant) {
   if (!c || variant < 0 || variant > 4) {
     set_error("exa_ctx_set_predictor_variant: invalid argument");
@@ -447,6 +484,66 @@ int exa_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* c, double* lduh, double* 
   if (picard_iterations) *picard_iterations = pic;
   return EXA_OK;
 }
+
+int exa_opt_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* c, double* lduh, double* lQhbnd, double* lFhbnd,
+                                                  const double* luh, double inverseDx, double dt,
+                                                  int* picard_iterations) {
+  if (!c || !lduh || !lQhbnd || !lFhbnd || !luh || !(inverseDx > 0)) {
+    set_error("exa_opt_fusedSpaceTimePredictorVolumeIntegral: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  if (c->k.dialect == EXA_DIALECT_GENERIC) {
+    set_error("exa_opt_fusedSpaceTimePredictorVolumeIntegral: context not in an optimised dialect");
+    return EXA_ERR_ARG;
+  }
+  const int D = c->k.dim, NS = nspace(c), NF = nface(c), V = c->k.nvar, ND = ndata(c);
+  const int VP = padded(V, c->vect), NDP = padded(ND, c->vect);
+  std::vector<double> u(luh, luh + size_t(NS) * ND), du(size_t(NS) * V), qb(size_t(2 * D) * NF * ND),
+      fb(size_t(2 * D) * NF * V);
+  const double dx[3] = {1.0 / inverseDx, 1.0 / inverseDx, 1.0 / inverseDx};
+  const int rc = exa_fusedSpaceTimePredictorVolumeIntegral(c, du.data(), qb.data(), nullptr, fb.data(), u.data(),
+                                                          nullptr, dx, 0.0, dt, 1, picard_iterations);
+  if (rc) return rc;
+  // the generated kernel zeroes lduh, lQhbnd and lFhbnd (padding included) first
+  std::fill(lduh, lduh + size_t(NS) * VP, 0.0);
+  std::fill(lQhbnd, lQhbnd + size_t(2 * D) * NF * NDP, 0.0);
+  std::fill(lFhbnd, lFhbnd + size_t(2 * D) * NF * VP, 0.0);
+  convert_rows(du.data(), lduh, NS, V, V, VP);
+  convert_rows(qb.data(), lQhbnd, long(2 * D) * NF, ND, ND, NDP);
+  convert_rows(fb.data(), lFhbnd, long(2 * D) * NF, V, V, VP);
+  return EXA_OK;
+}
+
+// converter_cpp.template: rows of the array (row count, values per row: nVar or nData)
+#define EXA_CONVERTER(NAME, ROWS, DATA)                                                          \
+  int exa_converter_##NAME##_optimised2generic(const exa_ctx* c, const double* opt, double* gen) { \
+    if (!c || !opt || !gen) {                                                                    \
+      set_error("exa_converter_" #NAME "_optimised2generic: NULL argument");                     \
+      return EXA_ERR_ARG;                                                                        \
+    }                                                                                            \
+    const int n = (DATA) ? ndata(c) : c->k.nvar;                                                 \
+    convert_rows(opt, gen, (ROWS), n, padded(n, c->vect), n);                                    \
+    return EXA_OK;                                                                               \
+  }                                                                                              \
+  int exa_converter_##NAME##_generic2optimised(const exa_ctx* c, const double* gen, double* opt) { \
+    if (!c || !opt || !gen) {                                                                    \
+      set_error("exa_converter_" #NAME "_generic2optimised: NULL argument");                     \
+      return EXA_ERR_ARG;                                                                        \
+    }                                                                                            \
+    const int n = (DATA) ? ndata(c) : c->k.nvar;                                                 \
+    convert_rows(gen, opt, (ROWS), n, n, padded(n, c->vect));                                    \
+    return EXA_OK;                                                                               \
+  }
+
+EXA_CONVERTER(Qhbnd, long(2 * c->k.dim) * nface(c), 1)
+EXA_CONVERTER(Fhbnd, long(2 * c->k.dim) * nface(c), 0)
+EXA_CONVERTER(QFace, long(nface(c)), 1)
+EXA_CONVERTER(FFace, long(nface(c)), 0)
+EXA_CONVERTER(Qhi, long(nspace(c)), 1)
+EXA_CONVERTER(Fhi, long(c->k.dim + 1) * nspace(c), 0)
+EXA_CONVERTER(PSi, long(c->k.order + 2) * nspace(c), 0)
+EXA_CONVERTER(Lduh, long(nspace(c)), 0)
+#undef EXA_CONVERTER
 
 int exa_riemannSolver(exa_ctx* c, double* FL, double* FR, const double* QL, const double* QR, double t, double dt,
                       const double* dx, int direction, int isBoundaryFace, int faceIndex) {

@@ -42,6 +42,9 @@ struct KernelCfg {
   const DevTables* tab;  // device pointer
   int pred_variant;      // 0 auto, 1 node-column, 2 line (VALU), 3 line (FP32 matrix cores),
                          // 4 tile (FP32, full-rate matrix cores)
+  int dialect;           // exa_dialect: 0 generic kernels; 1 optimised kernels' Picard rule
+                         // (MaxIterations 2B+1, count min(iter+1, Max)); 2 the same with the
+                         // CERK initial guess (MaxIterations max(B-3, 1)); 1, 2: node-column kernel
 };
 
 // Launch options of the batched kernels.

@@ -1088,7 +1088,7 @@ int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const d
 }
 
 bool predictor_f32_faces_supported(const KernelCfg& k) {
-  return lines_supported(k) && k.precision != 0 && k.pred_variant != 1;
+  return lines_supported(k) && k.precision != 0 && k.pred_variant != 1 && k.dialect == 0;
 }
 
 }  // namespace exa

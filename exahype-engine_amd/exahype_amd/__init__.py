@@ -21,6 +21,8 @@ HEADER_PATH = os.path.join(REPO_DIR, "include", "exahype_amd.h")
 
 PDE_EULER, PDE_ACOUSTIC, PDE_ADVECTION = 0, 1, 2
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+DIALECT_GENERIC, DIALECT_OPTIMISED, DIALECT_OPTIMISED_CERK = 0, 1, 2
+CONVERTERS = ("Qhbnd", "Fhbnd", "QFace", "FFace", "Qhi", "Fhi", "PSi", "Lduh")
 EXA_OK, EXA_ERR_ARG, EXA_ERR_HIP, EXA_ERR_UNSUPPORTED, EXA_ERR_NOMEM = 0, 1, 2, 3, 4
 
 if not os.path.exists(LIB_PATH):
@@ -54,6 +56,13 @@ _sig("exa_ctx_destroy", [_cp])
 _sig("exa_ctx_set_advection", [_cp, _dp])
 _sig("exa_ctx_set_point_source", [_cp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double])
 _sig("exa_ctx_set_predictor_variant", [_cp, ctypes.c_int])
+_sig("exa_ctx_set_dialect", [_cp, ctypes.c_int, ctypes.c_int])
+_sig("exa_ctx_padded_size", [_cp, ctypes.c_int, _ip])
+_sig("exa_opt_fusedSpaceTimePredictorVolumeIntegral",
+     [_cp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _ip])
+for _name in CONVERTERS:
+    _sig(f"exa_converter_{_name}_optimised2generic", [_cp, _dp, _dp])
+    _sig(f"exa_converter_{_name}_generic2optimised", [_cp, _dp, _dp])
 _sig("exa_tables", [ctypes.c_int] + [_dp] * 9)
 _sig("exa_ctx_tables", [_cp] + [_dp] * 8)
 _sig("exa_fusedSpaceTimePredictorVolumeIntegral",
@@ -226,6 +235,37 @@ class Context:
     def set_predictor_variant(self, variant):
         """0: fastest kernel for the shape (default), 1: generic node-column kernel."""
         _check(_lib.exa_ctx_set_predictor_variant(self.h, variant))
+
+    def set_dialect(self, dialect, vect_size=1):
+        """Generic (0) or optimised-kernel (1; 2 with the CERK guess) Picard rule and padding width."""
+        _check(_lib.exa_ctx_set_dialect(self.h, dialect, vect_size))
+
+    def padded_size(self, which):
+        """nVarPad (which = 0) or nDataPad (which = 1)."""
+        n = ctypes.c_int(0)
+        _check(_lib.exa_ctx_padded_size(self.h, which, ctypes.byref(n)))
+        return n.value
+
+    def opt_fusedSpaceTimePredictorVolumeIntegral(self, luh, inverseDx, dt):
+        """The optimised kernels' predictor + volume integral on padded layouts."""
+        D = self.dim
+        VP, NDP = self.padded_size(0), self.padded_size(1)
+        luh = np.ascontiguousarray(luh, dtype=np.float64)
+        lduh = np.full(self.nspace * VP, np.nan)
+        lQhbnd = np.full(2 * D * self.nface * NDP, np.nan)
+        lFhbnd = np.full(2 * D * self.nface * VP, np.nan)
+        pic = ctypes.c_int(0)
+        _check(_lib.exa_opt_fusedSpaceTimePredictorVolumeIntegral(self.h, _p(lduh), _p(lQhbnd), _p(lFhbnd), _p(luh),
+                                                                   inverseDx, dt, ctypes.byref(pic)))
+        return dict(iters=pic.value, lduh=lduh, lQhbnd=lQhbnd, lFhbnd=lFhbnd)
+
+    def convert(self, name, direction, src, out_size):
+        """exa_converter_<name>_<direction> (direction "optimised2generic" or
+        "generic2optimised"); the output starts as NaN so untouched entries show."""
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        out = np.full(out_size, np.nan)
+        _check(getattr(_lib, f"exa_converter_{name}_{direction}")(self.h, _p(src), _p(out)))
+        return out
 
     def set_point_source(self, x, amp, f0, t0):
         xs = None if x is None else np.ascontiguousarray(x, dtype=np.float64)

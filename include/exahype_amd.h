@@ -93,6 +93,32 @@ int exa_ctx_set_advection(exa_ctx* ctx, const double* a);
  * steps (FP32).  Meshes copy the context's settings when they are created. */
 int exa_ctx_set_predictor_variant(exa_ctx* ctx, int variant);
 
+/* Kernel dialect.  The reference generates a second, "optimised" set of
+ * kernels (KernelGenerator/kernelgenerator/templates/aderdg/; selected by
+ * optimisation=optimised in a specification file) whose nonlinear predictor
+ * differs from the generic one in its Picard rule
+ * (fusedSPTVI_nonlinear_cpp.template:177,307-309,607) and, optionally, its
+ * initial guess (CERK, :121-173), and whose arrays pad the variable dimension
+ * to a multiple of the vector width (controller.py:166-195 getSizeWithPadding).
+ *   EXA_DIALECT_GENERIC        the generic kernels (default)
+ *   EXA_DIALECT_OPTIMISED      MaxIterations = 2 nDof + 1, tolerance exit
+ *                              always on, count = min(iter + 1, MaxIterations)
+ *   EXA_DIALECT_OPTIMISED_CERK the same with the Continuous Extension
+ *                              Runge-Kutta initial guess and MaxIterations =
+ *                              max(nDof - 3, 1)
+ * vect_size (1, 2, 4 or 8 doubles: the generator's noarch, wsm, snb/hsw,
+ * knl/skx widths) sets nVarPad and nDataPad of the exa_opt_* and
+ * exa_converter_* entry points.  The optimised dialects run on the node-column
+ * predictor kernel.  Meshes copy the context's settings when created. */
+enum exa_dialect {
+  EXA_DIALECT_GENERIC = 0,
+  EXA_DIALECT_OPTIMISED = 1,
+  EXA_DIALECT_OPTIMISED_CERK = 2
+};
+int exa_ctx_set_dialect(exa_ctx* ctx, int dialect, int vect_size);
+/* *size = nVarPad (which = 0) or nDataPad (which = 1) of the context's dialect. */
+int exa_ctx_padded_size(const exa_ctx* ctx, int which, int* size);
+
 /* Operator tables of order N derived on the host (no device needed; each
  * output may be NULL).  Same layout as exa_ctx_tables plus
  * egp[B*B] = equidistantGridProjector. */
@@ -127,6 +153,45 @@ int exa_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* ctx, double* lduh, double
                                               double* lFhbnd, double* luh, const double* centre,
                                               const double* dx, double t, double dt,
                                               int addVolumeIntegralResultToUpdate, int* picard_iterations);
+
+/* fusedSpaceTimePredictorVolumeIntegral of the optimised kernels
+ * (KernelGenerator/kernelgenerator/templates/aderdg/fusedSPTVI_nonlinear_cpp.template:27-47,
+ * nonlinear, no source/NCP/viscous flux), context in an EXA_DIALECT_OPTIMISED*
+ * dialect.  Padded layouts: lduh[B^D][nVarPad], lQhbnd[2D][B^(D-1)][nDataPad],
+ * lFhbnd[2D][B^(D-1)][nVarPad] (padding entries written as 0, as the kernel
+ * zeroes its outputs first); luh[B^D][nData] unpadded.  Cubic cells
+ * (inverseDx = 1/dx for every direction).  The scratch arrays the generated
+ * kernel takes (lQi, rhs, lFi, ...) live on the device and are not arguments.
+ * *picard_iterations = min(iter + 1, MaxIterations) (:607). */
+int exa_opt_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* ctx, double* lduh, double* lQhbnd, double* lFhbnd,
+                                                  const double* luh, double inverseDx, double dt,
+                                                  int* picard_iterations);
+
+/* Layout converters between the generic kernels' arrays and the optimised
+ * kernels' padded arrays (KernelGenerator/kernelgenerator/templates/aderdg/
+ * converter_cpp.template:16-173, same names; sizes from the context's shape
+ * and exa_ctx_set_dialect's vect_size).  Padding entries of an optimised
+ * output are left untouched, as in the reference.
+ *   Qhbnd [2D][B^(D-1)][nData|nDataPad]   Fhbnd [2D][B^(D-1)][nVar|nVarPad]
+ *   QFace [B^(D-1)][nData|nDataPad]       FFace [B^(D-1)][nVar|nVarPad]
+ *   Qhi   [B^D][nData|nDataPad]           Fhi   [D+1][B^D][nVar|nVarPad]
+ *   PSi   [B+1][B^D][nVar|nVarPad]        Lduh  [B^D][nVar|nVarPad] */
+int exa_converter_Qhbnd_optimised2generic(const exa_ctx* ctx, const double* opt, double* gen);
+int exa_converter_Qhbnd_generic2optimised(const exa_ctx* ctx, const double* gen, double* opt);
+int exa_converter_Fhbnd_optimised2generic(const exa_ctx* ctx, const double* opt, double* gen);
+int exa_converter_Fhbnd_generic2optimised(const exa_ctx* ctx, const double* gen, double* opt);
+int exa_converter_QFace_optimised2generic(const exa_ctx* ctx, const double* opt, double* gen);
+int exa_converter_QFace_generic2optimised(const exa_ctx* ctx, const double* gen, double* opt);
+int exa_converter_FFace_optimised2generic(const exa_ctx* ctx, const double* opt, double* gen);
+int exa_converter_FFace_generic2optimised(const exa_ctx* ctx, const double* gen, double* opt);
+int exa_converter_Qhi_optimised2generic(const exa_ctx* ctx, const double* opt, double* gen);
+int exa_converter_Qhi_generic2optimised(const exa_ctx* ctx, const double* gen, double* opt);
+int exa_converter_Fhi_optimised2generic(const exa_ctx* ctx, const double* opt, double* gen);
+int exa_converter_Fhi_generic2optimised(const exa_ctx* ctx, const double* gen, double* opt);
+int exa_converter_PSi_optimised2generic(const exa_ctx* ctx, const double* opt, double* gen);
+int exa_converter_PSi_generic2optimised(const exa_ctx* ctx, const double* gen, double* opt);
+int exa_converter_Lduh_optimised2generic(const exa_ctx* ctx, const double* opt, double* gen);
+int exa_converter_Lduh_generic2optimised(const exa_ctx* ctx, const double* gen, double* opt);
 
 /* riemannSolver (…Implementation.template:287-305 -> generic/c/3d/riemannSolverNonlinear.cpph:37-128).
  * FL/FR are overwritten in place with the Rusanov flux. */

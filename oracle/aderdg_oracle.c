@@ -82,6 +82,75 @@ void orc_eigenvalues(const orc_solver* s, const double* Q, int d, double* lambda
 }
 
 /* ------------------------------------------------------------------------ */
+/* CERK initial guess of the optimised kernels                                 */
+/* (KernelGenerator/kernelgenerator/templates/aderdg/                          */
+/*  fusedSPTVI_nonlinear_cpp.template:121-173 and                              */
+/*  templates/subtemplates/RK_loop.template:12-40):                            */
+/*   K1 = sum_d dudx_d F_d(luh) / dx_d   (flux derivative in x, then y, z)     */
+/*   lwh = luh - dt K1 (parameters copied),  K2 = the same of lwh,             */
+/*   lQi(t) = luh - dt xi_t K1 - 1/2 dt xi_t^2 (K2 - K1)  (parameters copied). */
+/* The template stores dudx_T scaled by 1/dx and accumulates the three         */
+/* directions into K with beta = 1; lQi is written in this oracle's generic    */
+/* layout [space][t][nData].                                                   */
+/* ------------------------------------------------------------------------ */
+static void flux_divergence(const orc_solver* s, const double* u, int ldu, const double* invDx,
+                            double* K /* [space][V] */) {
+  const int B = s->b.order + 1, V = s->nvar, D = s->dim;
+  const int NS = (D == 3) ? B * B * B : B * B;
+  double* F = (double*)malloc(sizeof(double) * NS * D * V);
+  for (int i = 0; i < NS; i++) orc_flux(s, u + i * ldu, F + i * D * V);
+  for (int i = 0; i < NS * V; i++) K[i] = 0.0;
+  for (int d = 0; d < D; d++) {
+    const int stride = (d == 0) ? 1 : (d == 1 ? B : B * B);
+    for (int i = 0; i < NS; i++) {
+      const int c = (i / stride) % B;       /* coordinate of node i along d */
+      const int base = i - c * stride;      /* node of the line at coordinate 0 */
+      for (int v = 0; v < V; v++) {
+        double a = 0.0;
+        for (int n = 0; n < B; n++)
+          a += F[(base + n * stride) * D * V + d * V + v] * (invDx[d] * s->b.dudx[c * B + n]);
+        K[i * V + v] += a;
+      }
+    }
+  }
+  free(F);
+}
+
+static void cerk_guess(const orc_solver* s, const double* luh, const double* invDx, double dt,
+                       double* lQi) {
+  const int B = s->b.order + 1, V = s->nvar, ND = V + s->npar, D = s->dim;
+  const int NS = (D == 3) ? B * B * B : B * B;
+  double* K1 = (double*)malloc(sizeof(double) * NS * V);
+  double* K2 = (double*)malloc(sizeof(double) * NS * V);
+  double* lwh = (double*)malloc(sizeof(double) * NS * ND);
+  flux_divergence(s, luh, ND, invDx, K1);
+  for (int i = 0; i < NS; i++) {
+    for (int v = 0; v < V; v++) lwh[i * ND + v] = luh[i * ND + v] - dt * K1[i * V + v];
+    for (int v = V; v < ND; v++) lwh[i * ND + v] = luh[i * ND + v];
+  }
+  flux_divergence(s, lwh, ND, invDx, K2);
+  const double* xi = s->b.x;
+  for (int i = 0; i < NS; i++)
+    for (int t = 0; t < B; t++) {
+      double* q = lQi + (i * B + t) * ND;
+      for (int v = 0; v < V; v++)
+        q[v] = luh[i * ND + v] - (dt * xi[t] * K1[i * V + v]) -
+               (0.5 * dt * xi[t] * xi[t] * (K2[i * V + v] - K1[i * V + v]));
+      for (int v = V; v < ND; v++) q[v] = luh[i * ND + v];
+    }
+  free(K1);
+  free(K2);
+  free(lwh);
+}
+
+/* MaxIterations of the Picard loop: generic (gc3 :85-86) or optimised rule. */
+static int picard_max_iterations(const orc_solver* s) {
+  const int nDof = s->b.order + 1;
+  if (s->opt_iterations) return s->cerk_guess ? (nDof - 3 <= 1 ? 1 : nDof - 3) : 2 * nDof + 1;
+  return s->use_max_picard ? s->max_picard : 2 * (s->b.order + 1);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Space-time predictor, 3D: gc3/spaceTimePredictorNonlinear.cpph             */
 /* ------------------------------------------------------------------------ */
 
@@ -96,14 +165,18 @@ static int picard3(const orc_solver* s, const double* luh, const double* invDx, 
 #define LQI(z, y, x, t, v) lQi[((((z) * B + (y)) * B + (x)) * B + (t)) * ND + (v)]
 #define LFI(t, z, y, x, d, v) lFi[(((((t) * B + (z)) * B + (y)) * B + (x)) * (D + 1) + (d)) * V + (v)]
 #define RHS(t, z, y, x, v) rhs[((((t) * B + (z)) * B + (y)) * B + (x)) * V + (v)]
-  /* 1. trivial initial guess (:71-82) */
-  for (int i = 0; i < B; i++)
-    for (int j = 0; j < B; j++)
-      for (int k = 0; k < B; k++)
-        for (int l = 0; l < B; l++)
-          for (int m = 0; m < ND; m++) LQI(i, j, k, l, m) = LUH(i, j, k, m);
+  /* 1. trivial initial guess (:71-82), or the optimised kernels' CERK guess */
+  if (s->cerk_guess) {
+    cerk_guess(s, luh, invDx, dt, lQi);
+  } else {
+    for (int i = 0; i < B; i++)
+      for (int j = 0; j < B; j++)
+        for (int k = 0; k < B; k++)
+          for (int l = 0; l < B; l++)
+            for (int m = 0; m < ND; m++) LQI(i, j, k, l, m) = LUH(i, j, k, m);
+  }
   /* 2. Picard iterations (:84-432) */
-  const int maxIter = s->use_max_picard ? s->max_picard : 2 * (s->b.order + 1);
+  const int maxIter = picard_max_iterations(s);
   int iter = 0;
   double Fnode[3 * 64];
   for (; iter < maxIter; iter++) {
@@ -172,11 +245,12 @@ static int picard3(const orc_solver* s, const double* luh, const double* invDx, 
             }
         }
     /* 4. exit condition (:407-423) */
-    if (!s->use_max_picard) {
+    if (!s->use_max_picard || s->opt_iterations) {
       const double tol = 1e-7;
       if (sq_res < tol * tol) break;
     }
   }
+  if (s->opt_iterations) return iter + 1 < maxIter ? iter + 1 : maxIter;  /* template :607 */
   return iter;
 #undef LUH
 #undef LQI
@@ -281,12 +355,16 @@ static int picard2(const orc_solver* s, const double* luh, const double* invDx, 
 #define LQI(y, x, t, v) lQi[(((y) * B + (x)) * B + (t)) * ND + (v)]
 #define LFI(t, y, x, d, v) lFi[((((t) * B + (y)) * B + (x)) * (D + 1) + (d)) * V + (v)]
 #define RHS(t, y, x, v) rhs[((((t) * B + (y)) * B + (x))) * V + (v)]
-  /* initial guess (:61-69) */
-  for (int j = 0; j < B; j++)
-    for (int k = 0; k < B; k++)
-      for (int l = 0; l < B; l++)
-        for (int m = 0; m < ND; m++) LQI(j, k, l, m) = LUH(j, k, m);
-  const int maxIter = s->use_max_picard ? s->max_picard : 2 * (s->b.order + 1);
+  /* initial guess (:61-69), or the optimised kernels' CERK guess */
+  if (s->cerk_guess) {
+    cerk_guess(s, luh, invDx, dt, lQi);
+  } else {
+    for (int j = 0; j < B; j++)
+      for (int k = 0; k < B; k++)
+        for (int l = 0; l < B; l++)
+          for (int m = 0; m < ND; m++) LQI(j, k, l, m) = LUH(j, k, m);
+  }
+  const int maxIter = picard_max_iterations(s);
   int iter = 0;
   double Fnode[2 * 64];
   for (; iter < maxIter; iter++) {
@@ -339,12 +417,16 @@ static int picard2(const orc_solver* s, const double* luh, const double* invDx, 
             LQI(j, k, l, m) = lQi_new;
           }
       }
-    /* exit condition: the 2D kernel forces iter = MaxIterations + 1 (:296-306) */
-    if (!s->use_max_picard) {
+    /* exit condition: the 2D kernel forces iter = MaxIterations + 1 (:296-306);
+     * the optimised kernel breaks (template :307-309) */
+    if (s->opt_iterations) {
+      if (sq_res < 1e-7 * 1e-7) break;
+    } else if (!s->use_max_picard) {
       const double tol = 1e-7;
       if (sq_res < tol * tol) iter = maxIter + 1;
     }
   }
+  if (s->opt_iterations) return iter + 1 < maxIter ? iter + 1 : maxIter;  /* template :607 */
   return iter;
 #undef LUH
 #undef LQI

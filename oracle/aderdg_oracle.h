@@ -80,6 +80,13 @@ typedef struct orc_solver {
   double pnpm;        /* PNPM */
   double adv[3];      /* advection velocity (ORC_PDE_ADVECTION only) */
   orc_basis b;
+  /* optimised-kernel dialect (KernelGenerator/kernelgenerator/templates/aderdg/
+   * fusedSPTVI_nonlinear_cpp.template): cerk_guess = Continuous Extension
+   * Runge-Kutta initial guess (:121-173); opt_iterations = that kernel's
+   * MaxIterations (:177: max(nDof-3, 1) with CERK, else 2 nDof + 1), tolerance
+   * exit always on (:307-309), return min(iter + 1, MaxIterations) (:607). */
+  int cerk_guess;
+  int opt_iterations;
 } orc_solver;
 
 /* ---- PDE hooks ---------------------------------------------------------- */
