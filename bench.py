@@ -56,7 +56,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def predictor_flops_per_cell(dim, B, V, ND, iters):
+def predictor_flops_per_cell(dim, B, V, ND, iters, cerk=False):
     """Algorithmic contraction flops of one predictor+volume-integral call.
 
     Per Picard iteration: D stiffness contractions 2*V*B^(D+2) each, the iK1
@@ -64,9 +64,13 @@ def predictor_flops_per_cell(dim, B, V, ND, iters):
     averages of lQi and the D fluxes 2*B^(D+1)*(ND + D*V), extrapolation to
     2D faces 2*2D*B^(D-1)*B*(ND + V), volume integral D*2*V*B^(D+1).
     Flux evaluations are not counted (the reference's counting convention,
-    Miscellaneous/KernelsPerfMeasurementsTests/report/report.tex:395-408)."""
+    Miscellaneous/KernelsPerfMeasurementsTests/report/report.tex:395-408).
+    CERK initial guess: two flux divergences 2 * D*2*V*B^(D+1) (derivative
+    contractions at every node) and the guess polynomial ~6*V*B^(D+1)."""
     per_iter = dim * 2 * V * B ** (dim + 2) + 2 * V * B ** (dim + 2) + 5 * V * B ** (dim + 1)
     once = 2 * B ** (dim + 1) * (ND + dim * V) + 2 * 2 * dim * B ** (dim - 1) * B * (ND + V) + dim * 2 * V * B ** (dim + 1)
+    if cerk:
+        once += 2 * dim * 2 * V * B ** (dim + 1) + 6 * V * B ** (dim + 1)
     return per_iter * iters + once
 
 
@@ -79,10 +83,11 @@ def executed_iterations(counts, dim, max_iter, adaptive):
     return np.where(counts == max_iter + 2, counts, max_iter)  # 2D convention loses the index
 
 
-def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, variant=0):
+def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, variant=0, guess=0):
     dim, V = 3, 5
     ctx = E.Context(dim, order, V, 0, precision=precision, picard_max=picard)
     ctx.set_predictor_variant(variant)
+    ctx.set_initial_guess(guess)
     mesh = E.Mesh(ctx, n, 1.0 / n)
     prm = P.density_wave_params(dim, seed, modes)
     # warm-up steps on the same state, then restart from the initial condition
@@ -110,7 +115,7 @@ def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, vari
     adaptive = picard <= 0
     max_iter = 2 * (order + 1) if adaptive else picard
     iters = executed_iterations(pic, dim, max_iter, adaptive)
-    flops = float(np.sum(predictor_flops_per_cell(dim, B, V, V, iters)))
+    flops = float(np.sum(predictor_flops_per_cell(dim, B, V, V, iters, cerk=bool(guess))))
     pred_ms = ph["predictor_ms"] / steps
     kind = "fp64" if precision == E.PREC_FP64 else "fp32"
     achieved = flops / (pred_ms * 1e-3) / 1e12
@@ -345,6 +350,8 @@ def main():
     ap.add_argument("--picard-fp32", type=int, default=0,
                     help="fixed Picard count of the FP32 form (0: the largest count the FP64 form executed)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--guess", choices=("trivial", "cerk"), default="cerk",
+                    help="initial guess of the FP32 form's Picard loop (with --sweep-variants: of every form)")
     ap.add_argument("--acoustic-predictor", type=int, default=0, choices=(0, 1),
                     help="acoustic forward predictor: 0 x-line kernel, 1 node kernel")
     ap.add_argument("--modes", type=int, default=8)
@@ -407,25 +414,50 @@ def main():
             for variant, vname in ((1, "node-column"), (2, "line"), (3, "line+mfma"), (4, "tile+mfma")):
                 if (prec == E.PREC_FP64 and variant >= 3) or (args.variant is not None and variant != args.variant):
                     continue
-                r = run_form(E, P, args.n, args.order, args.steps, args.warmup, prec, pic, args.seed, args.modes, variant)
+                r = run_form(E, P, args.n, args.order, args.steps, args.warmup, prec, pic, args.seed, args.modes, variant,
+                             guess=1 if args.guess == "cerk" else 0)
                 log(f"[sweep] {name} {vname:12s} predictor {r['predictor_ms']:.2f} ms/step "
-                    f"({r['roofline']['achieved']:.2f} TFLOP/s)")
+                    f"({r['roofline']['achieved']:.2f} TFLOP/s), picard {r['picard_hist']}")
         return
     r64 = run_form(E, P, args.n, args.order, args.steps, args.warmup, E.PREC_FP64, 0, args.seed, args.modes)
     log(f"[bench] fp64: {r64['value']:.4e} DoF/s, {r64['ms_per_step']:.2f} ms/step "
         f"(pred {r64['predictor_ms']:.2f} riem {r64['riemann_ms']:.2f} corr {r64['corrector_ms']:.2f}), "
         f"picard {r64['picard_hist']}, {r64['roofline']['achieved']:.2f} TFLOP/s")
-    k32 = args.picard_fp32 if args.picard_fp32 > 0 else r64["picard_max_executed"]
-    r32 = run_form(E, P, args.n, args.order, args.steps, args.warmup, E.PREC_FP32, k32, args.seed, args.modes)
+    cerk = args.guess == "cerk"
+    r64c = None
+    if cerk:
+        # the FP64 form with the FP32 form's initial guess (CERK) and the generic adaptive rule:
+        # its largest executed Picard count is the FP32 form's fixed count
+        r64c = run_form(E, P, args.n, args.order, args.steps, args.warmup, E.PREC_FP64, 0, args.seed, args.modes,
+                        guess=1)
+        log(f"[bench] fp64 (CERK guess): {r64c['value']:.4e} DoF/s, {r64c['ms_per_step']:.2f} ms/step "
+            f"(pred {r64c['predictor_ms']:.2f} corr {r64c['corrector_ms']:.2f}), picard {r64c['picard_hist']}")
+    kref = r64c if cerk else r64
+    k32 = args.picard_fp32 if args.picard_fp32 > 0 else kref["picard_max_executed"]
+    r32 = run_form(E, P, args.n, args.order, args.steps, args.warmup, E.PREC_FP32, k32, args.seed, args.modes,
+                   guess=1 if cerk else 0)
     log(f"[bench] fp32: {r32['value']:.4e} DoF/s, {r32['ms_per_step']:.2f} ms/step "
         f"(pred {r32['predictor_ms']:.2f} riem {r32['riemann_ms']:.2f} corr {r32['corrector_ms']:.2f}), "
-        f"picard {k32}, {r32['roofline']['achieved']:.2f} TFLOP/s")
+        f"guess {args.guess}, picard {k32}, {r32['roofline']['achieved']:.2f} TFLOP/s")
 
-    a = r32.pop("solution").reshape(-1, 5)
-    b = r64.pop("solution").reshape(-1, 5)
-    err = [float(np.linalg.norm(a[:, v] - b[:, v]) / np.linalg.norm(b[:, v])) for v in range(5)]
-    del a, b
+    def rel_err(x, y):
+        a = x.reshape(-1, 5)
+        b = y.reshape(-1, 5)
+        return [float(np.linalg.norm(a[:, v] - b[:, v]) / np.linalg.norm(b[:, v])) for v in range(5)]
+
+    # errors against the FP64 form of the reference's generic kernels (trivial guess, adaptive Picard)
+    err = rel_err(r32.pop("solution"), r64["solution"])
     log(f"[bench] rel L2 error fp32 vs fp64 per variable: {['%.2e' % e for e in err]}")
+    r32t = None
+    if cerk:
+        k32t = r64["picard_max_executed"]
+        r32t = run_form(E, P, args.n, args.order, args.steps, args.warmup, E.PREC_FP32, k32t, args.seed, args.modes)
+        r32t["rel_l2_err_vs_fp64"] = rel_err(r32t.pop("solution"), r64["solution"])
+        r32t["picard_fixed"] = k32t
+        log(f"[bench] fp32 (trivial guess): {r32t['value']:.4e} DoF/s, {r32t['ms_per_step']:.2f} ms/step "
+            f"(pred {r32t['predictor_ms']:.2f}), picard {k32t}, max err {max(r32t['rel_l2_err_vs_fp64']):.2e}")
+        r64c["rel_l2_err_vs_fp64"] = rel_err(r64c.pop("solution"), r64["solution"])
+    r64.pop("solution")
 
     cpu = None
     if not args.no_cpu:
@@ -455,8 +487,14 @@ def main():
             "workload": f"3D compressible Euler ADER-DG order {args.order} ({args.order + 1}^3 nodes), 5 vars, "
                         f"{args.n}^3 periodic cells, density wave + {args.modes} seeded Fourier modes "
                         f"(amp 1e-2, |k|<=4) in rho and u, CFL 0.9, {args.steps} global steps; "
-                        f"FP32 predictor (fixed {k32} Picard iterations), FP64 Riemann/corrector/update/CFL",
-            "cells": args.n ** 3, "order": args.order, "picard_fp32": k32, "seed": args.seed,
+                        + (f"FP32 predictor with the CERK initial guess of the reference's optimised kernels and "
+                           f"{k32} fixed Picard iteration(s) (the largest count the FP64 form with the same guess "
+                           f"executes under the adaptive 1e-7 rule)" if cerk else
+                           f"FP32 predictor (fixed {k32} Picard iterations)")
+                        + ", FP64 Riemann/corrector/update/CFL; errors vs the FP64 generic kernels (trivial guess, "
+                          "adaptive Picard)",
+            "cells": args.n ** 3, "order": args.order, "picard_fp32": k32, "guess_fp32": args.guess,
+            "seed": args.seed,
         },
         "roofline": r32["roofline"],
         "cpu_baseline": cpu,
@@ -470,6 +508,12 @@ def main():
         "rel_l2_err_fp32_vs_fp64": err,
         "max_rel_l2_err": max(err),
     }
+    if cerk:
+        keys = ("value", "ms_per_step", "predictor_ms", "corrector_ms", "picard_hist", "rel_l2_err_vs_fp64")
+        out["fp64_cerk_guess"] = {k: r64c[k] for k in keys}
+        out["fp32_trivial_guess"] = {k: r32t[k] for k in keys if k != "picard_hist"}
+        out["fp32_trivial_guess"]["picard_fixed"] = r32t["picard_fixed"]
+        out["speedup_fp32_vs_fp64_same_guess"] = r32["value"] / r64c["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -991,17 +991,17 @@ struct Launch {
     // Picard rule: generic (adaptive or fixed count) or the optimised kernels'
     // (fusedSPTVI_nonlinear_cpp.template:177, tolerance exit always on)
     const int adaptive = k.dialect ? 1 : k.picard_max <= 0;
-    const int maxIter = k.dialect == 2 ? (B - 3 <= 1 ? 1 : B - 3)
-                        : k.dialect == 1 ? 2 * B + 1
-                        : (adaptive ? 2 * (k.order + 1) : k.picard_max);
+    const int maxIter = k.dialect ? (k.guess ? (B - 3 <= 1 ? 1 : B - 3) : 2 * B + 1)
+                                  : (adaptive ? 2 * (k.order + 1) : k.picard_max);
     const int dl = k.dialect;
+    const bool cerk = k.guess != 0;
     // the CERK form is compiled without the persistent variant (one workgroup per tile)
-    const bool persistent = grid < tiles && dl != 2;
+    const bool persistent = grid < tiles && !cerk;
     if (k.precision == 0)
-      launch_predictor_form<double>(st, dl == 2, persistent, grid, tiles, n, luh, h, dt, lduh, qb, fb, pic, k,
+      launch_predictor_form<double>(st, cerk, persistent, grid, tiles, n, luh, h, dt, lduh, qb, fb, pic, k,
                                     maxIter, adaptive, o, dl);
     else
-      launch_predictor_form<float>(st, dl == 2, persistent, grid, tiles, n, luh, h, dt, lduh, qb, fb, pic, k,
+      launch_predictor_form<float>(st, cerk, persistent, grid, tiles, n, luh, h, dt, lduh, qb, fb, pic, k,
                                    maxIter, adaptive, o, dl);
     return static_cast<int>(hipGetLastError());
   }
@@ -1131,7 +1131,7 @@ int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* lu
   if (!e) return -1;
   if (n <= 0) return 0;
   // auto: FP32 -> line kernel on the matrix cores; FP64 -> node-column kernel
-  if (lines_supported(k) && k.dialect == 0) {  // the optimised dialect runs on the node-column kernel
+  if (line_kernels_serve(k)) {
     const int v = k.pred_variant;
     const bool fp32 = k.precision != 0;
     if ((v == 0 && fp32) || v == 3)

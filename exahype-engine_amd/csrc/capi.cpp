@@ -312,8 +312,22 @@ int exa_ctx_set_dialect(exa_ctx* c, int dialect, int vect_size) {
     set_error("exa_ctx_set_dialect: the optimised dialect covers the nonlinear kernels");
     return EXA_ERR_UNSUPPORTED;
   }
-  c->k.dialect = dialect;
+  c->k.dialect = dialect != EXA_DIALECT_GENERIC ? 1 : 0;
+  c->k.guess = dialect == EXA_DIALECT_OPTIMISED_CERK ? 1 : 0;
   c->vect = vect_size;
+  return EXA_OK;
+}
+
+int exa_ctx_set_initial_guess(exa_ctx* c, int guess) {
+  if (!c || (guess != EXA_GUESS_TRIVIAL && guess != EXA_GUESS_CERK)) {
+    set_error("exa_ctx_set_initial_guess: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  if (is_linear(c) && guess != EXA_GUESS_TRIVIAL) {
+    set_error("exa_ctx_set_initial_guess: the CERK guess is part of the nonlinear predictor");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  c->k.guess = guess;
   return EXA_OK;
 }
 
@@ -492,7 +506,7 @@ int exa_opt_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* c, double* lduh, doub
     set_error("exa_opt_fusedSpaceTimePredictorVolumeIntegral: invalid argument");
     return EXA_ERR_ARG;
   }
-  if (c->k.dialect == EXA_DIALECT_GENERIC) {
+  if (!c->k.dialect) {
     set_error("exa_opt_fusedSpaceTimePredictorVolumeIntegral: context not in an optimised dialect");
     return EXA_ERR_ARG;
   }

@@ -42,9 +42,11 @@ struct KernelCfg {
   const DevTables* tab;  // device pointer
   int pred_variant;      // 0 auto, 1 node-column, 2 line (VALU), 3 line (FP32 matrix cores),
                          // 4 tile (FP32, full-rate matrix cores)
-  int dialect;           // exa_dialect: 0 generic kernels; 1 optimised kernels' Picard rule
-                         // (MaxIterations 2B+1, count min(iter+1, Max)); 2 the same with the
-                         // CERK initial guess (MaxIterations max(B-3, 1)); 1, 2: node-column kernel
+  int dialect;           // 0 generic Picard rule; 1 the optimised kernels' rule (MaxIterations
+                         // 2B+1, or max(B-3, 1) with the CERK guess; tolerance exit always on;
+                         // count min(iter+1, Max)); runs on the node-column kernel
+  int guess;             // initial guess: 0 luh at every time node; 1 CERK (node-column kernel,
+                         // and the FP32 matrix-core line kernel with a fixed Picard count)
 };
 
 // Launch options of the batched kernels.
@@ -119,6 +121,9 @@ int launch_solution_update(const KernelCfg& k, hipStream_t s, double* luh, const
                            const double* lduh, const double* dt);
 // Line-decomposed 3D predictor (predictor3d_lines.hip) for the shapes it covers.
 bool lines_supported(const KernelCfg& k);
+// The 3D order-5 line/tile predictors serve k: generic Picard rule, and with the
+// CERK guess only the FP32 matrix-core line kernel with a fixed Picard count.
+bool line_kernels_serve(const KernelCfg& k);
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
                              const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard,
                              bool mfma, const int* cinst = nullptr, double* luh_update = nullptr,

@@ -483,7 +483,7 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
   if (l == 0 && picard) picard[cell] = iters;
 }
 
-template <int B, typename R, bool MF, bool FIX>
+template <int B, typename R, bool MF, bool FIX, bool CG>
 // Fixed-count form: four workgroups per CU (LDS 39 KB each; the register
 // budget of 128 costs a few spills, measured faster than three workgroups)
 __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor3d_lines_kernel(
@@ -500,6 +500,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
   static_assert(S::CPB == 1, "one cell per workgroup");
   static_assert(!MF || (sizeof(R) == 4 && B <= 8), "matrix-core form: FP32, B <= 8");
   static_assert(!FIX || MF, "fixed-count form: FP32 matrix-core kernel");
+  static_assert(!CG || FIX, "CERK guess: fixed-count form");
   if constexpr (FIX) adaptive = 0;  // launched only for a fixed Picard count
   constexpr int NWAVE = S::THREADS / 64;
   constexpr int TILE_BYTES = TILE * int(sizeof(R));
@@ -566,7 +567,81 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
   // FIX: time averages at this lane's time-line point, lQhi in registers, lFhi_d in af
   [[maybe_unused]] float qh[V];
   [[maybe_unused]] float* af = reinterpret_cast<float*>(buf + TILE_BYTES);
-  if constexpr (FIX) {
+  if constexpr (CG) {
+    // CERK initial guess (the reference's optimised kernels,
+    // KernelGenerator/kernelgenerator/templates/aderdg/fusedSPTVI_nonlinear_cpp.template:121-173):
+    // K1 = sum_d dudx_d F_d(luh) / h, lwh = luh - dt K1, K2 the same of lwh,
+    // lQi(t) = luh - dt xi_t K1 - dt xi_t^2 (K2 - K1) / 2.  Lane l works on its
+    // time-line point p = (z, y, x = t) (u = luh there); the stage fluxes go
+    // through the tile, which the Picard loop has not used yet.
+    const float rdt = static_cast<float>(dt), rih = static_cast<float>(1.0 / h);
+    float* fl = tl;  // [3][NS][V]
+    float K1[V], K2[V], lw[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) lw[v] = u[v];
+#pragma unroll
+    for (int stage = 0; stage < 2; stage++) {
+      if (lane_on) {
+        float ir, pr;
+        euler_prim<float>(lw, ir, pr);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          float F[V];
+          euler_flux_dir<float>(lw, ir, pr, d, F);
+#pragma unroll
+          for (int v = 0; v < V; v++) fl[d * NS * V + pv + v] = F[v];
+        }
+      }
+      __syncthreads();
+      float K[V] = {0.f, 0.f, 0.f, 0.f, 0.f};
+      if (lane_on) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const int c = d == 0 ? t : (d == 1 ? y : z);              // coordinate of p along d
+          const int stride = d == 0 ? V : (d == 1 ? B * V : B * B * V);
+          const float* line = fl + d * NS * V + pv - c * stride;
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            float a = 0.f;
+#pragma unroll
+            for (int n = 0; n < B; n++) a += line[n * stride + v] * (rih * T->dudxf[c * B + n]);
+            K[v] += a;
+          }
+        }
+      }
+      __syncthreads();  // fl is rewritten by the next stage / the guess
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        if (stage == 0) {
+          K1[v] = K[v];
+          lw[v] = u[v] - rdt * K1[v];
+        } else {
+          K2[v] = K[v];
+        }
+      }
+    }
+    float qg[B][V];
+#pragma unroll
+    for (int i = 0; i < B; i++) {
+      const float xt = static_cast<float>(T->x[i]);
+#pragma unroll
+      for (int v = 0; v < V; v++) qg[i][v] = u[v] - (rdt * xt * K1[v]) - (0.5f * rdt * xt * xt * (K2[v] - K1[v]));
+    }
+    if (lane_on) {
+#pragma unroll
+      for (int i = 0; i < B; i++)
+#pragma unroll
+        for (int v = 0; v < V; v++) tl[tbase + i * (B * B * LS) + v] = qg[i][v];
+      if (maxIter == 1) time_line_flux_averages<B>(qg, T, af, pv);  // fluxes of the guess are averaged
+    }
+    __syncthreads();
+    if (lane_on) {  // this lane's x-line of the guess; the tile's next writes go to own lines only
+#pragma unroll
+      for (int x = 0; x < B; x++)
+#pragma unroll
+        for (int v = 0; v < V; v++) q[x][v] = tl[own + x * V + v];
+    }
+  } else if constexpr (FIX) {
     if (maxIter == 1 && lane_on) {  // the last iteration evaluates fluxes of the initial guess
       float qi[B][V];
 #pragma unroll
@@ -1071,24 +1146,34 @@ int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const d
   const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
   const int f32 = (f32faces && k.precision != 0) ? 1 : 0;  // FP32 faces exist only in the FP32 forms
   if (k.precision == 0) {
-    hipLaunchKernelGGL((predictor3d_lines_kernel<B, double, false, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h, dt,
+    if (k.guess) return -1;  // the FP64 line kernel has no CERK guess (the node-column kernel has)
+    hipLaunchKernelGGL((predictor3d_lines_kernel<B, double, false, false, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h, dt,
                        cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, 0);
   } else {
-    if (mfma && !adaptive)
-      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true>), dim3(blocks), dim3(S::THREADS), 0, st, n,
-                         luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
-    else if (mfma)
-      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, false>), dim3(blocks), dim3(S::THREADS), 0, st, n,
-                         luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
-    else
-      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, false, false>), dim3(blocks), dim3(S::THREADS), 0, st,
+    if (k.guess && !(mfma && !adaptive)) return -1;  // CERK: fixed-count matrix-core form only
+    if (mfma && !adaptive && k.guess)
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true>), dim3(blocks), dim3(S::THREADS), 0, st,
                          n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+    else if (mfma && !adaptive)
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, false>), dim3(blocks), dim3(S::THREADS), 0,
+                         st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+    else if (mfma)
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, false, false>), dim3(blocks), dim3(S::THREADS), 0,
+                         st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+    else
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, false, false, false>), dim3(blocks), dim3(S::THREADS), 0,
+                         st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
   }
   return static_cast<int>(hipGetLastError());
 }
 
+bool line_kernels_serve(const KernelCfg& k) {
+  if (!lines_supported(k) || k.dialect != 0) return false;
+  return !k.guess || (k.precision != 0 && k.picard_max > 0 && (k.pred_variant == 0 || k.pred_variant == 3));
+}
+
 bool predictor_f32_faces_supported(const KernelCfg& k) {
-  return lines_supported(k) && k.precision != 0 && k.pred_variant != 1 && k.dialect == 0;
+  return line_kernels_serve(k) && k.precision != 0 && k.pred_variant != 1;
 }
 
 }  // namespace exa

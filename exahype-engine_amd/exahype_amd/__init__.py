@@ -22,6 +22,7 @@ HEADER_PATH = os.path.join(REPO_DIR, "include", "exahype_amd.h")
 PDE_EULER, PDE_ACOUSTIC, PDE_ADVECTION = 0, 1, 2
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 DIALECT_GENERIC, DIALECT_OPTIMISED, DIALECT_OPTIMISED_CERK = 0, 1, 2
+GUESS_TRIVIAL, GUESS_CERK = 0, 1
 CONVERTERS = ("Qhbnd", "Fhbnd", "QFace", "FFace", "Qhi", "Fhi", "PSi", "Lduh")
 EXA_OK, EXA_ERR_ARG, EXA_ERR_HIP, EXA_ERR_UNSUPPORTED, EXA_ERR_NOMEM = 0, 1, 2, 3, 4
 
@@ -57,6 +58,7 @@ _sig("exa_ctx_set_advection", [_cp, _dp])
 _sig("exa_ctx_set_point_source", [_cp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double])
 _sig("exa_ctx_set_predictor_variant", [_cp, ctypes.c_int])
 _sig("exa_ctx_set_dialect", [_cp, ctypes.c_int, ctypes.c_int])
+_sig("exa_ctx_set_initial_guess", [_cp, ctypes.c_int])
 _sig("exa_ctx_padded_size", [_cp, ctypes.c_int, _ip])
 _sig("exa_opt_fusedSpaceTimePredictorVolumeIntegral",
      [_cp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _ip])
@@ -239,6 +241,10 @@ class Context:
     def set_dialect(self, dialect, vect_size=1):
         """Generic (0) or optimised-kernel (1; 2 with the CERK guess) Picard rule and padding width."""
         _check(_lib.exa_ctx_set_dialect(self.h, dialect, vect_size))
+
+    def set_initial_guess(self, guess):
+        """GUESS_TRIVIAL (luh at every time node) or GUESS_CERK."""
+        _check(_lib.exa_ctx_set_initial_guess(self.h, guess))
 
     def padded_size(self, which):
         """nVarPad (which = 0) or nDataPad (which = 1)."""

@@ -116,6 +116,17 @@ enum exa_dialect {
   EXA_DIALECT_OPTIMISED_CERK = 2
 };
 int exa_ctx_set_dialect(exa_ctx* ctx, int dialect, int vect_size);
+
+/* Initial guess of the nonlinear Picard loop, independent of the Picard rule:
+ * EXA_GUESS_TRIVIAL (luh at every time node, the generic kernels,
+ * spaceTimePredictorNonlinear.cpph:71-82) or EXA_GUESS_CERK (the optimised
+ * kernels' Continuous Extension Runge-Kutta guess,
+ * fusedSPTVI_nonlinear_cpp.template:121-173).  exa_ctx_set_dialect sets it
+ * too (CERK for EXA_DIALECT_OPTIMISED_CERK); a later call overrides.  With the
+ * generic rule the CERK guess runs on the node-column kernel, and for 3D Euler
+ * order 5 in FP32 with a fixed Picard count on the x-line matrix-core kernel. */
+enum exa_guess { EXA_GUESS_TRIVIAL = 0, EXA_GUESS_CERK = 1 };
+int exa_ctx_set_initial_guess(exa_ctx* ctx, int guess);
 /* *size = nVarPad (which = 0) or nDataPad (which = 1) of the context's dialect. */
 int exa_ctx_padded_size(const exa_ctx* ctx, int which, int* size);
 

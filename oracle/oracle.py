@@ -163,14 +163,15 @@ class Oracle:
 
     def __init__(self, dim, order, nvar=5, npar=0, pde=PDE_EULER, cfl=0.9, pnpm=None,
                  max_picard=None, adv=(1.0, 0.5, 0.25), table_override=None, dialect=None):
-        """dialect: None (generic kernels) or "optimised" / "optimised-cerk" (the
+        """dialect: None (generic kernels), "cerk" (generic Picard rule with the
+        CERK initial guess) or "optimised" / "optimised-cerk" (the
         KernelGenerator kernels' Picard rule, trivial or CERK initial guess)."""
         self.dim, self.order, self.nvar, self.npar = dim, order, nvar, npar
         self.B = order + 1
         s = Solver()
         s.dim, s.nvar, s.npar, s.pde = dim, nvar, npar, pde
         s.opt_iterations = 1 if dialect in ("optimised", "optimised-cerk") else 0
-        s.cerk_guess = 1 if dialect == "optimised-cerk" else 0
+        s.cerk_guess = 1 if dialect in ("optimised-cerk", "cerk") else 0
         s.use_max_picard = 1 if max_picard is not None else 0
         s.max_picard = max_picard if max_picard is not None else order + 1
         s.cfl = cfl

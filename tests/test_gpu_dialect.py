@@ -177,3 +177,58 @@ def test_dialect_error_paths():
     with pytest.raises(E.ExaError):
         lin.set_dialect(E.DIALECT_OPTIMISED, 1)
     lin.close()
+
+
+@pytest.mark.parametrize("D,order,n,steps", [(3, 5, 3, 3), (2, 4, 9, 4)])
+def test_mesh_with_cerk_guess_and_generic_rule_vs_oracle(D, order, n, steps):
+    """FP64 node-column kernel: CERK initial guess with the generic adaptive rule."""
+    print(f"[cerk generic] D={D} order={order} n={n}", flush=True)
+    luh0 = P.density_wave_3d(n, order) if D == 3 else P.density_wave_2d(n, order)
+    ref = luh0.copy()
+    orc = O.Oracle(D, order, dialect="cerk")
+    dt_ref, pic_ref = orc.run_uniform(ref, n, 1.0 / n, steps, 0.0, want_picard=True)
+    ctx = E.Context(D, order)
+    ctx.set_initial_guess(E.GUESS_CERK)
+    mesh = E.Mesh(ctx, n, 1.0 / n)
+    mesh.set_solution(luh0)
+    mesh.init_dt(0.0)
+    mesh.run(steps)
+    mesh.sync()
+    got = mesh.get_solution()
+    pic = mesh.picard()
+    t, dtn = mesh.time()
+    mesh.close()
+    ctx.close()
+    assert rel_l2(got, ref) <= TOL
+    np.testing.assert_array_equal(np.asarray(pic), np.asarray(pic_ref))
+    assert abs(dtn / dt_ref - 1) <= 1e-12
+
+
+@pytest.mark.parametrize("picard", [1, 2, 3])
+def test_fp32_line_kernel_with_cerk_guess(picard):
+    """The FP32 matrix-core line kernel with the CERK guess and a fixed count
+    agrees with the FP32 node-column kernel with the same guess and count, and
+    stays within 1e-6 of the FP64 generic adaptive oracle."""
+    n, order, steps = 3, 5, 3
+    luh = P.density_wave_3d(n, order)
+    outs = []
+    for variant in (3, 1):
+        ctx = E.Context(3, order, 5, 0, precision=E.PREC_FP32, picard_max=picard)
+        ctx.set_predictor_variant(variant)
+        ctx.set_initial_guess(E.GUESS_CERK)
+        mesh = E.Mesh(ctx, n, 1.0 / n)
+        mesh.set_solution(luh)
+        mesh.init_dt(0.0)
+        mesh.run(steps)
+        mesh.sync()
+        outs.append((mesh.get_solution(), mesh.picard()))
+        mesh.close()
+        ctx.close()
+    assert rel_l2(outs[0][0] - luh, outs[1][0] - luh) < 2e-3
+    assert rel_l2(outs[0][0], outs[1][0]) < 2e-6
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    ref = luh.copy()
+    O.Oracle(3, order).run_uniform(ref, n, 1.0 / n, steps, 0.0)
+    g, r = outs[0][0].reshape(-1, 5), ref.reshape(-1, 5)
+    for v in range(5):
+        assert rel_l2(g[:, v], r[:, v]) < 1e-6, (v, rel_l2(g[:, v], r[:, v]))
