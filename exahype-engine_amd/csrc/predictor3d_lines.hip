@@ -575,7 +575,18 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
     // time-line point p = (z, y, x = t) (u = luh there); the stage fluxes go
     // through the tile, which the Picard loop has not used yet.
     const float rdt = static_cast<float>(dt), rih = static_cast<float>(1.0 / h);
-    float* fl = tl;  // [3][NS][V]
+    float* fl = tl;  // [3][V][NS]: variable-major, a line read at fixed v is a point-strided row
+    const int pt = pv / V;
+    // dudx rows of p's coordinates, scaled by 1/h, from the operator columns in LDS
+    // (opl[0][r][hh B + n] = dudx[4 hh + r][n])
+    float dr[3][B];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const int c = d == 0 ? t : (d == 1 ? y : z);
+      const int cc = lane_on ? c : 0;
+#pragma unroll
+      for (int n = 0; n < B; n++) dr[d][n] = rih * opl[0][cc & 3][(cc >> 2) * B + n];
+    }
     float K1[V], K2[V], lw[V];
 #pragma unroll
     for (int v = 0; v < V; v++) lw[v] = u[v];
@@ -589,7 +600,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
           float F[V];
           euler_flux_dir<float>(lw, ir, pr, d, F);
 #pragma unroll
-          for (int v = 0; v < V; v++) fl[d * NS * V + pv + v] = F[v];
+          for (int v = 0; v < V; v++) fl[(d * V + v) * NS + pt] = F[v];
         }
       }
       __syncthreads();
@@ -598,13 +609,13 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           const int c = d == 0 ? t : (d == 1 ? y : z);              // coordinate of p along d
-          const int stride = d == 0 ? V : (d == 1 ? B * V : B * B * V);
-          const float* line = fl + d * NS * V + pv - c * stride;
+          const int stride = d == 0 ? 1 : (d == 1 ? B : B * B);       // point stride along d
+          const float* line = fl + d * V * NS + pt - c * stride;
 #pragma unroll
           for (int v = 0; v < V; v++) {
             float a = 0.f;
 #pragma unroll
-            for (int n = 0; n < B; n++) a += line[n * stride + v] * (rih * T->dudxf[c * B + n]);
+            for (int n = 0; n < B; n++) a += line[v * NS + n * stride] * dr[d][n];
             K[v] += a;
           }
         }
