@@ -47,9 +47,11 @@ for p in (os.path.join(ROOT, "exahype-engine_amd"), os.path.join(ROOT, "oracle")
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per FP32 predictor launch at 81^3, order 5, measured with rocprofv3 FETCH_SIZE (x2, the
-# factor a 1 GiB copy calibrates) + WRITE_SIZE (x1): profiles/euler3d_n5_81cubed_kernels.md.
-MEASURED_PREDICTOR_TRAFFIC = {(81, 5): 18.49e9}
+# HBM bytes per predictor launch at 81^3, order 5 (n, order, precision, CERK guess), measured with
+# rocprofv3 FETCH_SIZE (x2, the factor a 1 GiB copy calibrates) + WRITE_SIZE (x1):
+# profiles/euler3d_n5_81cubed_kernels.md.
+MEASURED_PREDICTOR_TRAFFIC = {(81, 5, "fp32", 1): 14.13e9, (81, 5, "fp32", 0): 14.26e9,
+                              (81, 5, "fp64", 1): 18.45e9, (81, 5, "fp64", 0): 18.45e9}
 
 
 def log(*a):
@@ -127,7 +129,7 @@ def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, vari
         picard_max_executed=int(iters.max()), predictor_flops=flops,
         roofline=dict(bound="mfma" if kind == "fp32" else "fp64-valu", achieved=achieved, peak=PEAK_TFLOPS[kind],
                       unit="TFLOP/s", frac=achieved / PEAK_TFLOPS[kind],
-                      traffic=MEASURED_PREDICTOR_TRAFFIC.get((n, order))),
+                      traffic=MEASURED_PREDICTOR_TRAFFIC.get((n, order, kind, int(bool(guess))))),
         t_end=t_end, dt_next=dt_next, solution=sol)
 
 
