@@ -21,9 +21,13 @@
 //   * Riemann: one wavefront per face, lanes over face nodes; the face
 //     averages are wave reductions (DPP shuffles).
 //   * corrector: thread per (cell, node); the per-cell CFL minimum is reduced
-//     in LDS and merged into the global dt with one 64-bit atomicMin per
-//     workgroup on the IEEE bit pattern (exact and order independent for
-//     positive doubles).
+//     in LDS and merged with one 64-bit atomicMin per workgroup on the IEEE
+//     bit pattern (exact and order independent for positive doubles), into
+//     the instance's dt_next or, on the uniform mesh, into one of 1024 slots
+//     on separate cache lines that dt_reduce_kernel merges afterwards.
+//   * CERK initial guess (optimised-kernel dialect, or chosen per context):
+//     two flux divergences through the flux slab before the Picard loop
+//     (predictor_tile<..., CG = true>).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
