@@ -406,7 +406,7 @@ def main():
         return
     args.n = args.n or 81
     args.order = args.order or 5
-    args.cpu_n = args.cpu_n or 27
+    args.cpu_n = args.cpu_n or 54   # ~10 s of 16-thread CPU work
     args.cpu_steps = args.cpu_steps or 4
 
     log(f"[bench] 3D Euler N={args.order} {args.n}^3 cells, steps={args.steps} warmup={args.warmup}")
