@@ -759,7 +759,8 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
 #pragma unroll
           for (int v = 0; v < V; v++) G[x][v] += tl[own + x * V + v];
       }
-      __syncthreads();
+      // no barrier: the next tile writes (F_z, or G) go to each lane's own
+      // x-line, which no other lane touches until the next barrier
     }
     // ---- t: time solve through the tile ----
     Pm = opaque(Pm);
@@ -865,7 +866,9 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
 #pragma unroll
         for (int v = 0; v < V; v++) q[x][v] = tl[own + x * V + v];
     }
-    __syncthreads();
+    // the next tile writes are to own x-lines (disjoint) behind the x phase;
+    // the adaptive rule rewrites red[] next iteration
+    if (adaptive) __syncthreads();
   }
 
   // ---- after the Picard loop: q = previous iterate, the tile holds the final one ----
