@@ -30,6 +30,7 @@ struct DevTables {
   float dudxf[kMaxBasis * kMaxBasis];
   double Pt[kMaxBasis * kMaxBasis];  // Pt[i][j] = iK1[i][j] * w[j]
   float Ptf[kMaxBasis * kMaxBasis];
+  float cwf[kMaxBasis];  // c[j] = sum_i w[i] Pt[i][j]: time average of one Picard update
   double fgp[3 * kMaxBasis * kMaxBasis];  // fineGridProjector[sub][n][m] = phi_n((sub + x_m) / 3), [(sub*B+n)*B+m]
 };
 

@@ -173,21 +173,21 @@ __device__ __forceinline__ void euler_flux_dir<float>(const float* Q, float irho
   F[4] = (Q[4] + p) * ud;
 }
 
-// lFhi_d = sum_t w_t F_d(q[t]) at one point from its time line q[t][v]
-// (in the order of the point-parallel averages below: t ascending, FP32),
-// stored at af[(d NS) V + pv + v], pv = point index * V.
+// At one point, from its time line q[t][v] (t ascending, FP32):
+// lFhi_d = sum_t w_t F_d(q[t]), stored at af[(d NS) V + pv + v] (pv = point
+// index * V), and fc_d = sum_t c_t F_d(q[t]) with c = T->cwf, returned.
 template <int B>
-__device__ __forceinline__ void time_line_flux_averages(const float (&q)[B][LShape<B>::V],
-                                                        const DevTables* __restrict__ T, float* af, int pv) {
+__device__ __forceinline__ void time_line_averages(const float (&q)[B][LShape<B>::V], const DevTables* __restrict__ T,
+                                                   float* af, int pv, float (&fc)[3][LShape<B>::V]) {
   constexpr int V = 5, NS = B * B * B;
   float fh[3][V];
 #pragma unroll
   for (int d = 0; d < 3; d++)
 #pragma unroll
-    for (int v = 0; v < V; v++) fh[d][v] = 0.f;
+    for (int v = 0; v < V; v++) fh[d][v] = fc[d][v] = 0.f;
 #pragma unroll
   for (int t = 0; t < B; t++) {
-    const float wt = T->wf[t];
+    const float wt = T->wf[t], ct = T->cwf[t];
     float ir, pr;
     euler_prim<float>(q[t], ir, pr);
 #pragma unroll
@@ -195,7 +195,10 @@ __device__ __forceinline__ void time_line_flux_averages(const float (&q)[B][LSha
       float F[V];
       euler_flux_dir<float>(q[t], ir, pr, d, F);
 #pragma unroll
-      for (int v = 0; v < V; v++) fh[d][v] += F[v] * wt;
+      for (int v = 0; v < V; v++) {
+        fh[d][v] += F[v] * wt;
+        fc[d][v] += F[v] * ct;
+      }
     }
   }
 #pragma unroll
@@ -550,10 +553,12 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
   R q[B][V];  // lQi at (t, z, y, x), x < B
   {
     const long base = (cell * NS + ((lane_on ? z : 0) * B + (lane_on ? y : 0)) * B) * V;
+    if constexpr (!CG) {  // the CERK form starts from the guess in the tile
 #pragma unroll
-    for (int x = 0; x < B; x++)
+      for (int x = 0; x < B; x++)
 #pragma unroll
-      for (int v = 0; v < V; v++) q[x][v] = static_cast<R>(luh[base + x * V + v]);
+        for (int v = 0; v < V; v++) q[x][v] = static_cast<R>(luh[base + x * V + v]);
+    }
 #pragma unroll
     for (int v = 0; v < V; v++) u[v] = static_cast<R>(luh[base + (lane_on ? t : 0) * V + v]);
   }
@@ -564,9 +569,22 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
   const int tbase = (z * B + y) * LS + t * V;             // time line of (z, y, x = t), stride B^2 LS
   [[maybe_unused]] const int pv = ((z * B + y) * B + t) * V;  // that point in [node][v] arrays
 
-  // FIX: time averages at this lane's time-line point, lQhi in registers, lFhi_d in af
-  [[maybe_unused]] float qh[V];
+  // FIX: the last Picard iteration is collapsed.  Its output lQi' = luh -
+  // dt/h P G enters only lQhi = sum_t w_t lQi'(t), and by linearity
+  //   lQhi = luh - dt/h sum_d dudx_d Fc_d,  Fc_d = sum_t c_t F_d(lQi(t)),
+  //   c_t = sum_t' w_t' P[t'][t] (T->cwf)
+  // with lQi the iterate that iteration evaluates fluxes of.  Its time lane
+  // forms lFhi_d (into af) and Fc_d (fc, later through the tile) from that
+  // iterate's time line; nfull = maxIter - 1 full sweeps remain.
   [[maybe_unused]] float* af = reinterpret_cast<float*>(buf + TILE_BYTES);
+  const int nfull = FIX ? maxIter - 1 : maxIter;
+  // Fc_d [3][NS][V] at the tile's start, written once no time line is read any more
+  [[maybe_unused]] auto store_fc = [&](const float (&fc)[3][V]) {
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int v = 0; v < V; v++) tl[d * NS * V + pv + v] = fc[d][v];
+  };
   if constexpr (CG) {
     // CERK initial guess (the reference's optimised kernels,
     // KernelGenerator/kernelgenerator/templates/aderdg/fusedSPTVI_nonlinear_cpp.template:121-173):
@@ -577,16 +595,6 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
     const float rdt = static_cast<float>(dt), rih = static_cast<float>(1.0 / h);
     float* fl = tl;  // [3][V][NS]: variable-major, a line read at fixed v is a point-strided row
     const int pt = pv / V;
-    // dudx rows of p's coordinates, scaled by 1/h, from the operator columns in LDS
-    // (opl[0][r][hh B + n] = dudx[4 hh + r][n])
-    float dr[3][B];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const int c = d == 0 ? t : (d == 1 ? y : z);
-      const int cc = lane_on ? c : 0;
-#pragma unroll
-      for (int n = 0; n < B; n++) dr[d][n] = rih * opl[0][cc & 3][(cc >> 2) * B + n];
-    }
     float K1[V], K2[V], lw[V];
 #pragma unroll
     for (int v = 0; v < V; v++) lw[v] = u[v];
@@ -615,7 +623,8 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
           for (int v = 0; v < V; v++) {
             float a = 0.f;
 #pragma unroll
-            for (int n = 0; n < B; n++) a += line[v * NS + n * stride] * dr[d][n];
+            // dudx[c][n] / h from the operator columns in LDS (opl[0][r][hh B + n] = dudx[4 hh + r][n])
+            for (int n = 0; n < B; n++) a += line[v * NS + n * stride] * (rih * opl[0][c & 3][(c >> 2) * B + n]);
             K[v] += a;
           }
         }
@@ -639,32 +648,40 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
       for (int v = 0; v < V; v++) qg[i][v] = u[v] - (rdt * xt * K1[v]) - (0.5f * rdt * xt * xt * (K2[v] - K1[v]));
     }
     if (lane_on) {
+      if (nfull > 0) {
 #pragma unroll
-      for (int i = 0; i < B; i++)
+        for (int i = 0; i < B; i++)
 #pragma unroll
-        for (int v = 0; v < V; v++) tl[tbase + i * (B * B * LS) + v] = qg[i][v];
-      if (maxIter == 1) time_line_flux_averages<B>(qg, T, af, pv);  // fluxes of the guess are averaged
+          for (int v = 0; v < V; v++) tl[tbase + i * (B * B * LS) + v] = qg[i][v];
+      } else {  // the collapsed iteration evaluates fluxes of the guess (stage reads are done)
+        float fc[3][V];
+        time_line_averages<B>(qg, T, af, pv, fc);
+        store_fc(fc);
+      }
     }
-    __syncthreads();
-    if (lane_on) {  // this lane's x-line of the guess; the tile's next writes go to own lines only
+    if (nfull > 0) {
+      __syncthreads();
+      if (lane_on) {  // this lane's x-line of the guess; the tile's next writes go to own lines only
 #pragma unroll
-      for (int x = 0; x < B; x++)
+        for (int x = 0; x < B; x++)
 #pragma unroll
-        for (int v = 0; v < V; v++) q[x][v] = tl[own + x * V + v];
+          for (int v = 0; v < V; v++) q[x][v] = tl[own + x * V + v];
+      }
     }
   } else if constexpr (FIX) {
-    if (maxIter == 1 && lane_on) {  // the last iteration evaluates fluxes of the initial guess
-      float qi[B][V];
+    if (nfull == 0 && lane_on) {  // the collapsed iteration evaluates fluxes of the initial guess
+      float qi[B][V], fc[3][V];
 #pragma unroll
       for (int i = 0; i < B; i++)
 #pragma unroll
         for (int v = 0; v < V; v++) qi[i][v] = u[v];
-      time_line_flux_averages<B>(qi, T, af, pv);
+      time_line_averages<B>(qi, T, af, pv, fc);
+      store_fc(fc);
     }
   }
 
   int iters = maxIter;
-  for (int iter = 0; iter < maxIter; iter++) {
+  for (int iter = 0; iter < nfull; iter++) {
     R G[B][V];
     Dm = opaque(Dm);
     launder(q);
@@ -786,23 +803,21 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
 #pragma unroll
           for (int i = 0; i < B; i++) qn[i][v] = u[v] - cdt * o[i];
         }
-        if (lane_on) {
-          if (iter < maxIter - 1) {
+        if (iter < nfull - 1) {  // block-uniform
+          if (lane_on) {
 #pragma unroll
             for (int i = 0; i < B; i++)
 #pragma unroll
               for (int v = 0; v < V; v++) tl[tbase + i * ts + v] = qn[i][v];
-            if (iter == maxIter - 2) time_line_flux_averages<B>(qn, T, af, pv);
-          } else {
-            float sum[V] = {0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int i = 0; i < B; i++) {
-              const float wt = T->wf[i];
-#pragma unroll
-              for (int v = 0; v < V; v++) sum[v] += qn[i][v] * wt;
-            }
-#pragma unroll
-            for (int v = 0; v < V; v++) qh[v] = sum[v];  // set in the last iteration only
+          }
+        } else {
+          // the iterate the collapsed iteration starts from; its flux averages
+          // are formed behind the barrier, once every time line has been read
+          __syncthreads();
+          if (lane_on) {
+            float fc[3][V];
+            time_line_averages<B>(qn, T, af, pv, fc);
+            store_fc(fc);
           }
         }
       } else {
@@ -845,7 +860,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
           sq += dq * dq;
         }
     }
-    bool last = (iter == maxIter - 1);
+    bool last = (iter == nfull - 1);
     if (adaptive) {
 #pragma unroll
       for (int m = 32; m >= 1; m >>= 1) sq += __shfl_xor(sq, m, 64);
@@ -873,13 +888,32 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
 
   // ---- after the Picard loop: q = previous iterate, the tile holds the final one ----
   if constexpr (FIX) {
-    // every time line has been read (barrier at the end of the last time phase)
+    // collapsed last iteration: Fc_d [3][NS][V] at the tile's start (stored
+    // above), lQhi [NS][V] behind it
+    float* fcb = tl;
+    float* aq = tl + 3 * NS * V;
+    __syncthreads();
     if (lane_on) {
+      // coordinates of p = (z, y, x = t), recomputed from pv (not kept live through the loop)
+      const int pp = pv / V, px = pp % B, py = (pp / B) % B, pz = pp / (B * B);
+      float dv[V] = {0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int v = 0; v < V; v++) tl[pv + v] = qh[v];
+      for (int d = 0; d < 3; d++) {
+        const int c = d == 0 ? px : (d == 1 ? py : pz);              // coordinate of p along d
+        const int stride = d == 0 ? V : (d == 1 ? B * V : B * B * V);  // [node][v] stride along d
+        const float* line = fcb + d * NS * V + pv - c * stride;
+#pragma unroll
+        for (int n = 0; n < B; n++) {
+          const float dn = opl[0][c & 3][(c >> 2) * B + n];           // dudx[c][n]
+#pragma unroll
+          for (int v = 0; v < V; v++) dv[v] += dn * line[n * stride + v];
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < V; v++) aq[pv + v] = u[v] - cdt * dv[v];
     }
     __syncthreads();
-    finish_from_averages<B>(tl, af, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd, S::THREADS,
+    finish_from_averages<B>(aq, af, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd, S::THREADS,
                             f32faces);
   } else if constexpr (sizeof(R) == 4) {
     float* t2 = reinterpret_cast<float*>(buf) + TILE;
