@@ -486,10 +486,10 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
   if (l == 0 && picard) picard[cell] = iters;
 }
 
-template <int B, typename R, bool MF, bool FIX, bool CG>
+template <int B, typename R, bool MF, bool FIX, bool CG, bool ONE = false>
 // Fixed-count form: four workgroups per CU (LDS 39 KB each; the register
 // budget of 128 costs a few spills, measured faster than three workgroups)
-__global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor3d_lines_kernel(
+__global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) void predictor3d_lines_kernel(
     long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ lduh, double* __restrict__ lQhbnd,
     double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, PdeParams P,
@@ -514,9 +514,14 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
   // iteration evaluates fluxes of) into af behind the tile, lQhi in the last
   // iteration; the tile then holds lQhi.
   constexpr int AF_BYTES = 3 * NS * V * 4;
-  constexpr int NEED = FIX ? TILE_BYTES + AF_BYTES : (sizeof(R) == 4 ? 2 * TILE_BYTES : TILE_BYTES);
+  // ONE (one Picard iteration, collapsed): no sweep through the tile; the
+  // buffer holds the CERK stage fluxes / Fc_d [3][NS][V] and lQhi [NS][V], then af
+  static_assert(!ONE || FIX, "one collapsed iteration: fixed-count form");
+  constexpr int ONE_BYTES = 4 * NS * V * 4;
+  constexpr int NEED = ONE ? ONE_BYTES + AF_BYTES
+                          : (FIX ? TILE_BYTES + AF_BYTES : (sizeof(R) == 4 ? 2 * TILE_BYTES : TILE_BYTES));
   constexpr int BUF_BYTES = (FIX || NEED > AVG_BYTES) ? NEED : AVG_BYTES;
-  static_assert(!MF || (lines_max_read<B, LS>() + 1) * 4 <= BUF_BYTES, "idle lanes' reads stay inside buf");
+  static_assert(!MF || ONE || (lines_max_read<B, LS>() + 1) * 4 <= BUF_BYTES, "idle lanes' reads stay inside buf");
   __shared__ __align__(16) unsigned char buf[BUF_BYTES];
   __shared__ double red[NWAVE];
   R* tl = reinterpret_cast<R*>(buf);
@@ -576,8 +581,8 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (FIX ? 4 : 1)) void predictor
   // with lQi the iterate that iteration evaluates fluxes of.  Its time lane
   // forms lFhi_d (into af) and Fc_d (fc, later through the tile) from that
   // iterate's time line; nfull = maxIter - 1 full sweeps remain.
-  [[maybe_unused]] float* af = reinterpret_cast<float*>(buf + TILE_BYTES);
-  const int nfull = FIX ? maxIter - 1 : maxIter;
+  [[maybe_unused]] float* af = reinterpret_cast<float*>(buf + (ONE ? ONE_BYTES : TILE_BYTES));
+  const int nfull = ONE ? 0 : (FIX ? maxIter - 1 : maxIter);
   // Fc_d [3][NS][V] at the tile's start, written once no time line is read any more
   [[maybe_unused]] auto store_fc = [&](const float (&fc)[3][V]) {
 #pragma unroll
@@ -1199,7 +1204,13 @@ int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const d
                        cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, 0);
   } else {
     if (k.guess && !(mfma && !adaptive)) return -1;  // CERK: fixed-count matrix-core form only
-    if (mfma && !adaptive && k.guess)
+    if (mfma && !adaptive && maxIter == 1 && k.guess)  // one collapsed iteration: no sweep code, smaller LDS
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true, true>), dim3(blocks), dim3(S::THREADS),
+                         0, st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+    else if (mfma && !adaptive && maxIter == 1)
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, false, true>), dim3(blocks), dim3(S::THREADS),
+                         0, st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+    else if (mfma && !adaptive && k.guess)
       hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true>), dim3(blocks), dim3(S::THREADS), 0, st,
                          n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
     else if (mfma && !adaptive)
