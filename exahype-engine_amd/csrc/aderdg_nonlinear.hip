@@ -762,7 +762,8 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
   // One face-node task per thread (3D order 5, 2D order >= 3): the thread
   // loads both sides' states and fluxes once and keeps them for phase 3.
   constexpr bool ONE = NTASK <= NS;
-  double rQL[ONE ? ND : 1], rQR[ONE ? ND : 1], rFL[ONE ? V : 1], rFR[ONE ? V : 1];
+  // kept in the face storage type (FP32 faces: half the registers), widened at use
+  TF rQL[ONE ? ND : 1], rQR[ONE ? ND : 1], rFL[ONE ? V : 1], rFR[ONE ? V : 1];
   // the node's solution, loaded up front so that its latency overlaps the face phases
   double Q[ND];
   if (valid) {
@@ -783,21 +784,21 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
       const TF* fnb = g >= 0 ? gf + g * NF * V : lFhbnd + (nb * 2 * D + (f ^ 1)) * NF * V;
 #pragma unroll
       for (int k = 0; k < ND; k++) {
-        const double a = face_val(qo, fn, k, NF, ND), b = face_val(qn, fn, k, NF, ND);
+        const TF a = static_cast<TF>(face_val(qo, fn, k, NF, ND)), b = static_cast<TF>(face_val(qn, fn, k, NF, ND));
         rQL[k] = o ? a : b;
         rQR[k] = o ? b : a;
       }
 #pragma unroll
       for (int k = 0; k < V; k++) {
-        const double a = face_val(fo, fn, k, NF, V), b = face_val(fnb, fn, k, NF, V);
+        const TF a = static_cast<TF>(face_val(fo, fn, k, NF, V)), b = static_cast<TF>(face_val(fnb, fn, k, NF, V));
         rFL[k] = o ? a : b;
         rFR[k] = o ? b : a;
       }
       const double wj = (D == 3) ? w[fn / B] * w[fn % B] : w[fn];
 #pragma unroll
       for (int k = 0; k < ND; k++) {
-        red[(task * 2 + 0) * ND + k] = wj * rQL[k];
-        red[(task * 2 + 1) * ND + k] = wj * rQR[k];
+        red[(task * 2 + 0) * ND + k] = wj * static_cast<double>(rQL[k]);
+        red[(task * 2 + 1) * ND + k] = wj * static_cast<double>(rQR[k]);
       }
     }
   } else if (valid) {
@@ -866,7 +867,10 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
       const int task = s;
       const double smax = sm[task / NF];
 #pragma unroll
-      for (int k = 0; k < V; k++) red[task * V + k] = 0.5 * (rFR[k] + rFL[k]) - 0.5 * smax * (rQR[k] - rQL[k]);
+      for (int k = 0; k < V; k++) {
+        const double fr = rFR[k], fl = rFL[k], qr = rQR[k], ql = rQL[k];
+        red[task * V + k] = 0.5 * (fr + fl) - 0.5 * smax * (qr - ql);
+      }
     }
   } else if (valid) {
     for (int task = s; task < NTASK; task += NS) {
