@@ -232,3 +232,68 @@ def test_fp32_line_kernel_with_cerk_guess(picard):
     g, r = outs[0][0].reshape(-1, 5), ref.reshape(-1, 5)
     for v in range(5):
         assert rel_l2(g[:, v], r[:, v]) < 1e-6, (v, rel_l2(g[:, v], r[:, v]))
+
+
+@pytest.mark.parametrize("D,order,n,steps,bc", [(3, 4, 3, 2, [1, 1, 0, 0, 2, 2]), (2, 3, 6, 4, [2, 2, 1, 1])])
+def test_mesh_with_boundaries_and_cerk_guess_vs_oracle(D, order, n, steps, bc):
+    """CERK guess (generic rule, FP64 node-column kernel) on a mesh with wall and
+    Dirichlet faces: the boundary path and the guess together."""
+    print(f"[cerk boundary] D={D} order={order} n={n} bc={bc}", flush=True)
+    o = O.Oracle(D, order, dialect="cerk")
+    pts = P.node_points(n, order, D)
+    x = np.full(pts.shape[:-1] + (3,), 0.5)
+    x[..., :D] = pts
+    t0 = 0.8
+    luh0 = np.ascontiguousarray(np.array([[o.entropy_wave(p, t0) for p in cell] for cell in x]))
+    ref = luh0.copy()
+    t_ref, dt_ref = o.run_bc(ref, n, 1.0 / n, steps, bc, t=t0)
+    ctx = E.Context(D, order)
+    ctx.set_initial_guess(E.GUESS_CERK)
+    for fused in (True, False):
+        mesh = E.Mesh(ctx, n, 1.0 / n)
+        try:
+            mesh.set_fused(fused)
+        except E.ExaError:
+            assert fused
+            mesh.close()
+            continue
+        mesh.set_boundary(bc)
+        mesh.set_solution(luh0)
+        mesh.init_dt(0.0)
+        mesh.set_time(t0)
+        mesh.run(steps)
+        mesh.sync()
+        got = mesh.get_solution()
+        t, dtn = mesh.time()
+        mesh.close()
+        assert rel_l2(got, ref) <= TOL, (fused, rel_l2(got, ref))
+        assert abs(t - t_ref) <= 1e-14 and abs(dtn / dt_ref - 1) <= 1e-12
+    ctx.close()
+
+
+def test_fp32_cerk_line_kernel_with_walls_agrees_with_node_kernel():
+    """FP32, CERK guess, one fixed iteration (the one-iteration line kernel with
+    FP64 faces, since walls rule out the FP32-face step) against the FP32
+    node-column kernel with the same guess and count."""
+    n, order, steps = 3, 5, 2
+    bc = [1, 1, 0, 0, 0, 0]
+    o = O.Oracle(3, order)
+    pts = P.node_points(n, order, 3)
+    luh = np.ascontiguousarray(np.array([[o.entropy_wave(p, 0.8) for p in cell] for cell in pts])).ravel()
+    outs = []
+    for variant in (3, 1):
+        ctx = E.Context(3, order, 5, 0, precision=E.PREC_FP32, picard_max=1)
+        ctx.set_predictor_variant(variant)
+        ctx.set_initial_guess(E.GUESS_CERK)
+        mesh = E.Mesh(ctx, n, 1.0 / n)
+        mesh.set_boundary(bc)
+        mesh.set_solution(luh)
+        mesh.init_dt(0.0)
+        mesh.set_time(0.8)
+        mesh.run(steps)
+        mesh.sync()
+        outs.append(mesh.get_solution())
+        mesh.close()
+        ctx.close()
+    assert rel_l2(outs[0] - luh, outs[1] - luh) < 2e-3
+    assert rel_l2(outs[0], outs[1]) < 2e-6
