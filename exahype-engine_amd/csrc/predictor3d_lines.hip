@@ -456,11 +456,15 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
     const int fn = f32faces ? o % NF : (o / V) % NF;
     const int d = o / (V * NF);
     const int a = fn / B, b = fn % B;
+    // the line through face node (a, b) along d: node(m) = base + m * stride
+    const int base = (d == 0) ? (a * B + b) * B : (d == 1 ? a * B * B + b : a * B + b);
+    const int sv = ((d == 0) ? 1 : (d == 1 ? B : B * B)) * V;
+    const float* pq = aq + base * V + v;
+    const float* pf = af + d * NS * V + base * V + v;
     float q0 = 0.f, q1 = 0.f, f0 = 0.f, f1 = 0.f;
 #pragma unroll
     for (int m = 0; m < B; m++) {
-      const int node = (d == 0) ? (a * B + b) * B + m : (d == 1 ? (a * B + m) * B + b : (m * B + a) * B + b);
-      const float qv = aq[node * V + v], fv = af[(d * NS + node) * V + v];
+      const float qv = pq[m * sv], fv = pf[m * sv];
       const float c0 = T->F0f[m], c1 = T->F1f[m];
       q0 += qv * c0;
       q1 += qv * c1;
@@ -617,6 +621,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
         }
       }
       __syncthreads();
+      // K = h K_stage = sum_d dudx_d F_d; the 1/h goes into the step coefficients
       float K[V] = {0.f, 0.f, 0.f, 0.f, 0.f};
       if (lane_on) {
 #pragma unroll
@@ -625,12 +630,11 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
           const int stride = d == 0 ? 1 : (d == 1 ? B : B * B);       // point stride along d
           const float* line = fl + d * V * NS + pt - c * stride;
 #pragma unroll
-          for (int v = 0; v < V; v++) {
-            float a = 0.f;
+          for (int n = 0; n < B; n++) {
+            // dudx[c][n] from the operator columns in LDS (opl[0][r][hh B + n] = dudx[4 hh + r][n])
+            const float dn = opl[0][c & 3][(c >> 2) * B + n];
 #pragma unroll
-            // dudx[c][n] / h from the operator columns in LDS (opl[0][r][hh B + n] = dudx[4 hh + r][n])
-            for (int n = 0; n < B; n++) a += line[v * NS + n * stride] * (rih * opl[0][c & 3][(c >> 2) * B + n]);
-            K[v] += a;
+            for (int v = 0; v < V; v++) K[v] += line[v * NS + n * stride] * dn;
           }
         }
       }
@@ -639,7 +643,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
       for (int v = 0; v < V; v++) {
         if (stage == 0) {
           K1[v] = K[v];
-          lw[v] = u[v] - rdt * K1[v];
+          lw[v] = u[v] - (rdt * rih) * K1[v];
         } else {
           K2[v] = K[v];
         }
@@ -649,8 +653,9 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
 #pragma unroll
     for (int i = 0; i < B; i++) {
       const float xt = static_cast<float>(T->x[i]);
+      const float a1 = rdt * rih * xt, a2 = 0.5f * rdt * rih * xt * xt;
 #pragma unroll
-      for (int v = 0; v < V; v++) qg[i][v] = u[v] - (rdt * xt * K1[v]) - (0.5f * rdt * xt * xt * (K2[v] - K1[v]));
+      for (int v = 0; v < V; v++) qg[i][v] = u[v] - a1 * K1[v] - a2 * (K2[v] - K1[v]);
     }
     if (lane_on) {
       if (nfull > 0) {
