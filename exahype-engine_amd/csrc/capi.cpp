@@ -225,6 +225,7 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
     h.F0[i] = c->tab.F0[i];
     h.F1[i] = c->tab.F1[i];
     h.wf[i] = static_cast<float>(c->tab.w[i]);
+    h.iw[i] = 1.0 / c->tab.w[i];
     h.F0f[i] = static_cast<float>(c->tab.F0[i]);
     h.F1f[i] = static_cast<float>(c->tab.F1[i]);
   }

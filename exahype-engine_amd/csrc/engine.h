@@ -32,6 +32,7 @@ struct DevTables {
   float Ptf[kMaxBasis * kMaxBasis];
   float cwf[kMaxBasis];  // c[j] = sum_i w[i] Pt[i][j]: time average of one Picard update
   double fgp[3 * kMaxBasis * kMaxBasis];  // fineGridProjector[sub][n][m] = phi_n((sub + x_m) / 3), [(sub*B+n)*B+m]
+  double iw[kMaxBasis];                   // 1 / w[i]
 };
 
 struct KernelCfg {

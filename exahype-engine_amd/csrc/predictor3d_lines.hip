@@ -415,7 +415,7 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
   if (l < NS) {
     // volume integral at node s = l = (z, y, x)
     const int x = l % B, y = (l / B) % B, z = l / (B * B);
-    const float invDx = static_cast<float>(1.0 / h);
+    const float invDx = __builtin_amdgcn_rcpf(static_cast<float>(h));  // 1 ulp: FP32 forms only
     float du[V];
 #pragma unroll
     for (int v = 0; v < V; v++) du[v] = 0.f;
@@ -438,7 +438,7 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
       for (int v = 0; v < V; v++) du[v] += part[v] * (tw * invDx);
     }
     if (luh_upd) {
-      const double upd = dt / (T->w[z] * T->w[y] * T->w[x]);
+      const double upd = dt * (T->iw[z] * T->iw[y] * T->iw[x]);
 #pragma unroll
       for (int v = 0; v < V; v++) luh_upd[(cell * NS + l) * V + v] += static_cast<double>(du[v]) * upd;
     } else {
@@ -601,7 +601,6 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
     // lQi(t) = luh - dt xi_t K1 - dt xi_t^2 (K2 - K1) / 2.  Lane l works on its
     // time-line point p = (z, y, x = t) (u = luh there); the stage fluxes go
     // through the tile, which the Picard loop has not used yet.
-    const float rdt = static_cast<float>(dt), rih = static_cast<float>(1.0 / h);
     float* fl = tl;  // [3][V][NS]: variable-major, a line read at fixed v is a point-strided row
     const int pt = pv / V;
     float K1[V], K2[V], lw[V];
@@ -643,7 +642,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
       for (int v = 0; v < V; v++) {
         if (stage == 0) {
           K1[v] = K[v];
-          lw[v] = u[v] - (rdt * rih) * K1[v];
+          lw[v] = u[v] - cdt * K1[v];
         } else {
           K2[v] = K[v];
         }
@@ -653,7 +652,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
 #pragma unroll
     for (int i = 0; i < B; i++) {
       const float xt = static_cast<float>(T->x[i]);
-      const float a1 = rdt * rih * xt, a2 = 0.5f * rdt * rih * xt * xt;
+      const float a1 = cdt * xt, a2 = 0.5f * cdt * xt * xt;
 #pragma unroll
       for (int v = 0; v < V; v++) qg[i][v] = u[v] - a1 * K1[v] - a2 * (K2[v] - K1[v]);
     }
