@@ -370,8 +370,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding of the path is a design note, not a runnable mode.", file=sys.stderr)
+        print(f"bench.py: the engine runs on one GPU (asked for --gpus {args.gpus}); sharding the mesh "
+              "over several GPUs is a design, not a runnable mode.", file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: launched with WORLD_SIZE > 1; one GPU only", file=sys.stderr)
