@@ -301,6 +301,41 @@ def test_conservation_and_determinism_full_size():
     ctx.close()
 
 
+def test_fp32_cerk_form_conservation_and_determinism_full_size():
+    """The FP32 form with the CERK guess and one collapsed Picard iteration (the
+    one-iteration line kernel, FP32 faces in the fused step) at the full 81^3
+    N=5 size: each face flux is the same bits for both cells, so mass changes
+    only by the FP32 rounding of the volume terms (bound 1e-9 over two steps);
+    reruns are bitwise identical."""
+    print("[full size fp32 cerk] 81^3 N=5", flush=True)
+    n, order = 81, 5
+    ctx = E.Context(3, order, 5, 0, precision=E.PREC_FP32, picard_max=1)
+    ctx.set_initial_guess(E.GUESS_CERK)
+    luh = P.density_wave_3d(n, order, modes=1)
+    w = E.tables(order)["w"]
+    W = np.einsum("i,j,k->ijk", w, w, w).ravel()
+    h = 1.0 / n
+
+    def mass(u):
+        return float(np.einsum("cn,n->", u.reshape(-1, 216, 5)[:, :, 0], W)) * h ** 3
+
+    m0 = mass(luh)
+    results = []
+    for _ in range(2):
+        mesh = E.Mesh(ctx, n, h)
+        mesh.set_solution(luh)
+        mesh.init_dt(0.0)
+        mesh.run(2)
+        mesh.sync()
+        assert np.all(mesh.picard() == 1)
+        results.append(mesh.get_solution())
+        mesh.close()
+    assert abs(mass(results[0]) / m0 - 1) < 1e-9
+    np.testing.assert_array_equal(results[0], results[1])
+    assert rel_l2(results[0], luh) > 1e-6  # the steps changed the state
+    ctx.close()
+
+
 @pytest.mark.parametrize("dim,n,order", [(3, 3, 5), (3, 4, 3), (2, 9, 3)])
 def test_device_initial_condition_matches_host(dim, n, order):
     """exa_mesh_adjust_solution (device) reproduces the host generator."""
