@@ -43,6 +43,14 @@
  *                   the exact entropy wave; mesh: rest state in a walled box,
  *                   mirror symmetry of a walled box, reduction to the periodic
  *                   driver, and convergence to the exact entropy wave
+ *   cerk_guess /    the optimised kernels' initial guess and Picard rule
+ *   picard rule     (no reference test runs them): the CERK expansion is the
+ *                   exact space-time solution of linear advection with data
+ *                   quadratic in space (one iteration, lQi = u(x - a xi dt));
+ *                   constant states are fixed points; CERK and trivial guesses
+ *                   converge to the same predictor; the trivial-guess arrays
+ *                   equal the generic kernel's at the same iteration count;
+ *                   the cap max(nDof - 3, 1) is reached on rough data
  */
 #ifndef EXAHYPE_AMD_ORACLE_H
 #define EXAHYPE_AMD_ORACLE_H
