@@ -148,6 +148,11 @@ int launch_advance_time(hipStream_t s, double* state, int ninst = 1);
 // points [npts][dim] inside the domain (tensor-product Lagrange interpolation).
 int launch_probe(const KernelCfg& k, hipStream_t s, int n, double h, const double* luh, int npts, const double* pts,
                  double* out);
+// Global observables of a uniform mesh: part holds observables_blocks(ncells)
+// * 3 * nData doubles, out 2 + 3 nData ([dgcells, fvcells, integral, min, max]).
+int observables_blocks(long ncells);
+int launch_observables(const KernelCfg& k, hipStream_t s, long ncells, double h, const double* luh, double* part,
+                       double* out);
 bool boundary_supported(const KernelCfg& k);
 int launch_boundary_mesh(const KernelCfg& k, hipStream_t s, int n, double h, const BcKinds& bk, const double* qb,
                          double* gq, double* gf, const double* state);

@@ -487,6 +487,29 @@ int exa_mesh_set_boundary(exa_mesh* m, const int* kinds) {
   return EXA_OK;
 }
 
+int exa_mesh_global_observables(exa_mesh* m, double* out, int nout) {
+  if (!m || !out || nout < 2 + 3 * m->ND) {
+    exa::set_error("exa_mesh_global_observables: invalid argument (out needs 2 + 3 nData values)");
+    return EXA_ERR_ARG;
+  }
+  const int nobs = 2 + 3 * m->ND;
+  double *d_part = nullptr, *d_out = nullptr;
+  hipError_t e = hipMalloc(&d_part, sizeof(double) * exa::observables_blocks(m->ncells) * 3 * m->ND);
+  if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * nobs);
+  int rc = EXA_OK;
+  if (e == hipSuccess) {
+    rc = status(exa::launch_observables(m->k, m->stream, m->ncells, m->h, m->d_luh, d_part, d_out),
+                "global observables");
+    if (!rc) e = hipMemcpyAsync(out, d_out, sizeof(double) * nobs, hipMemcpyDeviceToHost, m->stream);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  }
+  if (d_part) (void)hipFree(d_part);
+  if (d_out) (void)hipFree(d_out);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(e, "exa_mesh_global_observables");
+  return EXA_OK;
+}
+
 int exa_mesh_probe(exa_mesh* m, int npoints, const double* points, double* values) {
   if (!m || npoints < 0 || (npoints > 0 && (!points || !values))) {
     exa::set_error("exa_mesh_probe: invalid argument");

@@ -95,6 +95,7 @@ _sig("exa_boundaryConditions", [_cp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_doub
 _sig("exa_mesh_set_boundary", [_cp, _ip])
 _sig("exa_mesh_set_time", [_cp, ctypes.c_double])
 _sig("exa_mesh_probe", [_cp, ctypes.c_int, _dp, _dp])
+_sig("exa_mesh_global_observables", [_cp, _dp, ctypes.c_int])
 BC_EXCHANGE = 3
 _sig("exa_mesh_step_predict", [_cp])
 _sig("exa_mesh_step_correct", [_cp])
@@ -457,6 +458,14 @@ class Mesh:
 
     def step_advance(self):
         _check(_lib.exa_mesh_step_advance(self.h))
+
+    def global_observables(self):
+        """dict: dgcells, fvcells, integral[nData], min[nData], max[nData] of the current solution."""
+        nd = self.ctx.nd
+        out = np.zeros(2 + 3 * nd)
+        _check(_lib.exa_mesh_global_observables(self.h, _p(out), out.size))
+        return {"dgcells": out[0], "fvcells": out[1], "integral": out[2:2 + nd], "min": out[2 + nd:2 + 2 * nd],
+                "max": out[2 + 2 * nd:]}
 
     def halo_size(self):
         a, b = ctypes.c_long(0), ctypes.c_long(0)

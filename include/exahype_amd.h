@@ -375,6 +375,14 @@ int exa_mesh_set_dt_next(exa_mesh* mesh, double dt_next);
  * current solution at points[npoints][dim], each inside [0, n h)^dim, interpolated with the
  * cell's Lagrange basis in the cell with offset <= x < offset + h.  Synchronous. */
 int exa_mesh_probe(exa_mesh* mesh, int npoints, const double* points, double* values);
+/* Global observables of the current solution (the solver template's reset / map / merge cycle,
+ * Toolkit/exahype/toolkit/templates/solvers/AbstractGenericADERDGSolverImplementation.template:412-453,
+ * with the Euler example's dgcells / fvcells counts, ApplicationExamples/Euler/Euler/
+ * EulerSolver_ADERDG.cpp:607-628): out[0] = dgcells, out[1] = fvcells (0: no FV limiter),
+ * out[2 + k] = integral of quantity k over the domain (Gauss weights times h^dim),
+ * out[2 + nData + k] = nodal minimum, out[2 + 2 nData + k] = nodal maximum; nout >= 2 + 3 nData.
+ * Reduction order is fixed (bitwise reproducible).  Synchronous. */
+int exa_mesh_global_observables(exa_mesh* mesh, double* out, int nout);
 /* The mesh's HIP stream (hipStream_t as void*). */
 void* exa_mesh_stream(exa_mesh* mesh);
 /* Per-phase timing of the last exa_mesh_run with timing enabled: milliseconds
