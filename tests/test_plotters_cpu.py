@@ -1,0 +1,75 @@
+"""CPU: the Cartesian VTK plotter's file layout (the reference's
+vtk::Cartesian::vertices::ascii, ExaHyPE/exahype/plotters/VTK/
+ADERDG2CartesianVTK.cpp:293-335, 435-456).  Pinned by exact polynomial
+reproduction: a solution that is a polynomial of degree <= order in each
+direction is written at every vertex of every patch as that polynomial's value
+(the projector is the Gauss nodes' Lagrange basis at i / order), and by the
+patch geometry (order^dim axis-aligned sub-cells of edge h / order per cell).
+"""
+import numpy as np
+import pytest
+
+import exahype_amd as E
+from exahype_amd import plotters, problems as P
+
+
+def read_vtk(path):
+    lines = open(path).read().split("\n")
+    out, i = {}, 0
+    while i < len(lines):
+        tok = lines[i].split()
+        if tok and tok[0] == "POINTS":
+            m = int(tok[1])
+            out["points"] = np.array([list(map(float, l.split())) for l in lines[i + 1:i + 1 + m]])
+            i += m
+        elif tok and tok[0] == "CELLS":
+            m = int(tok[1])
+            out["cells"] = np.array([list(map(int, l.split())) for l in lines[i + 1:i + 1 + m]])
+            i += m
+        elif tok and tok[0] == "CELL_TYPES":
+            m = int(tok[1])
+            out["types"] = np.array([int(l) for l in lines[i + 1:i + 1 + m]])
+            i += m
+        elif tok and tok[0] in ("Q", "time") and len(tok) == 4:
+            ncomp, m = int(tok[1]), int(tok[2])
+            out[tok[0]] = np.array([list(map(float, l.split())) for l in lines[i + 1:i + 1 + m]]).reshape(m, ncomp)
+            i += m
+        i += 1
+    return out
+
+
+@pytest.mark.parametrize("D,order,n", [(3, 3, 2), (3, 2, 3), (2, 4, 3), (2, 1, 4)])
+def test_cartesian_vtk_reproduces_polynomials(D, order, n, tmp_path):
+    rng = np.random.default_rng(10 * D + order)
+    B, nd, h = order + 1, 5, 1.0 / n
+    coef = rng.normal(size=(nd,) + (B,) * D)
+
+    def poly(x):  # x[..., D] -> [..., nd]
+        val = np.zeros(x.shape[:-1] + (nd,))
+        for idx in np.ndindex(*(B,) * D):
+            mono = np.ones(x.shape[:-1])
+            for d in range(D):
+                mono = mono * x[..., d] ** idx[d]
+            val += mono[..., None] * coef[(slice(None),) + idx]
+        return val
+
+    luh = poly(P.node_points(n, order, D))  # [cells][nodes][nd]
+    pl = plotters.CartesianVTK(str(tmp_path / "sol"), 4, order, D, E.tables(order)["x"])
+    path = pl.write(luh.ravel(), n, h, nd, 0.25)
+    assert path.endswith("sol-0.vtk")
+    f = read_vtk(path)
+    npts = n ** D * B ** D
+    assert f["points"].shape == (npts, 3) and f["Q"].shape == (npts, 4)
+    np.testing.assert_array_equal(f["time"], 0.25)
+    exact = poly(f["points"][:, :D])[:, :4]
+    assert np.abs(f["Q"] - exact).max() <= 1e-11 * max(1.0, np.abs(exact).max())
+    # geometry: order^D sub-cells per patch, each an axis-aligned box of edge h / order
+    ncorner = 8 if D == 3 else 4
+    assert f["cells"].shape == (n ** D * order ** D, ncorner + 1)
+    assert np.all(f["cells"][:, 0] == ncorner)
+    np.testing.assert_array_equal(f["types"], 12 if D == 3 else 9)
+    for c in f["cells"][:: max(1, len(f["cells"]) // 7)]:
+        q = f["points"][c[1:]]
+        ext = q.max(0) - q.min(0)
+        np.testing.assert_allclose(ext[:D], h / order, rtol=1e-12)
+        np.testing.assert_allclose(q[0] + np.array([h / order, 0, 0]), q[1], rtol=0, atol=1e-14)
