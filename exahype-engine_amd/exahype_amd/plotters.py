@@ -50,6 +50,8 @@ class ProbeAscii:
 
 
 class CartesianVTK:
+    kind = "Cartesian"
+
     """The reference's ``vtk::Cartesian::vertices::ascii`` plotter
     (ExaHyPE/exahype/plotters/VTK/ADERDG2CartesianVTK.cpp:293-335, 435-456):
     every cell (patch) of edge h at offset o is written as its own grid of
@@ -76,6 +78,7 @@ class CartesianVTK:
             for m in range(B):
                 if m != ii:
                     self.proj[ii] *= (e - x[m]) / (x[ii] - x[m])
+        self.frac = e  # vertex positions within the cell, in units of h
         self.count = 0
 
     def vertex_values(self, luh, n, nd):
@@ -98,7 +101,7 @@ class CartesianVTK:
         self.count += 1
         vals = self.vertex_values(luh, n, nd)[:, :, :self.written]
         ncells = n ** D
-        k = np.arange(B) * (h / o)
+        k = self.frac * h
         c1 = np.arange(n) * h
         if D == 3:
             cz, cy, cx = np.meshgrid(c1, c1, c1, indexing="ij")
@@ -123,7 +126,7 @@ class CartesianVTK:
         conn = (np.arange(ncells)[:, None, None] * nv + local[None, :, :]).reshape(-1, len(corners))
         npts = ncells * nv
         with open(path, "w") as f:
-            f.write("# vtk DataFile Version 2.0\nExaHyPE Cartesian vertices, time %.17g\nASCII\n" % time)
+            f.write("# vtk DataFile Version 2.0\nExaHyPE %s vertices, time %.17g\nASCII\n" % (self.kind, time))
             f.write("DATASET UNSTRUCTURED_GRID\nPOINTS %d double\n" % npts)
             np.savetxt(f, pts.reshape(-1, 3), fmt="%.17g")
             f.write("CELLS %d %d\n" % (len(conn), len(conn) * (len(corners) + 1)))
@@ -136,3 +139,18 @@ class CartesianVTK:
             f.write("time 1 %d double\n" % npts)
             np.savetxt(f, np.full(npts, float(time)), fmt="%.17g")
         return path
+
+
+class LegendreVTK(CartesianVTK):
+    """The reference's ``vtk::Legendre::vertices::ascii`` plotter
+    (ExaHyPE/exahype/plotters/VTK/ADERDG2LegendreVTK.cpp:288-300, 333-383): the
+    patch vertices are the cell's Gauss-Legendre nodes o + x_i h and carry the
+    nodal values themselves; the order^dim sub-cells join neighbouring nodes
+    (they do not reach the cell faces)."""
+
+    kind = "Legendre"
+
+    def __init__(self, filename, written_unknowns, order, dim, gauss_nodes):
+        super().__init__(filename, written_unknowns, order, dim, gauss_nodes)
+        self.proj = np.eye(self.order + 1)
+        self.frac = np.asarray(gauss_nodes, dtype=np.float64)

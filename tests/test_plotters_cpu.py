@@ -73,3 +73,17 @@ def test_cartesian_vtk_reproduces_polynomials(D, order, n, tmp_path):
         ext = q.max(0) - q.min(0)
         np.testing.assert_allclose(ext[:D], h / order, rtol=1e-12)
         np.testing.assert_allclose(q[0] + np.array([h / order, 0, 0]), q[1], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("D,order,n", [(3, 3, 2), (2, 4, 3)])
+def test_legendre_vtk_writes_the_nodes(D, order, n, tmp_path):
+    """Vertices at the Gauss-Legendre nodes (ADERDG2LegendreVTK.cpp:296), values
+    the nodal values, in the solution's node order (x fastest)."""
+    rng = np.random.default_rng(D + order)
+    B, nd, h = order + 1, 5, 1.0 / n
+    luh = rng.normal(size=(n ** D, B ** D, nd))
+    pl = plotters.LegendreVTK(str(tmp_path / "leg"), nd, order, D, E.tables(order)["x"])
+    f = read_vtk(pl.write(luh.ravel(), n, h, nd, 1.5))
+    np.testing.assert_allclose(f["points"][:, :D], P.node_points(n, order, D).reshape(-1, D), rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(f["Q"], luh.reshape(-1, nd))
+    assert f["cells"].shape[0] == n ** D * order ** D
