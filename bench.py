@@ -117,33 +117,76 @@ def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, vari
     adaptive = picard <= 0
     max_iter = 2 * (order + 1) if adaptive else picard
     iters = executed_iterations(pic, dim, max_iter, adaptive)
-    flops = float(np.sum(predictor_flops_per_cell(dim, B, V, V, iters, cerk=bool(guess))))
+    # algorithmic predictor flops of one launch at the Picard counts of the last
+    # step (the survey's contraction model); the CERK guess's two flux
+    # divergences are reported apart
+    flops = float(np.sum(predictor_flops_per_cell(dim, B, V, V, iters)))
+    cerk_flops = float(np.sum(predictor_flops_per_cell(dim, B, V, V, iters, cerk=True))) - flops if guess else 0.0
     pred_ms = ph["predictor_ms"] / steps
     kind = "fp64" if precision == E.PREC_FP64 else "fp32"
     achieved = flops / (pred_ms * 1e-3) / 1e12
     dofs = cells * B ** dim * V * steps
+    if kind == "fp64":
+        bound = "fp64-valu"
+    elif picard == 1 and variant == 0 and dim == 3 and B == 6:
+        bound = "fp32-valu"  # one-iteration line kernel: vector ALU only
+    else:
+        bound = "fp32-valu+mfma"
     return dict(
         kind=kind, wall_s=wall, ms_per_step=wall * 1e3 / steps, value=dofs / wall,
         predictor_ms=pred_ms, riemann_ms=ph["riemann_ms"] / steps, corrector_ms=ph["corrector_ms"] / steps,
         picard_hist={int(k): int(v) for k, v in zip(*np.unique(iters, return_counts=True))},
-        picard_max_executed=int(iters.max()), predictor_flops=flops,
-        roofline=dict(bound="mfma" if kind == "fp32" else "fp64-valu", achieved=achieved, peak=PEAK_TFLOPS[kind],
+        picard_max_executed=int(iters.max()), predictor_flops=flops, cerk_flops=cerk_flops,
+        roofline=dict(bound=bound, kernel="predictor", achieved=achieved, peak=PEAK_TFLOPS[kind],
                       unit="TFLOP/s", frac=achieved / PEAK_TFLOPS[kind],
-                      traffic=MEASURED_PREDICTOR_TRAFFIC.get((n, order, kind, int(bool(guess))))),
+                      traffic=MEASURED_PREDICTOR_TRAFFIC.get((n, order, kind, int(bool(guess)))),
+                      flops_per_launch=flops, cerk_flops_per_launch=cerk_flops,
+                      kernel_ms=pred_ms,
+                      effective_with_cerk=(flops + cerk_flops) / (pred_ms * 1e-3) / 1e12),
         t_end=t_end, dt_next=dt_next, solution=sol)
 
 
-def cpu_baseline(O, P, order, n_cpu, steps_cpu, threads):
-    luh = P.density_wave_3d(n_cpu, order)
+def cpu_threads():
+    """(threads used, CPUs visible): the threads are the host's CPU share of one
+    GPU (16 on the GPU box), capped by what this process may run on."""
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        visible = os.cpu_count() or 1
+    return max(1, min(16, visible)), visible
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(O, P, order, n_cpu, steps_cpu, threads, visible, n_one=18, steps_one=1):
+    """The oracle on the host: `threads` OpenMP threads on n_cpu^3 cells, and one
+    thread on n_one^3 cells (the per-core figure)."""
     orc = O.Oracle(3, order)
+    luh = P.density_wave_3d(n_cpu, order)
     t0 = time.perf_counter()
     orc.run_uniform(luh, n_cpu, 1.0 / n_cpu, steps_cpu, 0.0, threads=threads)
     t = time.perf_counter() - t0
     dofs = n_cpu ** 3 * (order + 1) ** 3 * 5 * steps_cpu
+    luh1 = P.density_wave_3d(n_one, order)
+    t1 = time.perf_counter()
+    orc.run_uniform(luh1, n_one, 1.0 / n_one, steps_one, 0.0, threads=1)
+    t1 = time.perf_counter() - t1
+    dofs1 = n_one ** 3 * (order + 1) ** 3 * 5 * steps_one
     return dict(value=dofs / t, unit="DoF-updates/s", cores=threads, kind="port",
+                value_1core=dofs1 / t1, cpu_model=cpu_model(), cpus_visible=visible,
                 sample=f"oracle (FP64 C restatement of the reference generic kernels, OpenMP over cells) on "
                        f"{n_cpu}^3 periodic cells, order {order}, {steps_cpu} global steps of the same workload "
-                       f"({t:.2f} s wall, {threads} threads)")
+                       f"({t:.2f} s wall, {threads} threads = the host's CPU share of one GPU, {visible} CPUs "
+                       f"visible); 1-core figure: {n_one}^3 cells, {steps_one} step(s), one thread ({t1:.2f} s)")
 
 
 def acoustic_bytes_per_cell(B):
@@ -223,11 +266,7 @@ def bench_acoustic(args, E, P):
     cpu = None
     if not args.no_cpu:
         O = __import__("oracle")
-        try:
-            threads = len(os.sched_getaffinity(0))
-        except AttributeError:
-            threads = os.cpu_count() or 1
-        threads = max(1, min(16, threads))
+        threads, visible = cpu_threads()
         cpu = acoustic_cpu_baseline(O, P, order, args.cpu_n, args.cpu_steps, threads)
         log(f"[bench] cpu baseline: {cpu['value']:.4e} DoF/s on {threads} threads")
     out = {
@@ -297,11 +336,7 @@ def bench_batched(args, E, P):
     cpu = None
     if not args.no_cpu:
         O = __import__("oracle")
-        try:
-            threads = len(os.sched_getaffinity(0))
-        except AttributeError:
-            threads = os.cpu_count() or 1
-        threads = max(1, min(16, threads))
+        threads, visible = cpu_threads()
         o2 = O.Oracle(2, order)
         sample = [int(s) for s in sizes[:32]]
         nst = 4
@@ -464,12 +499,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         O = __import__("oracle")
-        try:
-            threads = len(os.sched_getaffinity(0))
-        except AttributeError:
-            threads = os.cpu_count() or 1
-        threads = max(1, min(16, threads))
-        cpu = cpu_baseline(O, P, args.order, args.cpu_n, args.cpu_steps, threads)
+        threads, visible = cpu_threads()
+        cpu = cpu_baseline(O, P, args.order, args.cpu_n, args.cpu_steps, threads, visible)
         log(f"[bench] cpu baseline: {cpu['value']:.4e} DoF/s on {threads} threads")
 
     out = {

@@ -87,7 +87,7 @@ def density_wave_3d(n, order, seed=0, modes=8, xp=np, device=None):
     Z = x1.reshape(n, 1, 1, B, 1, 1)
     rho = 1.0 + 0.2 * xp.sin(2 * np.pi * (X + Y + Z))
     u = 1.0 + 0.0 * (X + Y + Z)
-    amp = 0.01 / modes
+    amp = 0.01 / modes if modes else 0.0
     for m in range(modes):
         rho = rho + amp * xp.cos(2 * np.pi * (kr[m, 0] * X + kr[m, 1] * Y + kr[m, 2] * Z) + pr[m])
         u = u + amp * xp.cos(2 * np.pi * (ku[m, 0] * X + ku[m, 1] * Y + ku[m, 2] * Z) + pu[m])
@@ -108,7 +108,7 @@ def density_wave_params(dim, seed=0, modes=8):
     if dim == 2:
         return np.array([0.2, 1.0, 1.0, 0.0, 1.0, 0, 0.0])
     (kr, pr), (ku, pu) = fourier_modes(seed, modes)
-    prm = [0.2, 1.0, 1.0, 1.0, 1.0, modes, 0.01 / modes]
+    prm = [0.2, 1.0, 1.0, 1.0, 1.0, modes, 0.01 / modes if modes else 0.0]
     for k, ph in ((kr, pr), (ku, pu)):
         for m in range(modes):
             prm += [k[m, 0], k[m, 1], k[m, 2], ph[m]]
