@@ -45,7 +45,7 @@ for p in (os.path.join(ROOT, "exahype-engine_amd"), os.path.join(ROOT, "oracle")
     if p not in sys.path:
         sys.path.insert(0, p)
 
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per predictor launch at 81^3, order 5 (n, order, precision, CERK guess), measured with
 # rocprofv3 FETCH_SIZE (x2, the factor a 1 GiB copy calibrates) + WRITE_SIZE (x1):
@@ -108,6 +108,7 @@ def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, vari
     wall = time.perf_counter() - t0
     ph = mesh.phase_times()
     pic = mesh.picard()
+    recomputed = mesh.picard_guard_recomputed()
     t_end, dt_next = mesh.time()
     sol = mesh.get_solution()
     mesh.close()
@@ -123,11 +124,13 @@ def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, vari
     flops = float(np.sum(predictor_flops_per_cell(dim, B, V, V, iters)))
     cerk_flops = float(np.sum(predictor_flops_per_cell(dim, B, V, V, iters, cerk=True))) - flops if guess else 0.0
     pred_ms = ph["predictor_ms"] / steps
-    kind = "fp64" if precision == E.PREC_FP64 else "fp32"
+    kind = {E.PREC_FP64: "fp64", E.PREC_FP32: "fp32", E.PREC_BF16: "bf16"}[precision]
     achieved = flops / (pred_ms * 1e-3) / 1e12
     dofs = cells * B ** dim * V * steps
     if kind == "fp64":
         bound = "fp64-valu"
+    elif kind == "bf16":
+        bound = "bf16-mfma"
     elif picard == 1 and variant == 0 and dim == 3 and B == 6:
         bound = "fp32-valu"  # one-iteration line kernel: vector ALU only
     else:
@@ -136,6 +139,7 @@ def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, vari
         kind=kind, wall_s=wall, ms_per_step=wall * 1e3 / steps, value=dofs / wall,
         predictor_ms=pred_ms, riemann_ms=ph["riemann_ms"] / steps, corrector_ms=ph["corrector_ms"] / steps,
         picard_hist={int(k): int(v) for k, v in zip(*np.unique(iters, return_counts=True))},
+        guard_recomputed_cells=int(recomputed),
         picard_max_executed=int(iters.max()), predictor_flops=flops, cerk_flops=cerk_flops,
         roofline=dict(bound=bound, kernel="predictor", achieved=achieved, peak=PEAK_TFLOPS[kind],
                       unit="TFLOP/s", frac=achieved / PEAK_TFLOPS[kind],
@@ -385,7 +389,8 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="cells per dimension (81 euler, 54 acoustic)")
     ap.add_argument("--order", type=int, default=None, help="order (5 euler, 4 acoustic)")
     ap.add_argument("--picard-fp32", type=int, default=0,
-                    help="fixed Picard count of the FP32 form (0: the largest count the FP64 form executed)")
+                    help="fixed Picard count of the FP32 form (0: one with the CERK guess and the Picard guard, "
+                         "else the largest count the FP64 form executed)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--guess", choices=("trivial", "cerk"), default="cerk",
                     help="initial guess of the FP32 form's Picard loop (with --sweep-variants: of every form)")
@@ -469,13 +474,15 @@ def main():
                         guess=1)
         log(f"[bench] fp64 (CERK guess): {r64c['value']:.4e} DoF/s, {r64c['ms_per_step']:.2f} ms/step "
             f"(pred {r64c['predictor_ms']:.2f} corr {r64c['corrector_ms']:.2f}), picard {r64c['picard_hist']}")
-    kref = r64c if cerk else r64
-    k32 = args.picard_fp32 if args.picard_fp32 > 0 else kref["picard_max_executed"]
+    # FP32 fixed count: one iteration with the CERK guess (cells the Picard guard
+    # flags are recomputed with three), else the FP64 form's largest executed count
+    k32 = args.picard_fp32 if args.picard_fp32 > 0 else (1 if cerk else r64["picard_max_executed"])
     r32 = run_form(E, P, args.n, args.order, args.steps, args.warmup, E.PREC_FP32, k32, args.seed, args.modes,
                    guess=1 if cerk else 0)
     log(f"[bench] fp32: {r32['value']:.4e} DoF/s, {r32['ms_per_step']:.2f} ms/step "
         f"(pred {r32['predictor_ms']:.2f} riem {r32['riemann_ms']:.2f} corr {r32['corrector_ms']:.2f}), "
-        f"guess {args.guess}, picard {k32}, {r32['roofline']['achieved']:.2f} TFLOP/s")
+        f"guess {args.guess}, picard {k32}, guard recomputed {r32['guard_recomputed_cells']} cells, "
+        f"{r32['roofline']['achieved']:.2f} TFLOP/s")
 
     def rel_err(x, y):
         a = x.reshape(-1, 5)
@@ -494,6 +501,18 @@ def main():
         log(f"[bench] fp32 (trivial guess): {r32t['value']:.4e} DoF/s, {r32t['ms_per_step']:.2f} ms/step "
             f"(pred {r32t['predictor_ms']:.2f}), picard {k32t}, max err {max(r32t['rel_l2_err_vs_fp64']):.2e}")
         r64c["rel_l2_err_vs_fp64"] = rel_err(r64c.pop("solution"), r64["solution"])
+    # BF16 experiment: the tile kernel with BF16 matrix-core operands, beside the same kernel in FP32,
+    # both with the trivial guess and the FP64 form's executed Picard count
+    k16 = r64["picard_max_executed"]
+    r16 = run_form(E, P, args.n, args.order, args.steps, args.warmup, E.PREC_BF16, k16, args.seed, args.modes,
+                   variant=4)
+    r16["rel_l2_err_vs_fp64"] = rel_err(r16.pop("solution"), r64["solution"])
+    r32tile = run_form(E, P, args.n, args.order, args.steps, args.warmup, E.PREC_FP32, k16, args.seed, args.modes,
+                       variant=4)
+    r32tile["rel_l2_err_vs_fp64"] = rel_err(r32tile.pop("solution"), r64["solution"])
+    log(f"[bench] bf16 tile: {r16['value']:.4e} DoF/s, pred {r16['predictor_ms']:.2f} ms, picard {k16}, "
+        f"err {['%.2e' % e for e in r16['rel_l2_err_vs_fp64']]}; fp32 tile: pred {r32tile['predictor_ms']:.2f} ms, "
+        f"max err {max(r32tile['rel_l2_err_vs_fp64']):.2e}")
     r64.pop("solution")
 
     cpu = None
@@ -521,8 +540,9 @@ def main():
                         f"{args.n}^3 periodic cells, density wave + {args.modes} seeded Fourier modes "
                         f"(amp 1e-2, |k|<=4) in rho and u, CFL 0.9, {args.steps} global steps; "
                         + (f"FP32 predictor with the CERK initial guess of the reference's optimised kernels and "
-                           f"{k32} fixed Picard iteration(s) (the largest count the FP64 form with the same guess "
-                           f"executes under the adaptive 1e-7 rule)" if cerk else
+                           f"{k32} fixed Picard iteration(s); the Picard guard recomputes a cell with 3 iterations "
+                           f"when the first iteration changes its time-averaged predictor by >= 1e-6 relative"
+                           if cerk else
                            f"FP32 predictor (fixed {k32} Picard iterations)")
                         + ", FP64 Riemann/corrector/update/CFL; errors vs the FP64 generic kernels (trivial guess, "
                           "adaptive Picard)",
@@ -537,6 +557,7 @@ def main():
         "step_form": "two kernels per step: predictor (+ volume part of the update), then one corrector "
                      "kernel (Riemann problems of every cell's faces + face integrals + update + CFL); "
                      "riemann_ms is the boundary-condition slot between them (periodic: empty)",
+        "fp32_guard_recomputed_cells": r32["guard_recomputed_cells"],
         "speedup_fp32_vs_fp64": r32["value"] / r64["value"],
         "rel_l2_err_fp32_vs_fp64": err,
         "max_rel_l2_err": max(err),
@@ -547,6 +568,11 @@ def main():
         out["fp32_trivial_guess"] = {k: r32t[k] for k in keys if k != "picard_hist"}
         out["fp32_trivial_guess"]["picard_fixed"] = r32t["picard_fixed"]
         out["speedup_fp32_vs_fp64_same_guess"] = r32["value"] / r64c["value"]
+    keys = ("value", "ms_per_step", "predictor_ms", "corrector_ms", "rel_l2_err_vs_fp64")
+    out["bf16_experiment"] = dict({k: r16[k] for k in keys}, picard_fixed=k16, guess="trivial",
+                                  kernel="predictor3d_tile_kernel<6, BF16 operands>",
+                                  predictor_tflops=r16["roofline"]["achieved"],
+                                  fp32_same_kernel={k: r32tile[k] for k in keys})
     print(json.dumps(out), flush=True)
 
 

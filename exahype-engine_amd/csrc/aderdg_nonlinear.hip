@@ -118,7 +118,14 @@ __device__ __forceinline__ int stride_of(int d) {
 // One tile (CPB cells) of the predictor; called by predictor_kernel for
 // tile blk.  Threads of invalid cells return at the end, after the last
 // barrier of the tile.
-template <int D, int B, int V, int NP, typename R, bool CG>
+//
+// NCS: the PDE has a source and / or a non-conservative product
+// (spaceTimePredictorNonlinear.cpph:104-176, 323-367): per time slice the
+// node also publishes its state, reads the B-point lines through it for
+// gradQ = dudx lQi / dx, adds w_t W dt (S - B(Q) gradQ) to its rhs and
+// accumulates the time-averaged S, which the volume integral adds as W lShi
+// (volumeIntegralNonlinear.cpph:204-222).
+template <int D, int B, int V, int NP, typename R, bool CG, bool NCS = false>
 __device__ __forceinline__ void predictor_tile(
     long blk, long ncells, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
@@ -126,7 +133,8 @@ __device__ __forceinline__ void predictor_tile(
     const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect) {
   using S = Shape<D, B, V, NP>;
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, NSP = S::NSP;
-  constexpr int SLAB = NS * D * V;                       // one flux slab, per cell
+  static_assert(!NCS || V == 5, "user PDE hooks: 5 variables");
+  constexpr int SLAB = NS * D * V + (NCS ? NS * V : 0);  // one flux (+ state) slab, per cell
   constexpr int SLAB_BYTES = 2 * CPB * SLAB * int(sizeof(R));
   constexpr int END_BYTES = CPB * NS * (ND + D * V) * 8;  // time averages, FP64
   constexpr int BUF_BYTES = SLAB_BYTES > END_BYTES ? SLAB_BYTES : END_BYTES;
@@ -242,10 +250,15 @@ __device__ __forceinline__ void predictor_tile(
   }
 
   R fh[D][V];  // time-averaged flux of the last executed iteration
+  [[maybe_unused]] R sh[V];  // NCS: time-averaged source - ncp of the last executed iteration
 #pragma unroll
   for (int d = 0; d < D; d++)
 #pragma unroll
     for (int v = 0; v < V; v++) fh[d][v] = R(0);
+  if constexpr (NCS) {
+#pragma unroll
+    for (int v = 0; v < V; v++) sh[v] = R(0);
+  }
   bool active = valid;
   int iters = maxIter;
   for (int iter = 0; iter < maxIter; iter++) {
@@ -259,6 +272,10 @@ __device__ __forceinline__ void predictor_tile(
       for (int d = 0; d < D; d++)
 #pragma unroll
         for (int v = 0; v < V; v++) fh[d][v] = R(0);
+      if constexpr (NCS) {
+#pragma unroll
+        for (int v = 0; v < V; v++) sh[v] = R(0);
+      }
     }
 
 #pragma unroll
@@ -275,6 +292,10 @@ __device__ __forceinline__ void predictor_tile(
             fh[d][v] += F[d][v] * wt;
             slab[(s * D + d) * V + v] = F[d][v];
           }
+        if constexpr (NCS) {
+#pragma unroll
+          for (int v = 0; v < V; v++) slab[NS * D * V + s * V + v] = q[t][v];
+        }
       }
       __syncthreads();
       if (active) {
@@ -293,6 +314,31 @@ __device__ __forceinline__ void predictor_tile(
               const R f = slab[((base[d] + n * stride_of<D, B>(d)) * D + d) * V + v];
               r[v] -= upd * f * Kc[d][n];
             }
+        }
+        if constexpr (NCS) {
+          // gradQ of time slice t (:113-176), then rhs += w_t W dt (S - ncp) (:323-367)
+          const R* qs = slab + NS * D * V;
+          R gq[D][5];
+#pragma unroll
+          for (int d = 0; d < D; d++)
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+              R g = R(0);
+#pragma unroll
+              for (int n = 0; n < B; n++)
+                g += static_cast<R>(invDx) * qs[(base[d] + n * stride_of<D, B>(d)) * V + v] *
+                     tabD<R>(T, cc[d] * B + n);
+              gq[d][v] = g;
+            }
+          R Sv[5];
+          pde_ncs<D, R>(P, q[t], gq, Sv);
+          const R us = static_cast<R>(w[t] * W * dt);
+          const R wt = static_cast<R>(w[t]);
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            r[v] += us * Sv[v];
+            sh[v] += Sv[v] * wt;
+          }
         }
         // (K1)^-1 in time: lQi_new[t'] += iweight * rhs[t] * iK1[t'][t]
         const R iw = static_cast<R>(iweight);
@@ -374,6 +420,10 @@ __device__ __forceinline__ void predictor_tile(
         for (int m = 0; m < B; m++)
           du[v] += T->Kxi[c * B + m] * Fh[(d * NS + base[d] + m * stride_of<D, B>(d)) * V + v] * upd;
     }
+    if constexpr (NCS) {  // lduh += W lShi (volumeIntegralNonlinear.cpph:204-222)
+#pragma unroll
+      for (int v = 0; v < V; v++) du[v] += W * static_cast<double>(sh[v]);
+    }
     if (luh_upd) {
       // volume part of the solution update in place (solutionUpdate.cpph:28-67: dt / w)
       const double upd = dt / W;
@@ -450,14 +500,14 @@ __device__ __forceinline__ void predictor_tile(
 }
 
 // Predictor + volume integral, one workgroup per tile of CPB cells.
-template <int D, int B, int V, int NP, typename R, bool CG>
+template <int D, int B, int V, int NP, typename R, bool CG, bool NCS = false>
 __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_kernel(
     long ncells, long ntiles, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
     double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
     const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect) {
-  predictor_tile<D, B, V, NP, R, CG>(blockIdx.x, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard, T, P,
-                                 maxIter, adaptive, luh_upd, dialect);
+  predictor_tile<D, B, V, NP, R, CG, NCS>(blockIdx.x, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard,
+                                          T, P, maxIter, adaptive, luh_upd, dialect);
 }
 
 // Persistent form: a grid smaller than ntiles loops over the tiles.
@@ -522,6 +572,34 @@ __global__ __launch_bounds__(256) void riemann_kernel(long nfaces, const int4* _
     aR[k] = wave_sum(aR[k]);
   }
   const double smax = fmax(pde_max_abs_eig<D>(P, aL, d), pde_max_abs_eig<D>(P, aR, d));
+  if constexpr (V == 5) {
+    if (pde_has_ncp(P)) {
+      // fluctuations (riemannSolverNonlinear.cpph:95-120): the NCP of the node
+      // average with gradQ[d] = QR - QL; FR = F - ncp/2, FL = F + ncp/2
+      for (int j = lane; j < NF; j += 64) {
+        double gq[D][5], qa[5], ncp[5];
+#pragma unroll
+        for (int e = 0; e < D; e++)
+#pragma unroll
+          for (int k = 0; k < 5; k++) gq[e][k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+          gq[d][k] = QR[j * ND + k] - QL[j * ND + k];
+          qa[k] = 0.5 * (QR[j * ND + k] + QL[j * ND + k]);
+        }
+        ExaUserPde::ncp<D, double>(P.u, qa, gq, ncp);
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+          const double fl = FL[j * V + k], fr = FR[j * V + k];
+          const double ql = QL[j * ND + k], qr = QR[j * ND + k];
+          const double flux = 0.5 * (fr + fl) - 0.5 * smax * (qr - ql);
+          FR[j * V + k] = flux - 0.5 * ncp[k];
+          FL[j * V + k] = flux + 0.5 * ncp[k];
+        }
+      }
+      return;
+    }
+  }
   for (int j = lane; j < NF; j += 64) {
 #pragma unroll
     for (int k = 0; k < V; k++) {
@@ -1005,6 +1083,18 @@ struct Launch {
     const bool cerk = k.guess != 0;
     // the CERK form is compiled without the persistent variant (one workgroup per tile)
     const bool persistent = grid < tiles && !cerk;
+    if (pde_has_ncs(k.pde)) {
+      // source / non-conservative terms: FP64, trivial guess, one workgroup per tile, 5 variables
+      if constexpr (V == 5 && NP == 0) {
+        if (k.precision != 0 || cerk) return -1;
+        hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, double, false, true>), dim3(tiles), dim3(S::THREADS), 0,
+                           st, n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive,
+                           o.luh_update, dl);
+        return static_cast<int>(hipGetLastError());
+      } else {
+        return -1;
+      }
+    }
     if (k.precision == 0)
       launch_predictor_form<double>(st, cerk, persistent, grid, tiles, n, luh, h, dt, lduh, qb, fb, pic, k,
                                     maxIter, adaptive, o, dl);
@@ -1054,6 +1144,7 @@ struct Launch {
                                const double* qb, const double* fb, const double* h, const double* dt,
                                double* dt_next, const LaunchOpts& o) {
     if constexpr (FusedShape<D, B, V, NP>::ok) {
+      if (pde_has_ncp(k.pde)) return -1;  // the fused corrector solves conservative Rusanov problems only
       if (n <= 0) return 0;
       const long blocks = (n + S::CPB - 1) / S::CPB;
       if (o.faces_f32) {
@@ -1138,13 +1229,18 @@ int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* lu
   const Entry* e = find_entry(k);
   if (!e) return -1;
   if (n <= 0) return 0;
+  // BF16 operands: the tile kernel (3D Euler order 5, trivial guess) only
+  if (k.precision == 2) {
+    if (!line_kernels_serve(k) || k.guess) return -1;
+    return launch_predictor3d_tile(k, s, n, luh, h, dt, lduh, qb, fb, pic, o.cinst, o.luh_update, o.faces_f32);
+  }
   // auto: FP32 -> line kernel on the matrix cores; FP64 -> node-column kernel
   if (line_kernels_serve(k)) {
     const int v = k.pred_variant;
     const bool fp32 = k.precision != 0;
     if ((v == 0 && fp32) || v == 3)
       return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, fp32, o.cinst, o.luh_update,
-                                      o.faces_f32);
+                                      o.faces_f32, o.guard);
     if (v == 2)
       return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, false, o.cinst, o.luh_update,
                                       o.faces_f32);

@@ -41,6 +41,9 @@ struct exa_ctx {
   int bc_kind = EXA_BC_DIRICHLET_ENTROPY_WAVE;
   // vector width (doubles) of the optimised dialect's padded arrays
   int vect = 1;
+  // Picard-guard list of the batched predictor: {listed, listed in total, list[guard_cap]}
+  int* d_guard = nullptr;
+  long guard_cap = 0;
 };
 
 namespace {
@@ -72,6 +75,18 @@ int ensure_scratch(exa_ctx* c, size_t bytes) {
   c->scratch_bytes = 0;
   HIP_OK(hipMalloc(&c->d_scratch, bytes), "hipMalloc(scratch)");
   c->scratch_bytes = bytes;
+  return EXA_OK;
+}
+
+// Guard buffer for ncells cells (grown on demand, counters cleared when allocated).
+int ensure_guard(exa_ctx* c, long ncells) {
+  if (c->guard_cap >= ncells) return EXA_OK;
+  if (c->d_guard) (void)hipFree(c->d_guard);
+  c->d_guard = nullptr;
+  c->guard_cap = 0;
+  HIP_OK(hipMalloc(&c->d_guard, sizeof(int) * (2 + ncells)), "hipMalloc(guard)");
+  HIP_OK(hipMemset(c->d_guard, 0, 2 * sizeof(int)), "clear guard");
+  c->guard_cap = ncells;
   return EXA_OK;
 }
 
@@ -170,7 +185,7 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
     set_error("exa_ctx_create: dim must be 2 or 3");
     return EXA_ERR_ARG;
   }
-  if (pde != EXA_PDE_EULER && pde != EXA_PDE_ADVECTION && pde != EXA_PDE_ACOUSTIC) {
+  if (pde != EXA_PDE_EULER && pde != EXA_PDE_ADVECTION && pde != EXA_PDE_ACOUSTIC && pde != EXA_PDE_USER) {
     set_error("exa_ctx_create: unknown pde");
     return EXA_ERR_ARG;
   }
@@ -179,8 +194,16 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
     return EXA_ERR_ARG;
   }
   if (precision == EXA_PREC_BF16) {
-    set_error("exa_ctx_create: BF16 predictor not compiled into this library");
-    return EXA_ERR_UNSUPPORTED;
+    // the BF16 operand form is compiled for the 3D Euler order-5 tile kernel
+    // with a fixed Picard count (BF16 iterates cannot meet the 1e-7 rule)
+    if (dim != 3 || order != 5 || nvar != 5 || npar != 0 || pde != EXA_PDE_EULER) {
+      set_error("exa_ctx_create: BF16 predictor compiled for 3D Euler order 5 (nvar 5, npar 0) only");
+      return EXA_ERR_UNSUPPORTED;
+    }
+    if (picard_max <= 0) {
+      set_error("exa_ctx_create: the BF16 predictor needs a fixed Picard count (picard_max > 0)");
+      return EXA_ERR_ARG;
+    }
   }
   if (!(cfl > 0)) {
     set_error("exa_ctx_create: cfl must be positive");
@@ -213,6 +236,9 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
   c->k.picard_max = picard_max;
   c->k.cfl = cfl;
   c->k.pnpm = exa::pnpm_for_order(order);
+  // relative: the FP32 form's accuracy target (its divergences carry ~1e-8 relative rounding)
+  c->k.guard_tol = 1e-6;
+  c->k.guard_recount = 0;
   c->k.pde.id = pde;
   c->k.pde.a[0] = 1.0;
   c->k.pde.a[1] = 0.5;
@@ -261,10 +287,20 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
 int exa_ctx_destroy(exa_ctx* c) {
   if (!c) return EXA_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->d_guard) (void)hipFree(c->d_guard);
   if (c->d_scratch) (void)hipFree(c->d_scratch);
   if (c->d_tab) (void)hipFree(c->d_tab);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  return EXA_OK;
+}
+
+int exa_ctx_set_pde_params(exa_ctx* c, const double* prm, int n) {
+  if (!c || n < 0 || n > exa::kUserParams || (n > 0 && !prm)) {
+    set_error("exa_ctx_set_pde_params: invalid argument (at most 16 parameters)");
+    return EXA_ERR_ARG;
+  }
+  for (int i = 0; i < exa::kUserParams; i++) c->k.pde.u[i] = i < n ? prm[i] : 0.0;
   return EXA_OK;
 }
 
@@ -343,6 +379,16 @@ int exa_ctx_padded_size(const exa_ctx* c, int which, int* size) {
     return EXA_ERR_ARG;
   }
   *size = padded(which ? ndata(c) : c->k.nvar, c->vect);
+  return EXA_OK;
+}
+
+int exa_ctx_set_picard_guard(exa_ctx* c, double tol, int recompute_count) {
+  if (!c || recompute_count < 0 || recompute_count > 2 * (c->k.order + 1) || !(tol == tol)) {
+    set_error("exa_ctx_set_picard_guard: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  c->k.guard_tol = tol;
+  c->k.guard_recount = recompute_count;
   return EXA_OK;
 }
 
@@ -471,7 +517,7 @@ int exa_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* c, double* lduh, double* 
   const int NS = nspace(c), NF = nface(c), ND = ndata(c), V = c->k.nvar, D = c->k.dim;
   const size_t nLuh = size_t(NS) * ND, nLduh = size_t(NS) * V;
   const size_t nQb = size_t(2 * D) * NF * ND, nFb = size_t(2 * D) * NF * V;
-  const size_t total = nLuh + nLduh + nQb + nFb + 2 + 2;
+  const size_t total = nLuh + nLduh + nQb + nFb + 2 + 2 + 2;
   int rc = ensure_scratch(c, total * sizeof(double));
   if (rc) return rc;
   double* d_luh = c->d_scratch;
@@ -481,11 +527,15 @@ int exa_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* c, double* lduh, double* 
   double* d_h = d_fb + nFb;
   double* d_dt = d_h + 2;
   int* d_pic = reinterpret_cast<int*>(d_dt + 1);
+  int* d_guard = reinterpret_cast<int*>(d_dt + 2);  // {count, total, list[1]}
   double hd[3] = {dx[0], dt, 0.0};
   HIP_OK(hipMemcpyAsync(d_luh, luh, nLuh * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy luh");
   HIP_OK(hipMemcpyAsync(d_h, &hd[0], sizeof(double), hipMemcpyHostToDevice, c->stream), "copy dx");
   HIP_OK(hipMemcpyAsync(d_dt, &hd[1], sizeof(double), hipMemcpyHostToDevice, c->stream), "copy dt");
-  rc = launch_status(exa::launch_predictor(c->k, c->stream, 1, d_luh, d_h, d_dt, d_lduh, d_qb, d_fb, d_pic),
+  HIP_OK(hipMemsetAsync(d_guard, 0, 2 * sizeof(double), c->stream), "clear guard");
+  exa::LaunchOpts po;
+  po.guard = d_guard;
+  rc = launch_status(exa::launch_predictor(c->k, c->stream, 1, d_luh, d_h, d_dt, d_lduh, d_qb, d_fb, d_pic, po),
                      "predictor");
   if (rc) return rc;
   if (!addVolumeIntegralResultToUpdate) {
@@ -1016,8 +1066,14 @@ int exa_batch_predictor(exa_ctx* c, void* stream, long ncells, const double* d_l
     set_error("exa_batch_predictor: invalid argument");
     return EXA_ERR_ARG;
   }
+  exa::LaunchOpts o;
+  if (ncells > 0 && exa::predictor_uses_guard(c->k)) {
+    const int rc = ensure_guard(c, ncells);
+    if (rc) return rc;
+    o.guard = c->d_guard;
+  }
   return launch_status(exa::launch_predictor(c->k, static_cast<hipStream_t>(stream), ncells, d_luh, d_h, d_dt,
-                                             d_lduh, d_lQhbnd, d_lFhbnd, d_picard),
+                                             d_lduh, d_lQhbnd, d_lFhbnd, d_picard, o),
                        "exa_batch_predictor");
 }
 

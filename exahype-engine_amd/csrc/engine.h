@@ -49,6 +49,12 @@ struct KernelCfg {
                          // count min(iter+1, Max)); runs on the node-column kernel
   int guess;             // initial guess: 0 luh at every time node; 1 CERK (node-column kernel,
                          // and the FP32 matrix-core line kernel with a fixed Picard count)
+  // Picard guard of the one-iteration FP32 CERK form: a cell whose relative
+  // proxy residual |lQhi(1) - lQhi(0)| / |lQhi(1)| reaches guard_tol (<= 0:
+  // off) is recomputed with guard_recount fixed iterations (0: max(B - 3, 1),
+  // the optimised kernels' CERK cap)
+  double guard_tol;
+  int guard_recount;
 };
 
 // Launch options of the batched kernels.
@@ -80,6 +86,9 @@ struct LaunchOpts {
   // kDtSlots slots (separate cache lines) instead of all to dt_next; a
   // launch_dt_reduce then merges the slots into dt_next and resets them
   double* dt_slots = nullptr;
+  // predictor, one-iteration FP32 CERK form: Picard guard buffer of 2 + ncells
+  // ints {listed this launch (0 on entry), listed in total, list}; nullptr: no guard
+  int* guard = nullptr;
 };
 
 // Slots of the spread dt reduction: one device-scope atomic per workgroup on a
@@ -129,7 +138,7 @@ bool line_kernels_serve(const KernelCfg& k);
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
                              const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard,
                              bool mfma, const int* cinst = nullptr, double* luh_update = nullptr,
-                             bool f32faces = false);
+                             bool f32faces = false, int* guard = nullptr);
 // FP32 tile form of the same predictor (3D Euler, order 5): space-time iterate
 // in LDS, contractions on 16x16x4 matrix-core steps; -1 for FP64.
 int launch_predictor3d_tile(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
@@ -137,6 +146,9 @@ int launch_predictor3d_tile(const KernelCfg& k, hipStream_t s, long ncells, cons
                             const int* cinst = nullptr, double* luh_update = nullptr, bool f32faces = false);
 // The predictor selected by k writes FP32 faces on request (LaunchOpts::faces_f32).
 bool predictor_f32_faces_supported(const KernelCfg& k);
+// The predictor selected by k runs the Picard guard when given LaunchOpts::guard
+// (the one-iteration FP32 CERK line kernel with guard_tol > 0).
+bool predictor_uses_guard(const KernelCfg& k);
 // Time bookkeeping of the device step per instance, state = t[ninst], dt[ninst],
 // dt_next[ninst]: t += dt; dt = dt_next; dt_next = DBL_MAX.
 int launch_advance_time(hipStream_t s, double* state, int ninst = 1);

@@ -54,6 +54,8 @@ struct exa_mesh {
   double* d_halo = nullptr;  // staging of the host-side halo entry points
   // per-workgroup stable-dt minima spread over kDtSlots cache lines (DBL_MAX between steps)
   double* d_slots = nullptr;
+  // Picard guard of the one-iteration FP32 CERK predictor {listed, listed in total, list[ncells]}
+  int* d_guard = nullptr;
 };
 
 namespace {
@@ -113,6 +115,7 @@ int enqueue_predict(exa_mesh* m, hipEvent_t* ev) {
   exa::LaunchOpts o;
   if (m->fused) o.luh_update = m->d_luh;
   o.faces_f32 = f32_faces(m);
+  o.guard = m->d_guard;
   int rc = status(exa::launch_predictor(m->k, m->stream, m->ncells, m->d_luh, m->d_h, st + 1, m->d_lduh, m->d_qb,
                                         m->d_fb, m->d_picard, o),
                   "predictor");
@@ -167,6 +170,7 @@ void free_mesh(exa_mesh* m) {
     if (b) (void)hipFree(b);
   if (m->d_faces) (void)hipFree(m->d_faces);
   if (m->d_picard) (void)hipFree(m->d_picard);
+  if (m->d_guard) (void)hipFree(m->d_guard);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
 }
@@ -245,6 +249,7 @@ int exa_mesh_create_uniform(exa_mesh** out, exa_ctx* ctx, int n, double h) {
   if (e == hipSuccess) e = hipMalloc(&m->d_faces, sizeof(int) * 4 * D * (n + 1) * (m->ncells / n));
   if (e == hipSuccess) e = hipMalloc(&m->d_picard, sizeof(int) * m->ncells);
   if (e == hipSuccess) e = hipMalloc(&m->d_slots, sizeof(double) * exa::kDtSlots * exa::kDtSlotStride);
+  if (e == hipSuccess) e = hipMalloc(&m->d_guard, sizeof(int) * (2 + m->ncells));
   if (e != hipSuccess) {
     free_mesh(m);
     return fail(e, "exa_mesh_create_uniform");
@@ -256,6 +261,7 @@ int exa_mesh_create_uniform(exa_mesh** out, exa_ctx* ctx, int n, double h) {
   if (e == hipSuccess) e = hipMemcpy(m->d_h, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(m->d_state, st, sizeof(st), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(m->d_picard, 0, sizeof(int) * m->ncells);
+  if (e == hipSuccess) e = hipMemset(m->d_guard, 0, sizeof(int) * 2);
   if (e == hipSuccess) e = static_cast<hipError_t>(exa::launch_dt_slots_init(m->stream, m->d_slots));
   if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
   if (e != hipSuccess) {
@@ -424,6 +430,18 @@ int exa_mesh_time(exa_mesh* m, double* t, double* dt) {
   return EXA_OK;
 }
 
+int exa_mesh_picard_guard(exa_mesh* m, long* recomputed) {
+  if (!m || !recomputed) {
+    exa::set_error("exa_mesh_picard_guard: NULL argument");
+    return EXA_ERR_ARG;
+  }
+  int g[2] = {0, 0};
+  HIP_OK(hipMemcpyAsync(g, m->d_guard, sizeof(g), hipMemcpyDeviceToHost, m->stream), "picard guard");
+  HIP_OK(hipStreamSynchronize(m->stream), "sync");
+  *recomputed = g[1];
+  return EXA_OK;
+}
+
 int exa_mesh_picard(exa_mesh* m, int* counts) {
   if (!m || !counts) return EXA_ERR_ARG;
   HIP_OK(hipMemcpyAsync(counts, m->d_picard, sizeof(int) * m->ncells, hipMemcpyDeviceToHost, m->stream), "picard");
@@ -568,6 +586,12 @@ int exa_mesh_halo_pack(exa_mesh* m, int face, double* d_msg_q, double* d_msg_f) 
   if (!m || !d_msg_q || !d_msg_f || face < 0 || face >= 2 * m->k.dim) {
     exa::set_error("exa_mesh_halo_pack: invalid argument");
     return EXA_ERR_ARG;
+  }
+  if (f32_faces(m)) {
+    // the all-periodic FP32 step keeps its faces as FP32 [face][var][node]
+    exa::set_error("exa_mesh_halo_pack: the mesh's faces are FP32 (all-periodic FP32 step); "
+                   "halo messages need a domain face of kind EXA_BC_EXCHANGE");
+    return EXA_ERR_UNSUPPORTED;
   }
   return status(exa::launch_halo_pack(m->k, m->stream, m->n, face, m->d_qb, m->d_fb, d_msg_q, d_msg_f), "halo pack");
 }

@@ -4,6 +4,8 @@
 //   flux        :61-115  (2D keeps 5 variables, |j|^2 drops Q[3] in 2D, :81-85)
 //   eigenvalues :117-147 (u_n -+ c, u_n x3)
 // Linear advection (test PDE): F_d = a_d Q, eigenvalues a_d.
+// User PDE (kPdeUser): the static hooks of ExaUserPde (include/exahype_amd_pde.h,
+// or the header named by EXA_USER_PDE_HEADER) with the 16 user parameters.
 //
 // The expression order follows the reference hooks so that FP64 results
 // agree to rounding level.
@@ -11,17 +13,33 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/exahype_amd_pde.h"
+
 namespace exa {
 
-enum PdeId { kPdeEuler = 0, kPdeAdvection = 2 };
+enum PdeId { kPdeEuler = 0, kPdeAdvection = 2, kPdeUser = 3 };
+
+constexpr int kUserParams = 16;
 
 struct PdeParams {
   int id;
-  double a[3];  // advection velocity
+  double a[3];            // advection velocity
+  double u[kUserParams];  // user PDE parameters
 };
+
+// The user PDE adds source and / or non-conservative terms to the predictor
+// (and, with kNcp, fluctuations to the Riemann solve).
+__host__ __device__ inline bool pde_has_ncs(const PdeParams& P) {
+  return P.id == kPdeUser && (ExaUserPde::kNcp || ExaUserPde::kSource);
+}
+__host__ __device__ inline bool pde_has_ncp(const PdeParams& P) { return P.id == kPdeUser && ExaUserPde::kNcp; }
 
 template <int D, typename R>
 __device__ __forceinline__ void pde_flux(const PdeParams& P, const R* __restrict__ Q, R (&F)[D][5]) {
+  if (P.id == kPdeUser) {
+    ExaUserPde::flux<D, R>(P.u, Q, F);
+    return;
+  }
   if (P.id == kPdeAdvection) {
 #pragma unroll
     for (int d = 0; d < D; d++)
@@ -55,6 +73,7 @@ __device__ __forceinline__ void pde_flux(const PdeParams& P, const R* __restrict
 // max_v |lambda_v(Q, d)| with the reference's eigenvalue expressions.
 template <int D>
 __device__ __forceinline__ double pde_max_abs_eig(const PdeParams& P, const double* __restrict__ Q, int d) {
+  if (P.id == kPdeUser) return ExaUserPde::max_abs_eigenvalue<D>(P.u, Q, d);
   if (P.id == kPdeAdvection) return fabs(P.a[d]);
   constexpr double gamma = 1.4;
   const double irho = 1. / Q[0];
@@ -64,6 +83,24 @@ __device__ __forceinline__ double pde_max_abs_eig(const PdeParams& P, const doub
   const double c = sqrt(gamma * p * irho);
   const double l0 = fabs(u_n - c), l1 = fabs(u_n), l4 = fabs(u_n + c);
   return fmax(fmax(l0, l1), l4);
+}
+
+// Source minus non-conservative product at one point (spaceTimePredictorNonlinear.cpph:341-362):
+// S = algebraicSource(Q) - nonConservativeProduct(Q, gradQ).
+template <int D, typename R>
+__device__ __forceinline__ void pde_ncs(const PdeParams& P, const R* Q, const R (&gradQ)[D][5], R* S) {
+  if constexpr (ExaUserPde::kSource) {
+    ExaUserPde::source<R>(P.u, Q, S);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; v++) S[v] = R(0);
+  }
+  if constexpr (ExaUserPde::kNcp) {
+    R ncp[5];
+    ExaUserPde::ncp<D, R>(P.u, Q, gradQ, ncp);
+#pragma unroll
+    for (int v = 0; v < 5; v++) S[v] -= ncp[v];
+  }
 }
 
 // Exact entropy wave of the Euler example (ApplicationExamples/Euler/Euler/

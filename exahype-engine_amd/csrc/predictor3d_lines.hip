@@ -93,6 +93,20 @@ __device__ __forceinline__ void mfma_line(const float (&aop)[2][B], const float 
   for (int i = 0; i < B; i++) out[i] = i < 4 ? c0[i] : c1[i - 4];
 }
 
+// Sum of x over the 64 lanes of the wave, returned in every lane: DPP row
+// shifts build each 16-lane row's sum in its lane 15, row broadcasts carry the
+// row sums up to lane 63, whose value is read back as a scalar.  Must be
+// executed by all 64 lanes.
+__device__ __forceinline__ float wave_sum_f32(float x) {
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xf, 0xf, false));  // row_shr:1
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xf, 0xf, false));  // row_shr:2
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xf, 0xe, false));  // row_shr:4
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xf, 0xc, false));  // row_shr:8
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x142, 0xa, 0xf, false));  // row_bcast:15
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
 // Largest tile index (in values) any lane of the line kernel reads in the y,
 // z and t phases, lanes l >= B^3 included (their (t, z, y) run past the cell).
 template <int B, int LS>
@@ -490,14 +504,26 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
   if (l == 0 && picard) picard[cell] = iters;
 }
 
-template <int B, typename R, bool MF, bool FIX, bool CG, bool ONE = false>
+// Picard guard of the one-iteration CERK form and the cell list of its
+// recomputation.  guard = {cells listed this step, cells listed in total, list[]}.
+//   ONE && CG: with guard != nullptr a cell whose relative proxy residual
+//     sum (lQhi(1) - lQhi(0))^2 >= tol2 sum lQhi(1)^2 (FP64 accumulation;
+//     lQhi(k) = sum_t w_t lQi(k)(t)) writes nothing (luh, faces and its Picard
+//     count stay as they are) and appends itself to the list.
+//   LIST (fixed-count CERK form): the workgroups loop over the listed cells.
+struct GuardArgs {
+  int* guard = nullptr;
+  double tol2 = 0.0;
+};
+
+template <int B, typename R, bool MF, bool FIX, bool CG, bool ONE = false, bool LIST = false>
 // Fixed-count form: four workgroups per CU (LDS 39 KB each; the register
 // budget of 128 costs a few spills, measured faster than three workgroups)
 __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) void predictor3d_lines_kernel(
     long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ lduh, double* __restrict__ lQhbnd,
     double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, PdeParams P,
-    int maxIter, int adaptive, double* luh_upd, int f32faces) {
+    int maxIter, int adaptive, double* luh_upd, int f32faces, GuardArgs G) {
   using S = LShape<B>;
   constexpr int V = S::V, NS = S::NS;
   // x-line stride in the tile; the fixed-count form pads each line by one
@@ -521,6 +547,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
   // ONE (one Picard iteration, collapsed): no sweep through the tile; the
   // buffer holds the CERK stage fluxes / Fc_d [3][NS][V] and lQhi [NS][V], then af
   static_assert(!ONE || FIX, "one collapsed iteration: fixed-count form");
+  static_assert(!LIST || (FIX && CG && !ONE), "cell list: the fixed-count CERK form");
   constexpr int ONE_BYTES = 4 * NS * V * 4;
   constexpr int NEED = ONE ? ONE_BYTES + AF_BYTES
                           : (FIX ? TILE_BYTES + AF_BYTES : (sizeof(R) == 4 ? 2 * TILE_BYTES : TILE_BYTES));
@@ -528,20 +555,13 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
   static_assert(!MF || ONE || (lines_max_read<B, LS>() + 1) * 4 <= BUF_BYTES, "idle lanes' reads stay inside buf");
   __shared__ __align__(16) unsigned char buf[BUF_BYTES];
   __shared__ double red[NWAVE];
+  [[maybe_unused]] __shared__ double red2[ONE ? NWAVE : 1];
   R* tl = reinterpret_cast<R*>(buf);
 
   const int l = threadIdx.x;              // (t, z, y) for l < NS
   const bool lane_on = l < NS;            // lanes NS..THREADS-1 only join barriers
-  const long cell = blockIdx.x;
   const int t = l / (B * B), z = (l / B) % B, y = l % B;
 
-  const double dt = dtp[cinst ? cinst[cell] : 0];
-  const double h = hcell[cell];
-  // operators (uniform across lanes; read through the scalar cache at use):
-  //   Dm[i][j] = dudx[i][j] (derivative), Pm[i][j] = iK1[i][j] * w[j]
-  const R* Dm = tab_dudx<R>(T);
-  const R* Pm = tab_P<R>(T);
-  const R cdt = static_cast<R>(dt / h);
   // matrix-core operand columns, staged in LDS and loaded at each contraction
   // phase (registers are the occupancy limit): opl[0][r][h][n] = dudx[4h + r][n],
   // opl[1][r][h][n] = P[4h + r][n] (zero rows >= B); a lane uses r = l % 4
@@ -556,6 +576,15 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
     __syncthreads();
   }
   const int opr = l & 3;
+
+  auto cell_work = [&](const long cell) {
+  const double dt = dtp[cinst ? cinst[cell] : 0];
+  const double h = hcell[cell];
+  // operators (uniform across lanes; read through the scalar cache at use):
+  //   Dm[i][j] = dudx[i][j] (derivative), Pm[i][j] = iK1[i][j] * w[j]
+  const R* Dm = tab_dudx<R>(T);
+  const R* Pm = tab_P<R>(T);
+  const R cdt = static_cast<R>(dt / h);
 
   // time-line role: this lane solves in time for point (z, y, x = t)
   R u[V];
@@ -594,6 +623,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
 #pragma unroll
       for (int v = 0; v < V; v++) tl[d * NS * V + pv + v] = fc[d][v];
   };
+  [[maybe_unused]] float g0[V];
   if constexpr (CG) {
     // CERK initial guess (the reference's optimised kernels,
     // KernelGenerator/kernelgenerator/templates/aderdg/fusedSPTVI_nonlinear_cpp.template:121-173):
@@ -647,6 +677,12 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
           K2[v] = K[v];
         }
       }
+    }
+    if constexpr (ONE) {
+      // sum_t w_t xi_t = 1/2, sum_t w_t xi_t^2 = 1/3: the guess's time average
+      // is luh - cdt (K1 / 2 + (K2 - K1) / 6) = luh - cdt g0
+#pragma unroll
+      for (int v = 0; v < V; v++) g0[v] = K1[v] * (1.f / 3.f) + K2[v] * (1.f / 6.f);
     }
     float qg[B][V];
 #pragma unroll
@@ -901,6 +937,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
     // above), lQhi [NS][V] behind it
     float* fcb = tl;
     float* aq = tl + 3 * NS * V;
+    [[maybe_unused]] float gsq = 0.f, qsq = 0.f;
     __syncthreads();
     if (lane_on) {
       // coordinates of p = (z, y, x = t), recomputed from pv (not kept live through the loop)
@@ -919,9 +956,44 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
         }
       }
 #pragma unroll
-      for (int v = 0; v < V; v++) aq[pv + v] = u[v] - cdt * dv[v];
+      for (int v = 0; v < V; v++) {
+        const float qh = u[v] - cdt * dv[v];
+        aq[pv + v] = qh;
+        if constexpr (ONE && CG) {
+          // lQhi(1) - lQhi(0) = cdt (g0 - dv): luh cancels, so the difference
+          // carries no rounding of luh; the factor cdt^2 is applied to the sum
+          const float dq = g0[v] - dv[v];
+          gsq += dq * dq;
+          qsq += qh * qh;
+        }
+      }
     }
-    __syncthreads();
+    if constexpr (ONE && CG) {
+      if (G.guard) {  // kernel-uniform branch
+        // wave sums in FP32 (at most 64 x 5 terms), the workgroup sum in FP64
+        const float gw = wave_sum_f32(gsq), qw = wave_sum_f32(qsq);
+        if ((l & 63) == 0) {
+          red[l >> 6] = static_cast<double>(gw);
+          red2[l >> 6] = static_cast<double>(qw);
+        }
+      }
+    }
+    __syncthreads();  // lQhi (and the guard's wave sums) complete
+    if constexpr (ONE && CG) {
+      if (G.guard) {
+        double res = 0.0, nrm = 0.0;
+#pragma unroll
+        for (int i = 0; i < NWAVE; i++) {
+          res += red[i];
+          nrm += red2[i];
+        }
+        const double c2 = static_cast<double>(cdt) * static_cast<double>(cdt);
+        if (res * c2 >= G.tol2 * nrm) {  // block-uniform: leave the cell to the recomputation
+          if (l == 0) G.guard[2 + atomicAdd(G.guard, 1)] = static_cast<int>(cell);
+          return;
+        }
+      }
+    }
     finish_from_averages<B>(aq, af, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd, S::THREADS,
                             f32faces);
   } else if constexpr (sizeof(R) == 4) {
@@ -937,6 +1009,18 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
                           S::THREADS, f32faces);
   } else {
     finish_cell<B, R>(q, tl, buf, l, cell, dt, h, iters, T, lduh, lQhbnd, lFhbnd, picard, luh_upd);
+  }
+  };
+
+  if constexpr (LIST) {
+    // the listed cells (guard[0] of them, at guard + 2), grid-strided
+    const long nlist = G.guard[0];
+    for (long i = blockIdx.x; i < nlist; i += gridDim.x) {
+      cell_work(G.guard[2 + i]);
+      __syncthreads();  // the next cell rewrites the LDS buffers
+    }
+  } else {
+    cell_work(blockIdx.x);
   }
 }
 
@@ -961,7 +1045,34 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
 // line form issues 4x4x1 MFMAs at a quarter of the matrix-core rate.
 // Same mathematics and Picard rule as the line form (the FP32 sums are
 // ordered differently).
+//
+// BF16 operand form (template flag BF): the same pairs of the sum go through
+// one v_mfma_f32_16x16x16_bf16 per contraction instead of NK FP32 steps.  In
+// that instruction lane (g, j) = (l / 16, l % 16) supplies A[j][4g + i] and
+// B[4g + i][j], i < 4; the lane puts its FP32-step values of step s = i there
+// (i = NK..3 zero), so k-index 4g + i pairs the same lane values as k-index
+// 4i + g of the FP32 form.  Operator entries and flux / divergence values
+// are rounded to BF16 (nearest even); accumulation, the iterate in LDS, the
+// flux evaluations and everything after the Picard loop stay FP32.
 // ---------------------------------------------------------------------------
+typedef short exa_s4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16 bits, round to nearest even (finite inputs)
+__device__ __forceinline__ short bf16_bits(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<short>(u >> 16);
+}
+
+template <int NK>
+__device__ __forceinline__ exa_s4 bf16_pack(const float (&a)[NK]) {
+  static_assert(NK <= 4, "one 16x16x16 step holds four k values per lane");
+  exa_s4 r = {0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < NK; i++) r[i] = bf16_bits(a[i]);
+  return r;
+}
+
 template <int B>
 struct TShape {
   static constexpr int V = 5;
@@ -988,7 +1099,7 @@ __device__ __forceinline__ int line_point(int d, int L, int n) {
   return ((tt * B + n) * B + mid) * B + lo;
 }
 
-template <int B>
+template <int B, bool BF>
 __global__ __launch_bounds__(256, 3) void predictor3d_tile_kernel(
     long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ lduh, double* __restrict__ lQhbnd,
@@ -1028,6 +1139,7 @@ __global__ __launch_bounds__(256, 3) void predictor3d_tile_kernel(
       aP[s] = on ? T->Ptf[ri * B + kn[s]] : 0.f;
     }
   }
+  const exa_s4 aDb = bf16_pack<NK>(aD), aPb = bf16_pack<NK>(aP);
 
   float* tA = reinterpret_cast<float*>(buf);
   float* tB = tA + TILE;
@@ -1080,14 +1192,27 @@ __global__ __launch_bounds__(256, 3) void predictor3d_tile_kernel(
       for (int q = 0; q < S::PPW; q++)
 #pragma unroll
         for (int v = 0; v < V; v++) acc[q][v] = exa_f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int s = 0; s < NK; s++)
+      if (BF) {
 #pragma unroll
         for (int q = 0; q < S::PPW; q++)
           if (32 * (wave + S::NWAVE * q) < nlines)  // wave-uniform
 #pragma unroll
-            for (int v = 0; v < V; v++)
-              acc[q][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(aD[s], F[q][s][v], acc[q][v], 0, 0, 0);
+            for (int v = 0; v < V; v++) {
+              float fv[NK];
+#pragma unroll
+              for (int s = 0; s < NK; s++) fv[s] = F[q][s][v];
+              acc[q][v] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aDb, bf16_pack<NK>(fv), acc[q][v], 0, 0, 0);
+            }
+      } else {
+#pragma unroll
+        for (int s = 0; s < NK; s++)
+#pragma unroll
+          for (int q = 0; q < S::PPW; q++)
+            if (32 * (wave + S::NWAVE * q) < nlines)  // wave-uniform
+#pragma unroll
+              for (int v = 0; v < V; v++)
+                acc[q][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(aD[s], F[q][s][v], acc[q][v], 0, 0, 0);
+      }
 #pragma unroll
       for (int q = 0; q < S::PPW; q++) {
         const int p = wave + S::NWAVE * q;
@@ -1123,12 +1248,19 @@ __global__ __launch_bounds__(256, 3) void predictor3d_tile_kernel(
     exa_f4 tacc[S::TPW];
 #pragma unroll
     for (int k = 0; k < S::TPW; k++) tacc[k] = exa_f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int s = 0; s < NK; s++)
+    if (BF) {
 #pragma unroll
       for (int k = 0; k < S::TPW; k++)
         if (wave + S::NWAVE * k < S::TPAIRS)  // wave-uniform
-          tacc[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[s], X[k][s], tacc[k], 0, 0, 0);
+          tacc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aPb, bf16_pack<NK>(X[k]), tacc[k], 0, 0, 0);
+    } else {
+#pragma unroll
+      for (int s = 0; s < NK; s++)
+#pragma unroll
+        for (int k = 0; k < S::TPW; k++)
+          if (wave + S::NWAVE * k < S::TPAIRS)  // wave-uniform
+            tacc[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[s], X[k][s], tacc[k], 0, 0, 0);
+    }
     double sq = 0.0;
 #pragma unroll
     for (int k = 0; k < S::TPW; k++) {
@@ -1188,44 +1320,73 @@ int launch_predictor3d_tile(const KernelCfg& k, hipStream_t st, long n, const do
   if (k.precision == 0) return -1;  // FP32 form only
   const int adaptive = k.picard_max <= 0;
   const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
-  hipLaunchKernelGGL((predictor3d_tile_kernel<B>), dim3(n), dim3(TShape<B>::THREADS), 0, st, n, luh, h, dt, cinst,
-                     lduh, qb, fb, pic, k.tab, maxIter, adaptive, luh_upd, f32faces ? 1 : 0);
+  if (k.precision == 2)  // BF16 operands
+    hipLaunchKernelGGL((predictor3d_tile_kernel<B, true>), dim3(n), dim3(TShape<B>::THREADS), 0, st, n, luh, h, dt,
+                       cinst, lduh, qb, fb, pic, k.tab, maxIter, adaptive, luh_upd, f32faces ? 1 : 0);
+  else
+    hipLaunchKernelGGL((predictor3d_tile_kernel<B, false>), dim3(n), dim3(TShape<B>::THREADS), 0, st, n, luh, h, dt,
+                       cinst, lduh, qb, fb, pic, k.tab, maxIter, adaptive, luh_upd, f32faces ? 1 : 0);
   return static_cast<int>(hipGetLastError());
+}
+
+// guard[0] -> guard[1] (cells recomputed in total), guard[0] = 0
+__global__ void guard_reset_kernel(int* guard) {
+  if (threadIdx.x == 0) {
+    guard[1] += guard[0];
+    guard[0] = 0;
+  }
 }
 
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const double* luh, const double* h,
                              const double* dt, double* lduh, double* qb, double* fb, int* pic, bool mfma,
-                             const int* cinst, double* luh_upd, bool f32faces) {
+                             const int* cinst, double* luh_upd, bool f32faces, int* guard) {
   constexpr int B = 6;
   using S = LShape<B>;
   const long blocks = (n + S::CPB - 1) / S::CPB;
   const int adaptive = k.picard_max <= 0;
   const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
   const int f32 = (f32faces && k.precision != 0) ? 1 : 0;  // FP32 faces exist only in the FP32 forms
+  const dim3 grid(blocks), block(S::THREADS);
+  const GuardArgs none;
   if (k.precision == 0) {
     if (k.guess) return -1;  // the FP64 line kernel has no CERK guess (the node-column kernel has)
-    hipLaunchKernelGGL((predictor3d_lines_kernel<B, double, false, false, false>), dim3(blocks), dim3(S::THREADS), 0, st, n, luh, h, dt,
-                       cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, 0);
+    hipLaunchKernelGGL((predictor3d_lines_kernel<B, double, false, false, false>), grid, block, 0, st, n, luh, h, dt,
+                       cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, 0, none);
   } else {
     if (k.guess && !(mfma && !adaptive)) return -1;  // CERK: fixed-count matrix-core form only
-    if (mfma && !adaptive && maxIter == 1 && k.guess)  // one collapsed iteration: no sweep code, smaller LDS
-      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true, true>), dim3(blocks), dim3(S::THREADS),
-                         0, st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
-    else if (mfma && !adaptive && maxIter == 1)
-      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, false, true>), dim3(blocks), dim3(S::THREADS),
-                         0, st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+    if (mfma && !adaptive && maxIter == 1 && k.guess) {  // one collapsed iteration: no sweep code, smaller LDS
+      // Picard guard: cells whose proxy residual reaches the tolerance are
+      // recomputed with the fixed count k.guard_recount (listed cells only)
+      GuardArgs g;
+      if (guard && k.guard_tol > 0) {
+        g.guard = guard;
+        g.tol2 = k.guard_tol * k.guard_tol;
+      }
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true, true>), grid, block, 0, st, n, luh, h,
+                         dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32, g);
+      if (g.guard) {
+        // persistent grid over the list: four workgroups per CU of the fixed-count kernel
+        const long lgrid = n < 1024 ? n : 1024;
+        const int rc = k.guard_recount > 0 ? k.guard_recount : (B - 3 > 1 ? B - 3 : 1);
+        hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true, false, true>), dim3(lgrid), block, 0,
+                           st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, rc, adaptive, luh_upd, f32, g);
+        hipLaunchKernelGGL(guard_reset_kernel, dim3(1), dim3(64), 0, st, guard);
+      }
+    } else if (mfma && !adaptive && maxIter == 1)
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, false, true>), grid, block, 0, st, n, luh, h,
+                         dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32, none);
     else if (mfma && !adaptive && k.guess)
-      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true>), dim3(blocks), dim3(S::THREADS), 0, st,
-                         n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true>), grid, block, 0, st, n, luh, h, dt,
+                         cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32, none);
     else if (mfma && !adaptive)
-      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, false>), dim3(blocks), dim3(S::THREADS), 0,
-                         st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, false>), grid, block, 0, st, n, luh, h, dt,
+                         cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32, none);
     else if (mfma)
-      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, false, false>), dim3(blocks), dim3(S::THREADS), 0,
-                         st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, false, false>), grid, block, 0, st, n, luh, h, dt,
+                         cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32, none);
     else
-      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, false, false, false>), dim3(blocks), dim3(S::THREADS), 0,
-                         st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32);
+      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, false, false, false>), grid, block, 0, st, n, luh, h, dt,
+                         cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32, none);
   }
   return static_cast<int>(hipGetLastError());
 }
@@ -1233,6 +1394,11 @@ int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const d
 bool line_kernels_serve(const KernelCfg& k) {
   if (!lines_supported(k) || k.dialect != 0) return false;
   return !k.guess || (k.precision != 0 && k.picard_max > 0 && (k.pred_variant == 0 || k.pred_variant == 3));
+}
+
+bool predictor_uses_guard(const KernelCfg& k) {
+  return line_kernels_serve(k) && k.precision == 1 && k.picard_max == 1 && k.guess &&
+         (k.pred_variant == 0 || k.pred_variant == 3) && k.guard_tol > 0;
 }
 
 bool predictor_f32_faces_supported(const KernelCfg& k) {

@@ -19,7 +19,7 @@ REPO_DIR = os.path.dirname(ENGINE_DIR)
 LIB_PATH = os.path.join(ENGINE_DIR, "libexahype_amd.so")
 HEADER_PATH = os.path.join(REPO_DIR, "include", "exahype_amd.h")
 
-PDE_EULER, PDE_ACOUSTIC, PDE_ADVECTION = 0, 1, 2
+PDE_EULER, PDE_ACOUSTIC, PDE_ADVECTION, PDE_USER = 0, 1, 2, 3
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 DIALECT_GENERIC, DIALECT_OPTIMISED, DIALECT_OPTIMISED_CERK = 0, 1, 2
 GUESS_TRIVIAL, GUESS_CERK = 0, 1
@@ -55,10 +55,12 @@ _sig("exa_ctx_create", [ctypes.POINTER(_cp), ctypes.c_int, ctypes.c_int, ctypes.
                         ctypes.c_int, ctypes.c_int, ctypes.c_double])
 _sig("exa_ctx_destroy", [_cp])
 _sig("exa_ctx_set_advection", [_cp, _dp])
+_sig("exa_ctx_set_pde_params", [_cp, _dp, ctypes.c_int])
 _sig("exa_ctx_set_point_source", [_cp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double])
 _sig("exa_ctx_set_predictor_variant", [_cp, ctypes.c_int])
 _sig("exa_ctx_set_dialect", [_cp, ctypes.c_int, ctypes.c_int])
 _sig("exa_ctx_set_initial_guess", [_cp, ctypes.c_int])
+_sig("exa_ctx_set_picard_guard", [_cp, ctypes.c_double, ctypes.c_int])
 _sig("exa_ctx_padded_size", [_cp, ctypes.c_int, _ip])
 _sig("exa_opt_fusedSpaceTimePredictorVolumeIntegral",
      [_cp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _ip])
@@ -113,6 +115,7 @@ _sig("exa_mesh_run", [_cp, ctypes.c_int, ctypes.c_int])
 _sig("exa_mesh_sync", [_cp])
 _sig("exa_mesh_time", [_cp, _dp, _dp])
 _sig("exa_mesh_picard", [_cp, _ip])
+_sig("exa_mesh_picard_guard", [_cp, ctypes.POINTER(ctypes.c_long)])
 _sig("exa_mesh_stream", [_cp], ctypes.c_void_p)
 _sig("exa_mesh_set_timing", [_cp, ctypes.c_int])
 _sig("exa_mesh_set_fused", [_cp, ctypes.c_int])
@@ -247,6 +250,10 @@ class Context:
         """GUESS_TRIVIAL (luh at every time node) or GUESS_CERK."""
         _check(_lib.exa_ctx_set_initial_guess(self.h, guess))
 
+    def set_picard_guard(self, tol=1e-6, recompute_count=0):
+        """Relative proxy-residual guard of the FP32 one-iteration CERK form (tol <= 0: off)."""
+        _check(_lib.exa_ctx_set_picard_guard(self.h, tol, recompute_count))
+
     def padded_size(self, which):
         """nVarPad (which = 0) or nDataPad (which = 1)."""
         n = ctypes.c_int(0)
@@ -277,6 +284,11 @@ class Context:
     def set_point_source(self, x, amp, f0, t0):
         xs = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
         _check(_lib.exa_ctx_set_point_source(self.h, _p(xs), amp, f0, t0))
+
+    def set_pde_params(self, prm):
+        """Parameters of the user PDE hooks (at most 16)."""
+        p = np.ascontiguousarray(prm, dtype=np.float64)
+        _check(_lib.exa_ctx_set_pde_params(self.h, _p(p), p.size))
 
     def set_advection(self, a):
         _check(_lib.exa_ctx_set_advection(self.h, _p(np.ascontiguousarray(a, dtype=np.float64))))
@@ -520,6 +532,12 @@ class Mesh:
         out = np.zeros(self.ncells, dtype=np.int32)
         _check(_lib.exa_mesh_picard(self.h, out.ctypes.data_as(_ip)))
         return out
+
+    def picard_guard_recomputed(self):
+        """Cells the Picard guard recomputed since the mesh was created."""
+        v = ctypes.c_long(0)
+        _check(_lib.exa_mesh_picard_guard(self.h, ctypes.byref(v)))
+        return v.value
 
     @property
     def stream(self):

@@ -44,7 +44,12 @@ enum exa_pde {
   EXA_PDE_ACOUSTIC = 1,   /* linear acoustics, 3D: p,u,v,w; parameters rho, c
                              (ApplicationExamples/Linear/LinearWave3D/MyLinearWaveSolver.cpp:108-209);
                              linear (Cauchy-Kowalevski) kernels, exa_acoustic_* below */
-  EXA_PDE_ADVECTION = 2   /* linear advection F_d = a_d Q, eigenvalues a_d (test PDE)          */
+  EXA_PDE_ADVECTION = 2,  /* linear advection F_d = a_d Q, eigenvalues a_d (test PDE)          */
+  EXA_PDE_USER = 3        /* the device hooks of struct ExaUserPde (include/exahype_amd_pde.h:
+                             flux, eigenvalues, nonConservativeProduct, algebraicSource),
+                             compiled into the library; 5 variables, no parameters.  With
+                             a source or NCP: FP64, trivial guess, the four-kernel step,
+                             periodic / exchange domain faces */
 };
 
 /* Arithmetic of the space-time predictor (Picard loop, time averages,
@@ -53,7 +58,10 @@ enum exa_pde {
 enum exa_precision {
   EXA_PREC_FP64 = 0,
   EXA_PREC_FP32 = 1,       /* FP32 contractions (MFMA where compiled), FP32 accumulation */
-  EXA_PREC_BF16 = 2        /* experiment: BF16 operands, FP32 accumulation                */
+  EXA_PREC_BF16 = 2        /* experiment: the contractions' operands (operator entries,
+                              flux and divergence values) rounded to BF16 on the BF16
+                              matrix cores, FP32 accumulation, FP32 iterate; 3D Euler
+                              order 5 with a fixed Picard count and the trivial guess */
 };
 
 typedef struct exa_ctx exa_ctx;
@@ -85,6 +93,10 @@ int exa_ctx_set_point_source(exa_ctx* ctx, const double* x, double amp, double f
 
 /* Advection velocity for EXA_PDE_ADVECTION (a[dim]). */
 int exa_ctx_set_advection(exa_ctx* ctx, const double* a);
+/* Parameters of the EXA_PDE_USER hooks (prm[0..n), n <= 16, the rest zero);
+ * the hooks read them on the device.  Replaces the constants a user solver
+ * keeps as members (AbstractGenericADERDGSolverHeader.template). */
+int exa_ctx_set_pde_params(exa_ctx* ctx, const double* prm, int n);
 
 /* Predictor kernel selection: 0 (default) picks the fastest kernel compiled
  * for the shape, 1 forces the generic node-column kernel (every shape); for
@@ -127,6 +139,25 @@ int exa_ctx_set_dialect(exa_ctx* ctx, int dialect, int vect_size);
  * order 5 in FP32 with a fixed Picard count on the x-line matrix-core kernel. */
 enum exa_guess { EXA_GUESS_TRIVIAL = 0, EXA_GUESS_CERK = 1 };
 int exa_ctx_set_initial_guess(exa_ctx* ctx, int guess);
+
+/* Picard guard of the FP32 one-iteration CERK form (FP32, fixed count 1,
+ * EXA_GUESS_CERK; 3D Euler order 5).  That form computes no full residual: the
+ * last iteration is collapsed into its time average.  Per cell it forms, with
+ * FP64 accumulation, the relative proxy residual
+ *   r = |lQhi(1) - lQhi(0)| / |lQhi(1)|   (2-norms over nodes and variables),
+ * lQhi(k) = sum_t w_t lQi(k)(t) the time average of iterate k (lQi(0) the CERK
+ * guess), and recomputes a cell with r >= tol with recompute_count fixed
+ * iterations (0: max(nDof - 3, 1), the optimised kernels' CERK cap,
+ * fusedSPTVI_nonlinear_cpp.template:177); that cell reports the count, the
+ * others keep one.  The reference's rule is absolute, |lQi(1) - lQi(0)| < 1e-7
+ * (spaceTimePredictorNonlinear.cpph:370-423); FP32 iterates cannot resolve it
+ * (the flux divergences carry ~1e-8 relative rounding, ~1e-6 absolute per
+ * cell), so the guard bounds the first iteration's relative change instead:
+ * stopping there leaves an error of about the contraction factor times r.
+ * Default: on, tol = 1e-6 (the FP32 form's accuracy target), recompute_count 0;
+ * tol <= 0 switches it off.  No counterpart in the reference.  Meshes copy the
+ * setting at creation. */
+int exa_ctx_set_picard_guard(exa_ctx* ctx, double tol, int recompute_count);
 /* *size = nVarPad (which = 0) or nDataPad (which = 1) of the context's dialect. */
 int exa_ctx_padded_size(const exa_ctx* ctx, int which, int* size);
 
@@ -269,7 +300,10 @@ int exa_stableTimeStepSize(exa_ctx* ctx, const double* luh, const double* dx, do
  * ------------------------------------------------------------------------- */
 
 /* Predictor + volume integral for ncells cells (reference row: a4 + a5).
- * d_picard (nullable): per-cell Picard count. */
+ * d_picard (nullable): per-cell Picard count.  With the Picard guard of the
+ * FP32 one-iteration CERK form (exa_ctx_set_picard_guard) the cell list of
+ * the recomputation lives in a context buffer that the first call of a given
+ * size allocates. */
 int exa_batch_predictor(exa_ctx* ctx, void* stream, long ncells, const double* d_luh, const double* d_h,
                         const double* d_dt, double* d_lduh, double* d_lQhbnd, double* d_lFhbnd, int* d_picard);
 
@@ -327,6 +361,9 @@ int exa_mesh_sync(exa_mesh* mesh);
 /* Current time, dt of the next step, per-cell Picard counts of the last step. */
 int exa_mesh_time(exa_mesh* mesh, double* t, double* dt);
 int exa_mesh_picard(exa_mesh* mesh, int* counts);
+/* Cells the Picard guard recomputed since the mesh was created (see
+ * exa_ctx_set_picard_guard; no counterpart in the reference). */
+int exa_mesh_picard_guard(exa_mesh* mesh, long* recomputed);
 /* Step form: enabled != 0 (default where compiled) fuses the step into two
  * kernels -- the predictor adds the volume part of the update to luh, then one
  * kernel solves every cell's face Riemann problems from both sides' data and

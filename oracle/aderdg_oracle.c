@@ -30,7 +30,7 @@
  * Linear advection (ORC_PDE_ADVECTION) is a test PDE: F_d = a_d Q. */
 void orc_flux(const orc_solver* s, const double* Q, double* F) {
   const int V = s->nvar;
-  if (s->pde == ORC_PDE_ADVECTION) {
+  if (s->pde == ORC_PDE_ADVECTION || s->pde == ORC_PDE_ADV_NCP_SRC) {
     for (int d = 0; d < s->dim; d++)
       for (int v = 0; v < V; v++) F[d * V + v] = s->adv[d] * Q[v];
     return;
@@ -69,6 +69,10 @@ void orc_eigenvalues(const orc_solver* s, const double* Q, int d, double* lambda
     for (int v = 0; v < V; v++) lambda[v] = s->adv[d];
     return;
   }
+  if (s->pde == ORC_PDE_ADV_NCP_SRC) {  /* eigenvalues of a_d + b_d (flux and NCP together) */
+    for (int v = 0; v < V; v++) lambda[v] = s->adv[d] + s->ncp_b[d];
+    return;
+  }
   const double gamma = GAMMA;
   const double irho = 1. / Q[0];
   const double j2 = (s->dim == 3) ? Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]
@@ -79,6 +83,33 @@ void orc_eigenvalues(const orc_solver* s, const double* Q, int d, double* lambda
   for (int v = 0; v < 5; v++) lambda[v] = u_n;
   lambda[0] -= c;
   lambda[4] += c;
+}
+
+/* Test PDE ORC_PDE_ADV_NCP_SRC: BgradQ_v = sum_d b_d dQ_v/dx_d, S_v = -kappa Q_v. */
+void orc_ncp(const orc_solver* s, const double* Q, const double* gradQ, double* BgradQ) {
+  const int V = s->nvar;
+  (void)Q;
+  for (int v = 0; v < V; v++) {
+    double a = 0.0;
+    for (int d = 0; d < s->dim; d++) a += s->ncp_b[d] * gradQ[d * V + v];
+    BgradQ[v] = a;
+  }
+}
+
+void orc_source(const orc_solver* s, const double* Q, double* S) {
+  for (int v = 0; v < s->nvar; v++) S[v] = -s->kappa * Q[v];
+}
+
+/* Source and NCP terms of the Picard loop at one space-time point
+ * (gc3/spaceTimePredictorNonlinear.cpph:323-367, gc2 :211-243): S =
+ * algebraicSource(Q) - nonConservativeProduct(Q, gradQ). */
+static int has_ncs(const orc_solver* s) { return s->pde == ORC_PDE_ADV_NCP_SRC; }
+
+static void ncs_term(const orc_solver* s, const double* Q, const double* gradQ, double* S) {
+  double ncp[64];
+  orc_source(s, Q, S);
+  orc_ncp(s, Q, gradQ, ncp);
+  for (int v = 0; v < s->nvar; v++) S[v] -= ncp[v];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -227,6 +258,33 @@ static int picard3(const orc_solver* s, const double* luh, const double* invDx, 
               for (int n = 0; n < B; n++)
                 RHS(i, l, j, k, m) -= updateSize * LFI(i, n, j, k, 2, m) * Kxi[n * B + l];
         }
+      if (has_ncs(s)) {
+        /* gradQCur of time node i (:113-176): invDx_d lQi dudx along x, y, z;
+         * then S = source - ncp into slot 3 of lFi and w^4 dt S into rhs (:323-367) */
+        double gq[3 * 64], S[64];
+        for (int j = 0; j < B; j++)
+          for (int k = 0; k < B; k++)
+            for (int l = 0; l < B; l++) {
+              for (int m = 0; m < V; m++) {
+                double gx = 0.0, gy = 0.0, gz = 0.0;
+                for (int n = 0; n < B; n++) {
+                  gx += 1.0 * invDx[0] * LQI(j, k, n, i, m) * s->b.dudx[l * B + n];
+                  gy += 1.0 * invDx[1] * LQI(j, n, l, i, m) * s->b.dudx[k * B + n];
+                  gz += 1.0 * invDx[2] * LQI(n, k, l, i, m) * s->b.dudx[j * B + n];
+                }
+                gq[0 * V + m] = gx;
+                gq[1 * V + m] = gy;
+                gq[2 * V + m] = gz;
+              }
+              ncs_term(s, &LQI(j, k, l, i, 0), gq, S);
+              const double weight = w[i] * w[j] * w[k] * w[l];
+              const double updateSize = weight * dt;
+              for (int m = 0; m < V; m++) {
+                LFI(i, j, k, l, 3, m) = S[m];
+                RHS(i, j, k, l, m) += updateSize * S[m];
+              }
+            }
+      }
     }
     /* 3. multiply with iK1 (:370-405) */
     double sq_res = 0.0;
@@ -289,8 +347,10 @@ static void predictor_extrapolator3(const orc_solver* s, const double* lQi, cons
           }
         for (int l = 0; l < ND; l++)
           for (int m = 0; m < B; m++) lQhi[IQH(i, j, k, l)] += LQI(i, j, k, m, l) * w[m];
+        if (has_ncs(s))  /* lShi (:494-499) */
+          for (int l = 0; l < V; l++)
+            for (int m = 0; m < B; m++) lShi[IFH(i, j, k, l)] += LFI(m, i, j, k, 3, l) * w[m];
       }
-  (void)lShi;
   memset(lQhbnd, 0, sizeof(double) * 2 * D * B2 * ND);
   memset(lFhbnd, 0, sizeof(double) * 2 * D * B2 * V);
   for (int i = 0; i < B; i++)
@@ -401,6 +461,29 @@ static int picard2(const orc_solver* s, const double* luh, const double* invDx, 
             for (int n = 0; n < B; n++)
               RHS(i, l, k, m) -= updateSize * LFI(i, n, k, 1, m) * Kxi[n * B + l];
       }
+      if (has_ncs(s)) {
+        /* gradQCur of time node i (:99-135), then S = source - ncp (:211-243) */
+        double gq[2 * 64], S[64];
+        for (int k = 0; k < B; k++)
+          for (int l = 0; l < B; l++) {
+            for (int m = 0; m < V; m++) {
+              double gx = 0.0, gy = 0.0;
+              for (int n = 0; n < B; n++) {
+                gx += 1.0 * invDx[0] * LQI(k, n, i, m) * s->b.dudx[l * B + n];
+                gy += 1.0 * invDx[1] * LQI(n, l, i, m) * s->b.dudx[k * B + n];
+              }
+              gq[0 * V + m] = gx;
+              gq[1 * V + m] = gy;
+            }
+            ncs_term(s, &LQI(k, l, i, 0), gq, S);
+            const double weight = w[i] * w[k] * w[l];
+            const double updateSize = weight * dt;
+            for (int m = 0; m < V; m++) {
+              LFI(i, k, l, 2, m) = S[m];
+              RHS(i, k, l, m) += updateSize * S[m];
+            }
+          }
+      }
     }
     /* iK1 solve and residual (:230-248) */
     double sq_res = 0.0;
@@ -464,6 +547,9 @@ static void predictor_extrapolator2(const orc_solver* s, const double* lQi, cons
         }
       for (int l = 0; l < ND; l++)
         for (int m = 0; m < B; m++) lQhi[IQH(j, k, l)] += LQI(j, k, m, l) * w[m];
+      if (has_ncs(s))  /* lShi (gc2 :370-374) */
+        for (int l = 0; l < V; l++)
+          for (int m = 0; m < B; m++) lFhi[2 * B2 * V + IFH(j, k, l)] += LFI(m, j, k, 2, l) * w[m];
     }
   memset(lQhbnd, 0, sizeof(double) * 2 * D * B * ND);
   memset(lFhbnd, 0, sizeof(double) * 2 * D * B * V);
@@ -554,6 +640,15 @@ void orc_volume_integral_nonlinear(const orc_solver* s, double* lduh, const doub
             for (int m = 0; m < B; m++)
               lduh[I4(k, i, j, l)] += Kxi[k * B + m] * fz[I4(i, j, m, l)] * updateSize;
       }
+    if (has_ncs(s)) {  /* lduh += w lShi (:204-222) */
+      const double* sh = lFhi + 3 * B3 * V;
+      for (int i = 0; i < B; i++)
+        for (int j = 0; j < B; j++)
+          for (int k = 0; k < B; k++) {
+            const double weight = w[i] * w[j] * w[k];
+            for (int l = 0; l < V; l++) lduh[I4(i, j, k, l)] += weight * sh[I4(i, j, k, l)];
+          }
+    }
 #undef I4
   } else {
     const int B2 = B * B;
@@ -573,6 +668,14 @@ void orc_volume_integral_nonlinear(const orc_solver* s, double* lduh, const doub
       for (int k = 0; k < B; k++)
         for (int l = 0; l < V; l++)
           for (int m = 0; m < B; m++) lduh[I3(k, j, l)] += Kxi[k * B + m] * fy[I3(j, m, l)] * updateSize;
+    }
+    if (has_ncs(s)) {  /* lduh += w lShi (gc2 volumeIntegralNonlinear) */
+      const double* sh = lFhi + 2 * B2 * V;
+      for (int j = 0; j < B; j++)
+        for (int k = 0; k < B; k++) {
+          const double weight = w[j] * w[k];
+          for (int l = 0; l < V; l++) lduh[I3(j, k, l)] += weight * sh[I3(j, k, l)];
+        }
     }
 #undef I3
   }
@@ -614,12 +717,27 @@ void orc_riemann_nonlinear(const orc_solver* s, double* FL, double* FR, const do
     if (fabs(LR[k]) > mR) mR = fabs(LR[k]);
   }
   const double smax = mL > mR ? mL : mR;
-  for (int n = 0; n < nFace; n++)
+  for (int n = 0; n < nFace; n++) {
+    double ncp[64] = {0};
+    if (has_ncs(s)) {
+      /* fluctuation (riemannSolverNonlinear.cpph:95-120): gradQ[direction] = QR - QL,
+       * the NCP of the averaged state; FR = F - ncp/2, FL = F + ncp/2 */
+      double gq[3 * 64] = {0}, Qavg[64];
+      for (int l = 0; l < V; l++) gq[direction * V + l] = QR[n * ND + l] - QL[n * ND + l];
+      for (int l = 0; l < ND; l++) Qavg[l] = 0.5 * (QR[n * ND + l] + QL[n * ND + l]);
+      orc_ncp(s, Qavg, gq, ncp);
+    }
     for (int k = 0; k < V; k++) {
       const double f = 0.5 * (FR[n * V + k] + FL[n * V + k]) - 0.5 * smax * (QR[n * ND + k] - QL[n * ND + k]);
-      FL[n * V + k] = f;
-      FR[n * V + k] = f;
+      if (has_ncs(s)) {
+        FR[n * V + k] = f - 0.5 * ncp[k];
+        FL[n * V + k] = f + 0.5 * ncp[k];
+      } else {
+        FL[n * V + k] = f;
+        FR[n * V + k] = f;
+      }
     }
+  }
 }
 
 /* ------------------------------------------------------------------------ */

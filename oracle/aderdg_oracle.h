@@ -93,7 +93,8 @@ typedef struct orc_basis {
   double fgp[3 * ORC_MAXB * ORC_MAXB];/* fineGridProjector[order][sub][i][j] */
 } orc_basis;
 
-enum { ORC_PDE_EULER = 0, ORC_PDE_ACOUSTIC = 1, ORC_PDE_ADVECTION = 2, ORC_PDE_ELASTIC2D = 3 };
+enum { ORC_PDE_EULER = 0, ORC_PDE_ACOUSTIC = 1, ORC_PDE_ADVECTION = 2, ORC_PDE_ELASTIC2D = 3,
+       ORC_PDE_ADV_NCP_SRC = 4 };
 
 typedef struct orc_solver {
   int dim;            /* 2 or 3 */
@@ -113,11 +114,21 @@ typedef struct orc_solver {
    * exit always on (:307-309), return min(iter + 1, MaxIterations) (:607). */
   int cerk_guess;
   int opt_iterations;
+  /* ORC_PDE_ADV_NCP_SRC (test PDE with a non-conservative product and an
+   * algebraic source, dQ/dt + div(a Q) + b . grad Q = -kappa Q per variable):
+   * flux a Q (adv), ncp b . grad Q (ncp_b), source -kappa Q (kappa) */
+  double ncp_b[3];
+  double kappa;
 } orc_solver;
 
 /* ---- PDE hooks ---------------------------------------------------------- */
 void orc_flux(const orc_solver* s, const double* Q, double* F /* [dim][nvar] */);
 void orc_eigenvalues(const orc_solver* s, const double* Q, int d, double* lambda);
+/* nonConservativeProduct(Q, gradQ[dim][nvar], BgradQ) and algebraicSource(Q, S)
+ * of ORC_PDE_ADV_NCP_SRC (the hooks the predictor calls at :349-362 and the
+ * Rusanov solver at riemannSolverNonlinear.cpph:98-107) */
+void orc_ncp(const orc_solver* s, const double* Q, const double* gradQ, double* BgradQ);
+void orc_source(const orc_solver* s, const double* Q, double* S);
 
 /* ---- nonlinear cell kernels (reference generic c/{2d,3d}) --------------- */
 int orc_stp_nonlinear(const orc_solver* s, const double* luh, const double* invDx,

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "liborc.so")
 TABLES_PATH = os.path.join(ROOT, "tests", "golden", "legendre_tables.json")
 
 MAXB = 16
-PDE_EULER, PDE_ACOUSTIC, PDE_ADVECTION, PDE_ELASTIC2D = 0, 1, 2, 3
+PDE_EULER, PDE_ACOUSTIC, PDE_ADVECTION, PDE_ELASTIC2D, PDE_ADV_NCP_SRC = 0, 1, 2, 3, 4
 LIN_FLUX, LIN_NCP, LIN_MM = 1, 2, 4
 
 # PNPM per order (Toolkit/exahype/toolkit/solverController.py:17-23).
@@ -53,6 +53,8 @@ class Solver(ctypes.Structure):
         ("b", Basis),
         ("cerk_guess", ctypes.c_int),
         ("opt_iterations", ctypes.c_int),
+        ("ncp_b", ctypes.c_double * 3),
+        ("kappa", ctypes.c_double),
     ]
 
 
@@ -162,7 +164,8 @@ class Oracle:
     """One solver configuration (dimension, order, variables, PDE)."""
 
     def __init__(self, dim, order, nvar=5, npar=0, pde=PDE_EULER, cfl=0.9, pnpm=None,
-                 max_picard=None, adv=(1.0, 0.5, 0.25), table_override=None, dialect=None):
+                 max_picard=None, adv=(1.0, 0.5, 0.25), table_override=None, dialect=None,
+                 ncp_b=(0.0, 0.0, 0.0), kappa=0.0):
         """dialect: None (generic kernels), "cerk" (generic Picard rule with the
         CERK initial guess) or "optimised" / "optimised-cerk" (the
         KernelGenerator kernels' Picard rule, trivial or CERK initial guess)."""
@@ -178,6 +181,8 @@ class Oracle:
         s.pnpm = PNPM[order] if pnpm is None else pnpm
         for i in range(3):
             s.adv[i] = adv[i]
+            s.ncp_b[i] = ncp_b[i]
+        s.kappa = kappa
         t = dict(tables(order))
         if table_override:
             t.update(table_override)

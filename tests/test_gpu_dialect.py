@@ -216,6 +216,7 @@ def test_fp32_line_kernel_with_cerk_guess(picard):
         ctx = E.Context(3, order, 5, 0, precision=E.PREC_FP32, picard_max=picard)
         ctx.set_predictor_variant(variant)
         ctx.set_initial_guess(E.GUESS_CERK)
+        ctx.set_picard_guard(0.0)  # both kernels at the same fixed count (the guard has its own tests)
         mesh = E.Mesh(ctx, n, 1.0 / n)
         mesh.set_solution(luh)
         mesh.init_dt(0.0)
@@ -285,6 +286,7 @@ def test_fp32_cerk_line_kernel_with_walls_agrees_with_node_kernel():
         ctx = E.Context(3, order, 5, 0, precision=E.PREC_FP32, picard_max=1)
         ctx.set_predictor_variant(variant)
         ctx.set_initial_guess(E.GUESS_CERK)
+        ctx.set_picard_guard(0.0)  # both kernels at one fixed iteration
         mesh = E.Mesh(ctx, n, 1.0 / n)
         mesh.set_boundary(bc)
         mesh.set_solution(luh)

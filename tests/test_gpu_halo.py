@@ -126,3 +126,17 @@ def test_halves_match_the_oracle_box(D, order, n, walls):
     ctx.close()
     assert rel_l2(a, ref[:n]) <= 1e-12 and rel_l2(b, ref[n:]) <= 1e-12
     assert abs(t[0][0] - t_ref) <= 1e-14
+
+
+def test_halo_pack_rejects_fp32_faces():
+    """The all-periodic FP32 step stores its faces as FP32 [face][var][node]:
+    packing FP64 messages from them is refused, not read as FP64."""
+    ctx = E.Context(3, 5, precision=E.PREC_FP32, picard_max=3)
+    m = E.Mesh(ctx, 3, 1.0 / 3)
+    m.set_solution(P.density_wave_3d(3, 5))
+    m.init_dt(0.0)
+    m.step_predict()
+    with pytest.raises(E.ExaError):
+        m.halo_pack_host(0)
+    m.close()
+    ctx.close()
