@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cfloat>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -76,6 +77,47 @@ int ensure_scratch(exa_ctx* c, size_t bytes) {
   HIP_OK(hipMalloc(&c->d_scratch, bytes), "hipMalloc(scratch)");
   c->scratch_bytes = bytes;
   return EXA_OK;
+}
+
+// Picard-guard list of exa_batch_predictor_noalloc, in device memory that no
+// call allocates: {listed this call, listed in total, list[kNoAllocMaxCells]}
+// (4 MiB).  Each GPU the library runs on holds its own copy until the process
+// ends; host code uses the per-device address from hipGetSymbolAddress, never
+// the array's name.
+constexpr long kNoAllocMaxCells = 1L << 20;
+__device__ __align__(256) int scratch[2 + kNoAllocMaxCells];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+int* g_scratch[kMaxDevices] = {};
+
+// Device address of the scratch array on the device of `stream` (the current
+// device for the null stream), looked up once per device with that device current.
+int scratch_for_stream(hipStream_t stream, int** out) {
+  int cur = 0;
+  HIP_OK(hipGetDevice(&cur), "hipGetDevice");
+  int dev = cur;
+  if (stream) HIP_OK(hipStreamGetDevice(stream, &dev), "hipStreamGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("exa_batch_predictor_noalloc: device index beyond the library's table");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    if (dev != cur) HIP_OK(hipSetDevice(dev), "hipSetDevice");
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (dev != cur) HIP_OK(hipSetDevice(cur), "hipSetDevice");
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch[dev] = static_cast<int*>(p);
+  }
+  *out = g_scratch[dev];
+  return EXA_OK;
+}
+
+// Clears the list length at the start of a call (a kernel, so that a captured
+// call records no memset node).
+__global__ void guard_clear_kernel(int* guard) {
+  if (threadIdx.x == 0) guard[0] = 0;
 }
 
 // Guard buffer for ncells cells (grown on demand, counters cleared when allocated).
@@ -1075,6 +1117,31 @@ int exa_batch_predictor(exa_ctx* c, void* stream, long ncells, const double* d_l
   return launch_status(exa::launch_predictor(c->k, static_cast<hipStream_t>(stream), ncells, d_luh, d_h, d_dt,
                                              d_lduh, d_lQhbnd, d_lFhbnd, d_picard, o),
                        "exa_batch_predictor");
+}
+
+int exa_batch_predictor_noalloc(exa_ctx* c, void* stream, long ncells, const double* d_luh, const double* d_h,
+                                const double* d_dt, double* d_lduh, double* d_lQhbnd, double* d_lFhbnd,
+                                int* d_picard) {
+  if (!c || ncells < 0 || (ncells > 0 && (!d_luh || !d_h || !d_dt || !d_lduh || !d_lQhbnd || !d_lFhbnd))) {
+    set_error("exa_batch_predictor_noalloc: invalid argument");
+    return EXA_ERR_ARG;
+  }
+  if (ncells > kNoAllocMaxCells) {
+    set_error("exa_batch_predictor_noalloc: more than 1048576 cells (the static guard list's size)");
+    return EXA_ERR_ARG;
+  }
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  exa::LaunchOpts o;
+  if (ncells > 0 && exa::predictor_uses_guard(c->k)) {
+    int* g = nullptr;
+    const int rc = scratch_for_stream(st, &g);
+    if (rc) return rc;
+    hipLaunchKernelGGL(guard_clear_kernel, dim3(1), dim3(64), 0, st, g);
+    o.guard = g;
+  }
+  return launch_status(exa::launch_predictor(c->k, st, ncells, d_luh, d_h, d_dt, d_lduh, d_lQhbnd, d_lFhbnd,
+                                             d_picard, o),
+                       "exa_batch_predictor_noalloc");
 }
 
 int exa_batch_riemann(exa_ctx* c, void* stream, long nfaces, const int* d_faces, const double* d_lQhbnd,

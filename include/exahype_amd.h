@@ -307,6 +307,17 @@ int exa_stableTimeStepSize(exa_ctx* ctx, const double* luh, const double* dx, do
 int exa_batch_predictor(exa_ctx* ctx, void* stream, long ncells, const double* d_luh, const double* d_h,
                         const double* d_dt, double* d_lduh, double* d_lQhbnd, double* d_lFhbnd, int* d_picard);
 
+/* exa_batch_predictor without allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels and results as exa_batch_predictor; the Picard guard's cell list is
+ * a static device array of 2 + 1048576 ints (4 MiB per GPU, held until the
+ * process ends), so ncells > 1048576 is EXA_ERR_ARG.  Calls on one GPU share
+ * that array: calls that overlap on one GPU (concurrent streams) race. */
+int exa_batch_predictor_noalloc(exa_ctx* ctx, void* stream, long ncells, const double* d_luh, const double* d_h,
+                                const double* d_dt, double* d_lduh, double* d_lQhbnd, double* d_lFhbnd,
+                                int* d_picard);
+
 /* Rusanov flux on nfaces faces.  d_faces: int[4] per face = {left cell,
  * right cell, direction, unused}; reads left face 2*dir+1 and right face
  * 2*dir of lQhbnd/lFhbnd and writes the flux into both lFhbnd faces. */
