@@ -454,7 +454,8 @@ def main():
         k32 = args.picard_fp32 if args.picard_fp32 > 0 else 3
         for prec, name, pic in ((E.PREC_FP64, "fp64", 0), (E.PREC_FP32, "fp32", k32)):
             for variant, vname in ((1, "node-column"), (2, "line"), (3, "line+mfma"), (4, "tile+mfma")):
-                if (prec == E.PREC_FP64 and variant >= 3) or (args.variant is not None and variant != args.variant):
+                if (prec == E.PREC_FP64 and variant in (3, 4)) or \
+                        (args.variant is not None and variant != args.variant):
                     continue
                 r = run_form(E, P, args.n, args.order, args.steps, args.warmup, prec, pic, args.seed, args.modes, variant,
                              guess=1 if args.guess == "cerk" else 0)

@@ -112,6 +112,113 @@ __device__ __forceinline__ int stride_of(int d) {
   return d == 0 ? 1 : (d == 1 ? B : B * B);
 }
 
+// Volume integral (volumeIntegralNonlinear.cpph:133-223; lduh zeroed, then x, y,
+// z contributions) and extrapolation (spaceTimePredictorNonlinear.cpph:517-687)
+// of node s of one cell from its time averages in LDS: Qh [node][ND], Fh
+// [d][node][V] (FP64).  du_add (nullable): W-weighted extra term of the node
+// (the time-averaged source).  Shared by the node-column and slab predictors.
+template <int D, int B, int V, int NP>
+__device__ __forceinline__ void node_volume_and_faces(
+    const double* Qh, const double* Fh, int s, long cell, const double (&ta)[D], const double (&tb)[D],
+    const int (&cc)[D], const int (&base)[D], const double (&invDx)[D], double W, double dt, int iters,
+    const double* du_add, const DevTables* __restrict__ T, double* __restrict__ lduh, double* __restrict__ lQhbnd,
+    double* __restrict__ lFhbnd, int* __restrict__ picard, double* luh_upd) {
+  using S = Shape<D, B, V, NP>;
+  constexpr int ND = S::ND, NS = S::NS, NF = S::NF;
+  // ---- volume integral (lduh zeroed, then x, y, z contributions) ----
+  {
+    double du[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) du[v] = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const double upd = (D == 3 ? ta[d] * tb[d] : ta[d]) * invDx[d];
+      const int c = cc[d];
+#pragma unroll
+      for (int v = 0; v < V; v++)
+#pragma unroll
+        for (int m = 0; m < B; m++)
+          du[v] += T->Kxi[c * B + m] * Fh[(d * NS + base[d] + m * stride_of<D, B>(d)) * V + v] * upd;
+    }
+    if (du_add) {  // source / NCP: lduh += W lShi (volumeIntegralNonlinear.cpph:204-222)
+#pragma unroll
+      for (int v = 0; v < V; v++) du[v] += W * du_add[v];
+    }
+    if (luh_upd) {
+      // volume part of the solution update in place (solutionUpdate.cpph:28-67: dt / w)
+      const double upd = dt / W;
+#pragma unroll
+      for (int v = 0; v < V; v++) luh_upd[(cell * NS + s) * ND + v] += du[v] * upd;
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; v++) lduh[(cell * NS + s) * V + v] = du[v];
+    }
+  }
+
+  // ---- extrapolation to the 2D faces (faces zeroed, then FCoeff sums) ----
+  {
+    constexpr int NQ = 2 * D * NF * ND;
+    for (int o = s; o < NQ; o += NS) {
+      const int v = o % ND;
+      const int fn = (o / ND) % NF;
+      const int f = o / (ND * NF);
+      const int d = f >> 1;
+      const double* FC = (f & 1) ? T->F1 : T->F0;
+      // node of the face line at parameter l
+      int a = fn / B, b = fn % B;
+      if (D == 2) { a = 0; b = fn; }
+      double acc = 0.0;
+      const bool skip = (D == 2) && (d == 0) && (v >= V);  // 2D x-faces carry no parameters
+      if (!skip) {
+#pragma unroll
+        for (int l = 0; l < B; l++) {
+          int nz, ny, nx;
+          if (D == 3) {
+            if (d == 0) { nz = a; ny = b; nx = l; }
+            else if (d == 1) { nz = a; ny = l; nx = b; }
+            else { nz = l; ny = a; nx = b; }
+          } else {
+            nz = 0;
+            if (d == 0) { ny = b; nx = l; }
+            else { ny = l; nx = b; }
+          }
+          const int node = (D == 3) ? (nz * B + ny) * B + nx : ny * B + nx;
+          acc += Qh[node * ND + v] * FC[l];
+        }
+      }
+      lQhbnd[cell * (2 * D * NF * ND) + o] = acc;
+    }
+    constexpr int NFL = 2 * D * NF * V;
+    for (int o = s; o < NFL; o += NS) {
+      const int v = o % V;
+      const int fn = (o / V) % NF;
+      const int f = o / (V * NF);
+      const int d = f >> 1;
+      const double* FC = (f & 1) ? T->F1 : T->F0;
+      int a = fn / B, b = fn % B;
+      if (D == 2) { a = 0; b = fn; }
+      double acc = 0.0;
+#pragma unroll
+      for (int l = 0; l < B; l++) {
+        int nz, ny, nx;
+        if (D == 3) {
+          if (d == 0) { nz = a; ny = b; nx = l; }
+          else if (d == 1) { nz = a; ny = l; nx = b; }
+          else { nz = l; ny = a; nx = b; }
+        } else {
+          nz = 0;
+          if (d == 0) { ny = b; nx = l; }
+          else { ny = l; nx = b; }
+        }
+        const int node = (D == 3) ? (nz * B + ny) * B + nx : ny * B + nx;
+        acc += Fh[(d * NS + node) * V + v] * FC[l];
+      }
+      lFhbnd[cell * (2 * D * NF * V) + o] = acc;
+    }
+  }
+  if (s == 0 && picard) picard[cell] = iters;
+}
+
 // ---------------------------------------------------------------------------
 // Space-time predictor + volume integral
 // ---------------------------------------------------------------------------
@@ -130,9 +237,10 @@ __device__ __forceinline__ void predictor_tile(
     long blk, long ncells, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
     double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
-    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect) {
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect,
+    int hdim) {
   using S = Shape<D, B, V, NP>;
-  constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, NSP = S::NSP;
+  constexpr int ND = S::ND, NS = S::NS, CPB = S::CPB, NSP = S::NSP;
   static_assert(!NCS || V == 5, "user PDE hooks: 5 variables");
   constexpr int SLAB = NS * D * V + (NCS ? NS * V : 0);  // one flux (+ state) slab, per cell
   constexpr int SLAB_BYTES = 2 * CPB * SLAB * int(sizeof(R));
@@ -153,8 +261,10 @@ __device__ __forceinline__ void predictor_tile(
   node_zyx<D, B>(s, z, y, x);
   const double* w = T->w;
   const double dt = dtp[(cinst && valid) ? cinst[cell] : 0];
-  const double h = valid ? hcell[cell] : 1.0;
-  const double invDx = 1.0 / h;
+  // 1/dx per direction: hcell[cell] (cubic cells, hdim = 0) or hcell[cell][d] (hdim = D)
+  double invDx[D];
+#pragma unroll
+  for (int d = 0; d < D; d++) invDx[d] = 1.0 / (valid ? hcell[hdim ? cell * hdim + d : cell] : 1.0);
 
   double u[ND];
   R q[B][ND];
@@ -176,7 +286,6 @@ __device__ __forceinline__ void predictor_tile(
   const double iweight = 1.0 / W;
   double ta[D], tb[D];
   int cc[D], base[D];
-  R Kc[D][B];  // Kxi[n][c_d]: the stiffness column this node needs per direction
 #pragma unroll
   for (int d = 0; d < D; d++) {
     if (D == 3) {
@@ -188,9 +297,11 @@ __device__ __forceinline__ void predictor_tile(
     }
     cc[d] = coord<D, B>(d, z, y, x);
     base[d] = s - cc[d] * stride_of<D, B>(d);
-#pragma unroll
-    for (int n = 0; n < B; n++) Kc[d][n] = tabK<R>(T, n * B + cc[d]);
   }
+  // Kxi in LDS: a node reads its stiffness column Kxi[n][c_d] per use instead
+  // of holding D B entries in registers (they would cost the second wave per SIMD)
+  __shared__ R kxs[B * B];
+  if (tid < B * B) kxs[tid] = tabK<R>(T, tid);  // read after the first barrier below
 
   // CERK initial guess of the optimised kernels
   // (KernelGenerator/kernelgenerator/templates/aderdg/fusedSPTVI_nonlinear_cpp.template:121-173,
@@ -224,7 +335,7 @@ __device__ __forceinline__ void predictor_tile(
 #pragma unroll
             for (int n = 0; n < B; n++)
               a += slab[((base[d] + n * stride_of<D, B>(d)) * D + d) * V + v] *
-                   (static_cast<R>(invDx) * tabD<R>(T, cc[d] * B + n));
+                   (static_cast<R>(invDx[d]) * tabD<R>(T, cc[d] * B + n));
             K[v] += a;
           }
       }
@@ -247,6 +358,13 @@ __device__ __forceinline__ void predictor_tile(
       for (int v = 0; v < V; v++)
         q[t][v] = static_cast<R>(u[v]) - (rdt * xt * K1[v]) - (R(0.5) * rdt * xt * xt * (K2[v] - K1[v]));
     }
+  } else {
+    // trivial guess: hide that the B slices start equal, or the compiler keeps
+    // the copies folded and its schedule of the first iteration spills
+#pragma unroll
+    for (int t = 0; t < B; t++)
+#pragma unroll
+      for (int v = 0; v < ND; v++) asm volatile("" : "+v"(q[t][v]));
   }
 
   R fh[D][V];  // time-averaged flux of the last executed iteration
@@ -306,14 +424,16 @@ __device__ __forceinline__ void predictor_tile(
         for (int v = 0; v < V; v++) r[v] = rw * static_cast<R>(u[v]);
 #pragma unroll
         for (int d = 0; d < D; d++) {
-          const R upd = static_cast<R>(w[t] * ta[d] * tb[d] * dt * invDx);
+          const R upd = static_cast<R>(w[t] * ta[d] * tb[d] * dt * invDx[d]);
 #pragma unroll
-          for (int v = 0; v < V; v++)
+          for (int n = 0; n < B; n++) {
+            const R kc = kxs[n * B + cc[d]];  // Kxi[n][c_d]
 #pragma unroll
-            for (int n = 0; n < B; n++) {
+            for (int v = 0; v < V; v++) {
               const R f = slab[((base[d] + n * stride_of<D, B>(d)) * D + d) * V + v];
-              r[v] -= upd * f * Kc[d][n];
+              r[v] -= upd * f * kc;
             }
+          }
         }
         if constexpr (NCS) {
           // gradQ of time slice t (:113-176), then rhs += w_t W dt (S - ncp) (:323-367)
@@ -326,7 +446,7 @@ __device__ __forceinline__ void predictor_tile(
               R g = R(0);
 #pragma unroll
               for (int n = 0; n < B; n++)
-                g += static_cast<R>(invDx) * qs[(base[d] + n * stride_of<D, B>(d)) * V + v] *
+                g += static_cast<R>(invDx[d]) * qs[(base[d] + n * stride_of<D, B>(d)) * V + v] *
                      tabD<R>(T, cc[d] * B + n);
               gq[d][v] = g;
             }
@@ -405,109 +525,29 @@ __device__ __forceinline__ void predictor_tile(
   __syncthreads();
   if (!valid) return;
 
-  // ---- volume integral (lduh zeroed, then x, y, z contributions) ----
-  {
-    double du[V];
+  // ---- volume integral and extrapolation ----
+  double shd[V];
+  if constexpr (NCS) {
 #pragma unroll
-    for (int v = 0; v < V; v++) du[v] = 0.0;
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      const double upd = (D == 3 ? ta[d] * tb[d] : ta[d]) * invDx;
-      const int c = cc[d];
-#pragma unroll
-      for (int v = 0; v < V; v++)
-#pragma unroll
-        for (int m = 0; m < B; m++)
-          du[v] += T->Kxi[c * B + m] * Fh[(d * NS + base[d] + m * stride_of<D, B>(d)) * V + v] * upd;
-    }
-    if constexpr (NCS) {  // lduh += W lShi (volumeIntegralNonlinear.cpph:204-222)
-#pragma unroll
-      for (int v = 0; v < V; v++) du[v] += W * static_cast<double>(sh[v]);
-    }
-    if (luh_upd) {
-      // volume part of the solution update in place (solutionUpdate.cpph:28-67: dt / w)
-      const double upd = dt / W;
-#pragma unroll
-      for (int v = 0; v < V; v++) luh_upd[(cell * NS + s) * ND + v] += du[v] * upd;
-    } else {
-#pragma unroll
-      for (int v = 0; v < V; v++) lduh[(cell * NS + s) * V + v] = du[v];
-    }
+    for (int v = 0; v < V; v++) shd[v] = static_cast<double>(sh[v]);
   }
-
-  // ---- extrapolation to the 2D faces (faces zeroed, then FCoeff sums) ----
-  {
-    constexpr int NQ = 2 * D * NF * ND;
-    for (int o = s; o < NQ; o += NS) {
-      const int v = o % ND;
-      const int fn = (o / ND) % NF;
-      const int f = o / (ND * NF);
-      const int d = f >> 1;
-      const double* FC = (f & 1) ? T->F1 : T->F0;
-      // node of the face line at parameter l
-      int a = fn / B, b = fn % B;
-      if (D == 2) { a = 0; b = fn; }
-      double acc = 0.0;
-      const bool skip = (D == 2) && (d == 0) && (v >= V);  // 2D x-faces carry no parameters
-      if (!skip) {
-#pragma unroll
-        for (int l = 0; l < B; l++) {
-          int nz, ny, nx;
-          if (D == 3) {
-            if (d == 0) { nz = a; ny = b; nx = l; }
-            else if (d == 1) { nz = a; ny = l; nx = b; }
-            else { nz = l; ny = a; nx = b; }
-          } else {
-            nz = 0;
-            if (d == 0) { ny = b; nx = l; }
-            else { ny = l; nx = b; }
-          }
-          const int node = (D == 3) ? (nz * B + ny) * B + nx : ny * B + nx;
-          acc += Qh[node * ND + v] * FC[l];
-        }
-      }
-      lQhbnd[cell * (2 * D * NF * ND) + o] = acc;
-    }
-    constexpr int NFL = 2 * D * NF * V;
-    for (int o = s; o < NFL; o += NS) {
-      const int v = o % V;
-      const int fn = (o / V) % NF;
-      const int f = o / (V * NF);
-      const int d = f >> 1;
-      const double* FC = (f & 1) ? T->F1 : T->F0;
-      int a = fn / B, b = fn % B;
-      if (D == 2) { a = 0; b = fn; }
-      double acc = 0.0;
-#pragma unroll
-      for (int l = 0; l < B; l++) {
-        int nz, ny, nx;
-        if (D == 3) {
-          if (d == 0) { nz = a; ny = b; nx = l; }
-          else if (d == 1) { nz = a; ny = l; nx = b; }
-          else { nz = l; ny = a; nx = b; }
-        } else {
-          nz = 0;
-          if (d == 0) { ny = b; nx = l; }
-          else { ny = l; nx = b; }
-        }
-        const int node = (D == 3) ? (nz * B + ny) * B + nx : ny * B + nx;
-        acc += Fh[(d * NS + node) * V + v] * FC[l];
-      }
-      lFhbnd[cell * (2 * D * NF * V) + o] = acc;
-    }
-  }
-  if (s == 0 && picard) picard[cell] = iters;
+  node_volume_and_faces<D, B, V, NP>(Qh, Fh, s, cell, ta, tb, cc, base, invDx, W, dt, iters, NCS ? shd : nullptr, T,
+                                     lduh, lQhbnd, lFhbnd, picard, luh_upd);
 }
 
-// Predictor + volume integral, one workgroup per tile of CPB cells.
+// Predictor + volume integral, one workgroup per tile of CPB cells.  Two
+// waves per SIMD (at most 256 registers per lane) for the conservative forms;
+// the source / NCP form keeps the extra state of its gradient and source terms
+// and runs at one.
 template <int D, int B, int V, int NP, typename R, bool CG, bool NCS = false>
-__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_kernel(
+__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS), NCS ? 1 : 2) void predictor_kernel(
     long ncells, long ntiles, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
     double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
-    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect) {
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect,
+    int hdim) {
   predictor_tile<D, B, V, NP, R, CG, NCS>(blockIdx.x, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard,
-                                          T, P, maxIter, adaptive, luh_upd, dialect);
+                                          T, P, maxIter, adaptive, luh_upd, dialect, hdim);
 }
 
 // Persistent form: a grid smaller than ntiles loops over the tiles.
@@ -516,10 +556,11 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_persi
     long ncells, long ntiles, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
     double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
-    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect) {
+    const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect,
+    int hdim) {
   for (long blk = blockIdx.x; blk < ntiles; blk += gridDim.x) {
     predictor_tile<D, B, V, NP, R, CG>(blk, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard, T, P,
-                                   maxIter, adaptive, luh_upd, dialect);
+                                   maxIter, adaptive, luh_upd, dialect, hdim);
     __syncthreads();  // LDS of this tile fully consumed before the next one
   }
 }
@@ -624,8 +665,8 @@ __device__ __forceinline__ void atomic_min_pos_double(double* addr, double v) {
 // For Euler the direction-independent 1/rho, p and c are computed once; the
 // per-direction maximum keeps pde_max_abs_eig's expressions (same bits).
 template <int D, int B, int V, int NP>
-__device__ __forceinline__ double node_cfl_dt(const double* __restrict__ Q, const PdeParams& P, double invDx,
-                                              double cflpnpm) {
+__device__ __forceinline__ double node_cfl_dt(const double* __restrict__ Q, const PdeParams& P,
+                                              const double (&invDx)[D], double cflpnpm) {
   double den = 0.0;
   if (P.id == kPdeEuler) {
     constexpr double gamma = 1.4;
@@ -637,12 +678,12 @@ __device__ __forceinline__ double node_cfl_dt(const double* __restrict__ Q, cons
     for (int d = 0; d < D; d++) {
       const double u_n = Q[d + 1] * irho;
       const double l0 = fabs(u_n - c), l1 = fabs(u_n), l4 = fabs(u_n + c);
-      den += fmax(fmax(l0, l1), l4) * invDx;
+      den += fmax(fmax(l0, l1), l4) * invDx[d];
     }
     return cflpnpm / den;
   }
 #pragma unroll
-  for (int d = 0; d < D; d++) den += pde_max_abs_eig<D>(P, Q, d) * invDx;
+  for (int d = 0; d < D; d++) den += pde_max_abs_eig<D>(P, Q, d) * invDx[d];
   return cflpnpm / den;
 }
 
@@ -664,7 +705,7 @@ __device__ __forceinline__ void corrector_tile(
     long blk, long ncells, double* __restrict__ luh, const double* __restrict__ lduh,
     const double* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P,
-    double cflpnpm, int do_update, double* sred, double* __restrict__ slots) {
+    double cflpnpm, int do_update, double* sred, double* __restrict__ slots, int hdim) {
   using S = Shape<D, B, V, NP>;
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, THREADS = S::THREADS;
   const int tid = threadIdx.x;
@@ -678,7 +719,9 @@ __device__ __forceinline__ void corrector_tile(
     int z, y, x;
     node_zyx<D, B>(s, z, y, x);
     const double* w = T->w;
-    const double h = hcell[cell];
+    double invDx[D];  // hcell[cell] (cubic cells, hdim = 0) or hcell[cell][d] (hdim = D)
+#pragma unroll
+    for (int d = 0; d < D; d++) invDx[d] = 1.0 / hcell[hdim ? cell * hdim + d : cell];
     double Q[ND];
 #pragma unroll
     for (int v = 0; v < ND; v++) Q[v] = luh[(cell * NS + s) * ND + v];
@@ -694,7 +737,7 @@ __device__ __forceinline__ void corrector_tile(
         const double tw = transverse_weight<D>(w, d, z, y, x);
 #pragma unroll
         for (int o = 0; o < 2; o++) {
-          const double scaling = 1.0 / h * (2.0 * o - 1.0);
+          const double scaling = invDx[d] * (2.0 * o - 1.0);
           const double sw = tw * scaling;
           const double fcoef = o ? T->F1[c] : T->F0[c];
           const double* F = lFhbnd + ((cell * 2 * D + 2 * d + o) * NF + fn) * V;
@@ -710,7 +753,7 @@ __device__ __forceinline__ void corrector_tile(
         luh[(cell * NS + s) * ND + v] = Q[v];
       }
     }
-    mydt = node_cfl_dt<D, B, V, NP>(Q, P, 1.0 / h, cflpnpm);
+    mydt = node_cfl_dt<D, B, V, NP>(Q, P, invDx, cflpnpm);
   }
   // cells of one instance are contiguous: a tile whose first and last cell
   // share the instance reduces once, otherwise every cell merges its own dt
@@ -730,10 +773,10 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_kerne
     long ncells, long ntiles, double* __restrict__ luh, const double* __restrict__ lduh,
     const double* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P,
-    double cflpnpm, int do_update, double* __restrict__ slots) {
+    double cflpnpm, int do_update, double* __restrict__ slots, int hdim) {
   __shared__ double sred[Shape<D, B, V, NP>::THREADS / 64];
   corrector_tile<D, B, V, NP>(blockIdx.x, ncells, luh, lduh, lFhbnd, hcell, dtp, cinst, dt_next, T, P, cflpnpm,
-                              do_update, sred, slots);
+                              do_update, sred, slots, hdim);
 }
 
 template <int D, int B, int V, int NP>
@@ -741,11 +784,11 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_persi
     long ncells, long ntiles, double* __restrict__ luh, const double* __restrict__ lduh,
     const double* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P,
-    double cflpnpm, int do_update, double* __restrict__ slots) {
+    double cflpnpm, int do_update, double* __restrict__ slots, int hdim) {
   __shared__ double sred[Shape<D, B, V, NP>::THREADS / 64];
   for (long blk = blockIdx.x; blk < ntiles; blk += gridDim.x) {
     corrector_tile<D, B, V, NP>(blk, ncells, luh, lduh, lFhbnd, hcell, dtp, cinst, dt_next, T, P, cflpnpm,
-                                do_update, sred, slots);
+                                do_update, sred, slots, hdim);
     __syncthreads();
   }
 }
@@ -1005,7 +1048,10 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
       Q[v] = Q[v] + du[v] * upd;
       luh[(cell * NS + s) * ND + v] = Q[v];
     }
-    mydt = node_cfl_dt<D, B, V, NP>(Q, P, 1.0 / h, cflpnpm);
+    double invDx[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) invDx[d] = 1.0 / h;
+    mydt = node_cfl_dt<D, B, V, NP>(Q, P, invDx, cflpnpm);
   }
   const double bmin = block_min<THREADS>(mydt, sred);
   double* dst = slots ? slots + (blockIdx.x % kDtSlots) * kDtSlotStride : dt_next;
@@ -1089,7 +1135,7 @@ struct Launch {
         if (k.precision != 0 || cerk) return -1;
         hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, double, false, true>), dim3(tiles), dim3(S::THREADS), 0,
                            st, n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive,
-                           o.luh_update, dl);
+                           o.luh_update, dl, o.hdim);
         return static_cast<int>(hipGetLastError());
       } else {
         return -1;
@@ -1110,14 +1156,14 @@ struct Launch {
                                     const LaunchOpts& o, int dl) {
     if (cerk)
       hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, R, true>), dim3(tiles), dim3(S::THREADS), 0, st, n, tiles,
-                         luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update, dl);
+                         luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update, dl, o.hdim);
     else if (persistent)
       hipLaunchKernelGGL((predictor_persistent_kernel<D, B, V, NP, R, false>), dim3(grid), dim3(S::THREADS), 0, st,
                          n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive,
-                         o.luh_update, dl);
+                         o.luh_update, dl, o.hdim);
     else
       hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, R, false>), dim3(tiles), dim3(S::THREADS), 0, st, n, tiles,
-                         luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update, dl);
+                         luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update, dl, o.hdim);
   }
   static int riemann(const KernelCfg& k, hipStream_t st, long nf, const int* faces, const double* qb,
                      double* fb, const LaunchOpts& o) {
@@ -1134,10 +1180,10 @@ struct Launch {
     const long grid = grid_of(tiles, o);
     if (grid < tiles)
       hipLaunchKernelGGL((corrector_persistent_kernel<D, B, V, NP>), dim3(grid), dim3(S::THREADS), 0, st, n, tiles,
-                         luh, lduh, fb, h, dt, o.cinst, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update, o.dt_slots);
+                         luh, lduh, fb, h, dt, o.cinst, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update, o.dt_slots, o.hdim);
     else
       hipLaunchKernelGGL((corrector_kernel<D, B, V, NP>), dim3(grid), dim3(S::THREADS), 0, st, n, tiles, luh, lduh,
-                         fb, h, dt, o.cinst, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update, o.dt_slots);
+                         fb, h, dt, o.cinst, dt_next, k.tab, k.pde, k.cfl * k.pnpm, do_update, o.dt_slots, o.hdim);
     return static_cast<int>(hipGetLastError());
   }
   static int corrector_riemann(const KernelCfg& k, hipStream_t st, long n, int nside, double* luh,
@@ -1231,11 +1277,12 @@ int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* lu
   if (n <= 0) return 0;
   // BF16 operands: the tile kernel (3D Euler order 5, trivial guess) only
   if (k.precision == 2) {
-    if (!line_kernels_serve(k) || k.guess) return -1;
+    if (!line_kernels_serve(k) || k.guess || o.hdim) return -1;
     return launch_predictor3d_tile(k, s, n, luh, h, dt, lduh, qb, fb, pic, o.cinst, o.luh_update, o.faces_f32);
   }
   // auto: FP32 -> line kernel on the matrix cores; FP64 -> node-column kernel
-  if (line_kernels_serve(k)) {
+  // (non-cubic cells: node-column kernel)
+  if (line_kernels_serve(k) && !o.hdim) {
     const int v = k.pred_variant;
     const bool fp32 = k.precision != 0;
     if ((v == 0 && fp32) || v == 3)

@@ -551,32 +551,34 @@ int exa_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* c, double* lduh, double* 
     if (picard_iterations) *picard_iterations = 0;
     return EXA_OK;
   }
-  for (int d = 1; d < c->k.dim; d++)
-    if (dx[d] != dx[0]) {
-      set_error("exa_fusedSpaceTimePredictorVolumeIntegral: cells must be cubic (dx[d] equal)");
+  const int NS = nspace(c), NF = nface(c), ND = ndata(c), V = c->k.nvar, D = c->k.dim;
+  for (int d = 0; d < D; d++)
+    if (!(dx[d] > 0)) {
+      set_error("exa_fusedSpaceTimePredictorVolumeIntegral: dx must be positive");
       return EXA_ERR_ARG;
     }
-  const int NS = nspace(c), NF = nface(c), ND = ndata(c), V = c->k.nvar, D = c->k.dim;
+  bool cubic = true;
+  for (int d = 1; d < D; d++) cubic = cubic && dx[d] == dx[0];
   const size_t nLuh = size_t(NS) * ND, nLduh = size_t(NS) * V;
   const size_t nQb = size_t(2 * D) * NF * ND, nFb = size_t(2 * D) * NF * V;
-  const size_t total = nLuh + nLduh + nQb + nFb + 2 + 2 + 2;
+  const size_t total = nLuh + nLduh + nQb + nFb + 3 + 1 + 1 + 2;
   int rc = ensure_scratch(c, total * sizeof(double));
   if (rc) return rc;
   double* d_luh = c->d_scratch;
   double* d_lduh = d_luh + nLuh;
   double* d_qb = d_lduh + nLduh;
   double* d_fb = d_qb + nQb;
-  double* d_h = d_fb + nFb;
-  double* d_dt = d_h + 2;
+  double* d_h = d_fb + nFb;  // dx[D] (one value for cubic cells)
+  double* d_dt = d_h + 3;
   int* d_pic = reinterpret_cast<int*>(d_dt + 1);
   int* d_guard = reinterpret_cast<int*>(d_dt + 2);  // {count, total, list[1]}
-  double hd[3] = {dx[0], dt, 0.0};
   HIP_OK(hipMemcpyAsync(d_luh, luh, nLuh * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy luh");
-  HIP_OK(hipMemcpyAsync(d_h, &hd[0], sizeof(double), hipMemcpyHostToDevice, c->stream), "copy dx");
-  HIP_OK(hipMemcpyAsync(d_dt, &hd[1], sizeof(double), hipMemcpyHostToDevice, c->stream), "copy dt");
+  HIP_OK(hipMemcpyAsync(d_h, dx, (cubic ? 1 : D) * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy dx");
+  HIP_OK(hipMemcpyAsync(d_dt, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream), "copy dt");
   HIP_OK(hipMemsetAsync(d_guard, 0, 2 * sizeof(double), c->stream), "clear guard");
   exa::LaunchOpts po;
   po.guard = d_guard;
+  po.hdim = cubic ? 0 : D;  // non-cubic cells: FP64 node-column or slab predictor, dx per direction
   rc = launch_status(exa::launch_predictor(c->k, c->stream, 1, d_luh, d_h, d_dt, d_lduh, d_qb, d_fb, d_pic, po),
                      "predictor");
   if (rc) return rc;
@@ -968,22 +970,24 @@ int exa_stableTimeStepSize(exa_ctx* c, const double* luh, const double* dx, doub
     HIP_OK(hipStreamSynchronize(c->stream), "sync");
     return EXA_OK;
   }
-  for (int d = 1; d < c->k.dim; d++)
-    if (dx[d] != dx[0]) {
-      set_error("exa_stableTimeStepSize: cells must be cubic (dx[d] equal)");
+  const int NS = nspace(c), ND = ndata(c), D = c->k.dim;
+  for (int d = 0; d < D; d++)
+    if (!(dx[d] > 0)) {
+      set_error("exa_stableTimeStepSize: dx must be positive");
       return EXA_ERR_ARG;
     }
-  const int NS = nspace(c), ND = ndata(c);
-  const size_t total = size_t(NS) * ND + 2;
+  const size_t total = size_t(NS) * ND + 4;
   int rc = ensure_scratch(c, total * sizeof(double));
   if (rc) return rc;
   double* d_luh = c->d_scratch;
-  double* d_h = d_luh + size_t(NS) * ND;
-  double* d_out = d_h + 1;
-  const double init[2] = {dx[0], DBL_MAX};
+  double* d_h = d_luh + size_t(NS) * ND;  // dx[D]
+  double* d_out = d_h + 3;
+  const double init[4] = {dx[0], D > 1 ? dx[1] : 0.0, D > 2 ? dx[2] : 0.0, DBL_MAX};
   HIP_OK(hipMemcpyAsync(d_luh, luh, NS * ND * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
-  HIP_OK(hipMemcpyAsync(d_h, init, 2 * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
-  rc = launch_status(exa::launch_stable_dt(c->k, c->stream, 1, d_luh, d_h, d_out), "stable dt");
+  HIP_OK(hipMemcpyAsync(d_h, init, 4 * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+  exa::LaunchOpts o;
+  o.hdim = D;  // dx per direction
+  rc = launch_status(exa::launch_stable_dt(c->k, c->stream, 1, d_luh, d_h, d_out, o), "stable dt");
   if (rc) return rc;
   HIP_OK(hipMemcpyAsync(dt_out, d_out, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
   HIP_OK(hipStreamSynchronize(c->stream), "sync");

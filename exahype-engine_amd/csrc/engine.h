@@ -89,6 +89,9 @@ struct LaunchOpts {
   // predictor, one-iteration FP32 CERK form: Picard guard buffer of 2 + ncells
   // ints {listed this launch (0 on entry), listed in total, list}; nullptr: no guard
   int* guard = nullptr;
+  // node-column predictor and corrector: 0 = one edge length per cell (h[cell]),
+  // D = one per direction (h[cell][d]); the other kernels take cubic cells only
+  int hdim = 0;
 };
 
 // Slots of the spread dt reduction: one device-scope atomic per workgroup on a
