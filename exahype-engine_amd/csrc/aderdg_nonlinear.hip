@@ -552,7 +552,7 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS), NCS ? 1 : 2) void pr
 
 // Persistent form: a grid smaller than ntiles loops over the tiles.
 template <int D, int B, int V, int NP, typename R, bool CG>
-__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void predictor_persistent_kernel(
+__global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS), 2) void predictor_persistent_kernel(
     long ncells, long ntiles, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
     double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
