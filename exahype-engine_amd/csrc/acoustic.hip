@@ -247,7 +247,14 @@ __device__ __forceinline__ void line_derivatives(const double* __restrict__ Dm, 
 // volume update is dt * A(qh) - sigma0 g (A: one Cauchy-Kowalevski level,
 // i.e. the reference's -sum_d lFhi[d] of the linear flux), so a last
 // derivative pass on qh replaces per-level update accumulators.
-template <int B, bool UPDATE>
+// SPLIT: the optimised kernels' split Cauchy-Kowalevski predictor
+// (KernelGenerator/kernelgenerator/templates/aderdg/fusedSPTVI_linear_split_ck_cpp.template):
+// levels 1..B, the NCP term D_d p without 1/rho (the template's material
+// matrix multiplies the flux part only, :201-331), the template's averaging
+// factors f_0 = dt/2, f_{k+1} = f_k dt/(k+2) (:196, :346-353), source terms
+// s_k of the per-step row {sigma0, s_0..s_{B-1}}; the volume update is
+// dt * A(qh) - sigma0 g with that A (:474-640).
+template <int B, bool UPDATE, bool SPLIT>
 __global__ void __launch_bounds__(kAcThreads) ac_predict_lines_kernel(AcousticArgs a) {
   using S = AcLineShape<B>;
   constexpr int N = S::N, F = S::F, CPB = S::CPB;
@@ -279,11 +286,11 @@ __global__ void __launch_bounds__(kAcThreads) ac_predict_lines_kernel(AcousticAr
 #pragma unroll
     for (int x = 0; x < B; x++) P[x] = U[x] = Vv[x] = Wv[x] = 0.0;
   }
-  const double lam = rho * c * c, irho = 1.0 / rho;
+  const double lam = rho * c * c, irho = SPLIT ? 1.0 : 1.0 / rho;
   const int step = a.step_override >= 0 ? a.step_override : *a.step_ptr;
   const bool src = on && cell == a.src_cell;
   const double* gp = src ? a.src_g + ln * B : nullptr;
-  const double* stp = src ? a.src_tab + long(step) * B : nullptr;
+  const double* stp = src ? a.src_tab + long(step) * (SPLIT ? B + 1 : B) : nullptr;
   if (src) {
 #pragma unroll
     for (int x = 0; x < B; x++) P[x] += stp[0] * gp[x];  // level 0
@@ -297,8 +304,9 @@ __global__ void __launch_bounds__(kAcThreads) ac_predict_lines_kernel(AcousticAr
     qh[3][x] = Wv[x];
   }
   double an = 0.5 * a.dt;  // a_{i+1}
+  constexpr int LEVELS = SPLIT ? B : B - 1;
 #pragma unroll 1
-  for (int i = 0; i + 1 < B; i++) {
+  for (int i = 0; i < LEVELS; i++) {
     double* sb = sbuf[i & 1][cs];
     if (slot) {
 #pragma unroll
@@ -311,7 +319,7 @@ __global__ void __launch_bounds__(kAcThreads) ac_predict_lines_kernel(AcousticAr
     __syncthreads();
     double div[B], px[B], py[B], pz[B];
     line_derivatives<B>(Dm, invh, z, y, P, U, sb, div, px, py, pz);
-    const double si = src ? stp[i + 1] : 0.0;
+    const double si = src ? stp[i + 1] : 0.0;  // s_{i+1} (generic row), s_i (split row)
 #pragma unroll
     for (int x = 0; x < B; x++) {
       P[x] = lam * div[x] + (src ? si * gp[x] : 0.0);
@@ -323,10 +331,10 @@ __global__ void __launch_bounds__(kAcThreads) ac_predict_lines_kernel(AcousticAr
       qh[2][x] += an * Vv[x];
       qh[3][x] += an * Wv[x];
     }
-    an *= a.dt / (i + 3);
+    an *= a.dt / (SPLIT ? i + 2 : i + 3);
   }
   // A(qh): the volume update, and qh on the faces
-  constexpr int LAST = (B - 1) & 1;
+  constexpr int LAST = LEVELS & 1;  // the buffer the last level did not use
   double* sb = sbuf[LAST][cs];
   if (slot) {
 #pragma unroll
@@ -740,14 +748,18 @@ int launch_all(int which, const AcousticArgs& a, hipStream_t s) {
   const unsigned lblocks = static_cast<unsigned>((a.ncells + AcLineShape<B>::CPB - 1) / AcLineShape<B>::CPB);
   switch (which) {
     case kAcPredictUpdate:
-      if (a.pred_variant == 0)
-        ac_predict_lines_kernel<B, true><<<lblocks, kAcThreads, 0, s>>>(a);
+      if (a.split_ck && a.pred_variant != 0) return -1;  // the split predictor has the x-line form only
+      if (a.split_ck)
+        ac_predict_lines_kernel<B, true, true><<<lblocks, kAcThreads, 0, s>>>(a);
+      else if (a.pred_variant == 0)
+        ac_predict_lines_kernel<B, true, false><<<lblocks, kAcThreads, 0, s>>>(a);
       else
         ac_predict_kernel<B, true><<<blocks, kAcThreads, 0, s>>>(a);
       break;
     case kAcPredictFaces:
+      if (a.split_ck) return -1;  // the adjoint sweeps transpose the generic predictor only
       if (a.pred_variant == 0)
-        ac_predict_lines_kernel<B, false><<<lblocks, kAcThreads, 0, s>>>(a);
+        ac_predict_lines_kernel<B, false, false><<<lblocks, kAcThreads, 0, s>>>(a);
       else
         ac_predict_kernel<B, false><<<blocks, kAcThreads, 0, s>>>(a);
       break;

@@ -144,7 +144,8 @@ int ensure_source_table(exa_acoustic* ac, int steps) {
   const double* dudx = ac->tab.dudx;
   const double* F0 = ac->tab.F0;
   const int cap = steps + 64;
-  std::vector<double> tabh(size_t(cap) * B);
+  const int SW = ac->k.split_ck ? B + 1 : B;  // row width
+  std::vector<double> tabh(size_t(cap) * SW);
   std::vector<double> fn(B + 1), psd(B), tmp(B);
   for (int k = 0; k < cap; k++) {
     const double t = k * ac->dt;
@@ -154,7 +155,26 @@ int ensure_source_table(exa_acoustic* ac, int steps) {
     }
     double sum = 0.0;
     for (int m = 1; m < B + 1; m++) sum += w[m - 1] * fn[m];
-    double* row = &tabh[size_t(k) * B];
+    double* row = &tabh[size_t(k) * SW];
+    if (ac->k.split_ck) {
+      // split predictor (fusedSPTVI_linear_split_ck_cpp.template:179-199, 313-331):
+      // {dt sum_t w_t f_t, s_0 .. s_{B-1}}, s_k = derPS_k . f at the Gauss time
+      // nodes, derPS_0 = FLCoeff, derPS_{k+1}[j] = sum_i derPS_k[i] dudx[i][j]
+      row[0] = ac->dt * sum;
+      std::vector<double> der(F0, F0 + B), nxt(B);
+      for (int q = 0; q < B; q++) {
+        double v = 0.0;
+        for (int j = 0; j < B; j++) v += der[j] * fn[j + 1];
+        row[1 + q] = v;
+        for (int j = 0; j < B; j++) {
+          double acc = 0.0;
+          for (int i = 0; i < B; i++) acc += der[i] * dudx[i * B + j];
+          nxt[j] = acc;
+        }
+        der = nxt;
+      }
+      continue;
+    }
     row[0] = ac->dt * sum;
     if (B > 1) row[1] = fn[0];
     for (int m = 0; m < B; m++) psd[m] = fn[m + 1];
@@ -203,6 +223,7 @@ exa::AcousticArgs args(exa_acoustic* ac, double* u) {
   a.qbbar = ac->d_qbbar;
   a.gc = ac->d_gc;
   a.pred_variant = ac->pred_variant;
+  a.split_ck = ac->k.split_ck;
   return a;
 }
 
@@ -542,6 +563,11 @@ int exa_acoustic_misfit_gradient(exa_acoustic* ac, int nsteps, int every, const 
   if (!ac || nsteps < 1 || every < 1 || !obs || !J || !grad_c)
     return argerr("exa_acoustic_misfit_gradient: invalid argument");
   if (ac->nrec < 1) return argerr("exa_acoustic_misfit_gradient: no receivers");
+  if (ac->k.split_ck) {
+    exa::set_error("exa_acoustic_misfit_gradient: the adjoint transposes the generic predictor (context dialect "
+                   "EXA_DIALECT_SPLIT_CK)");
+    return EXA_ERR_UNSUPPORTED;
+  }
   const int s0 = ac->step, K = nsteps, R = ac->nrec, o = ac->k.order;
   const int nck = (K + every - 1) / every;
   const int seg = every < K ? every : K;
@@ -660,6 +686,10 @@ int exa_acoustic_set_timing(exa_acoustic* ac, int enabled) {
 int exa_acoustic_set_predictor(exa_acoustic* ac, int variant) {
   if (!ac) return argerr("exa_acoustic_set_predictor: NULL acoustic");
   if (variant != 0 && variant != 1) return argerr("exa_acoustic_set_predictor: variant must be 0 (x-line) or 1 (node)");
+  if (variant == 1 && ac->k.split_ck) {
+    exa::set_error("exa_acoustic_set_predictor: the split Cauchy-Kowalevski predictor has the x-line form only");
+    return EXA_ERR_UNSUPPORTED;
+  }
   ac->pred_variant = variant;
   return EXA_OK;
 }

@@ -189,6 +189,29 @@ bool point_source_terms(const exa_ctx* c, const double* centre, const double* dx
                                  (lagrange(x, B, k, xr[0]) * lagrange(x, B, j, xr[1]) * lagrange(x, B, i, xr[2]));
   std::vector<double> fn(B + 1), psd(B), tmp(B);
   for (int m = 0; m < B + 1; m++) fn[m] = c->src_amp * ricker(m == 0 ? t : dt * x[m - 1] + t, c->src_f0, c->src_t0);
+  if (c->k.split_ck) {
+    // split predictor (fusedSPTVI_linear_split_ck_cpp.template:179-199, 313-331):
+    // {dt sum_t w_t f_t, s_0 .. s_{B-1}, sum_t w_t f_t}, s_k = derPS_k . f with
+    // derPS_0 = FLCoeff and derPS_{k+1}[j] = sum_i derPS_k[i] dudx[i][j]
+    row.assign(B + 2, 0.0);
+    double sum = 0.0;
+    for (int m = 0; m < B; m++) sum += w[m] * fn[m + 1];
+    row[0] = dt * sum;
+    std::vector<double> der(c->tab.F0, c->tab.F0 + B), nxt(B);
+    for (int k = 0; k < B; k++) {
+      double v = 0.0;
+      for (int j = 0; j < B; j++) v += der[j] * fn[j + 1];
+      row[1 + k] = v;
+      for (int j = 0; j < B; j++) {
+        double a = 0.0;
+        for (int i = 0; i < B; i++) a += der[i] * c->tab.dudx[i * B + j];
+        nxt[j] = a;
+      }
+      der = nxt;
+    }
+    row[B + 1] = sum;
+    return true;
+  }
   row.assign(B + 1, 0.0);
   double sum = 0.0;
   for (int m = 1; m < B + 1; m++) sum += w[m - 1] * fn[m];
@@ -387,17 +410,22 @@ int exa_ctx_set_boundary_values(exa_ctx* c, int kind) {
 }
 
 int exa_ctx_set_dialect(exa_ctx* c, int dialect, int vect_size) {
-  if (!c || dialect < EXA_DIALECT_GENERIC || dialect > EXA_DIALECT_OPTIMISED_CERK ||
+  if (!c || dialect < EXA_DIALECT_GENERIC || dialect > EXA_DIALECT_SPLIT_CK ||
       !(vect_size == 1 || vect_size == 2 || vect_size == 4 || vect_size == 8)) {
-    set_error("exa_ctx_set_dialect: invalid argument (dialect 0..2, vect_size 1, 2, 4 or 8)");
+    set_error("exa_ctx_set_dialect: invalid argument (dialect 0..3, vect_size 1, 2, 4 or 8)");
     return EXA_ERR_ARG;
   }
-  if (is_linear(c) && dialect != EXA_DIALECT_GENERIC) {
-    set_error("exa_ctx_set_dialect: the optimised dialect covers the nonlinear kernels");
+  if (is_linear(c) && (dialect == EXA_DIALECT_OPTIMISED || dialect == EXA_DIALECT_OPTIMISED_CERK)) {
+    set_error("exa_ctx_set_dialect: the optimised Picard dialect covers the nonlinear kernels");
     return EXA_ERR_UNSUPPORTED;
   }
-  c->k.dialect = dialect != EXA_DIALECT_GENERIC ? 1 : 0;
+  if (!is_linear(c) && dialect == EXA_DIALECT_SPLIT_CK) {
+    set_error("exa_ctx_set_dialect: the split Cauchy-Kowalevski predictor is linear-only");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  c->k.dialect = (dialect == EXA_DIALECT_OPTIMISED || dialect == EXA_DIALECT_OPTIMISED_CERK) ? 1 : 0;
   c->k.guess = dialect == EXA_DIALECT_OPTIMISED_CERK ? 1 : 0;
+  c->k.split_ck = dialect == EXA_DIALECT_SPLIT_CK ? 1 : 0;
   c->vect = vect_size;
   return EXA_OK;
 }
@@ -520,7 +548,7 @@ int exa_fusedSpaceTimePredictorVolumeIntegral(exa_ctx* c, double* lduh, double* 
       HIP_OK(hipMemcpyAsync(d_row, row.data(), row.size() * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
     }
     exa::LinearCellArgs a;
-    a.op = exa::kLinPredictor;
+    a.op = c->k.split_ck ? exa::kLinPredictorSplit : exa::kLinPredictor;
     a.tab = c->d_tab;
     a.luh = d_luh;
     for (int d = 0; d < 3; d++) a.dx[d] = dx[d];

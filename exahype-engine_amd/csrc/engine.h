@@ -55,6 +55,10 @@ struct KernelCfg {
   // the optimised kernels' CERK cap)
   double guard_tol;
   int guard_recount;
+  // linear path: 1 = the optimised kernels' split Cauchy-Kowalevski predictor
+  // (one derivative level kept, B levels in the time average, material matrix
+  // on the flux part only); the other linear kernels stay generic
+  int split_ck;
 };
 
 // Launch options of the batched kernels.
@@ -186,7 +190,7 @@ int launch_entropy_wave(hipStream_t s, long ncells, int n, int D, int B, int ND,
 int launch_fill(hipStream_t s, double* p, long n, double value);
 
 // ---- per-cell linear (acoustic) kernels (linear_cell.hip) ------------------
-enum { kLinPredictor = 0, kLinRiemann, kLinFace, kLinUpdate, kLinDt };
+enum { kLinPredictor = 0, kLinRiemann, kLinFace, kLinUpdate, kLinDt, kLinPredictorSplit };
 struct LinearCellArgs {
   int op = 0;
   const DevTables* tab = nullptr;
@@ -195,7 +199,8 @@ struct LinearCellArgs {
   double dx[3] = {1, 1, 1};
   double dt = 0, cflpnpm = 0;
   const double* sg = nullptr;    // point source: spatial factor [B^3]
-  const double* srow = nullptr;  // point source: {sigma0, s_1..s_{B-1}, sum_t w_t f_t}
+  const double* srow = nullptr;  // point source: {sigma0, s_1..s_{B-1}, sum_t w_t f_t};
+                                 // split predictor: {sigma0, s_0..s_{B-1}, sum_t w_t f_t}
   double* lduh = nullptr;        // [B^3][4]
   double* lQb = nullptr;         // [6][B^2][6]
   double* lFb = nullptr;         // [6][B^2][4]
@@ -237,7 +242,7 @@ struct AcousticArgs {
   double* qb = nullptr;
   long src_cell = -1;              // cell holding the point source, -1: none
   const double* src_g = nullptr;   // [B^3] spatial factor of the source moments
-  const double* src_tab = nullptr; // [steps][B] time factors per step
+  const double* src_tab = nullptr; // [steps][B] time factors per step ([steps][B + 1] split)
   const int* step_ptr = nullptr;   // device step counter
   int step_override = -1;          // >= 0: use this step instead of *step_ptr
   int nrec = 0;
@@ -249,6 +254,7 @@ struct AcousticArgs {
   double* qbbar = nullptr;
   double* gc = nullptr;
   int pred_variant = 0;  // forward predictor: 0 x-line form, 1 node form
+  int split_ck = 0;      // split Cauchy-Kowalevski predictor (x-line form only)
 };
 enum { kAcPredictUpdate = 0, kAcPredictFaces, kAcCorrector, kAcFbar, kAcRiemannAdj, kAcPredictAdj, kAcReceivers };
 bool acoustic_order_supported(int order);

@@ -7,6 +7,9 @@
 //   lin_cell_face_kernel       faceIntegralLinear.cpph:8-101
 //   lin_cell_update_kernel     solutionUpdate.cpph:28-67
 //   lin_cell_dt_kernel         stableTimeStepSize.cpph:14-62 with eigenvalues (c, -c, 0, 0)
+//   lin_cell_split_kernel      the optimised kernels' split Cauchy-Kowalevski predictor +
+//                              volume integral (KernelGenerator/kernelgenerator/templates/
+//                              aderdg/fusedSPTVI_linear_split_ck_cpp.template)
 // Hooks: ApplicationExamples/Linear/LinearWave3D/MyLinearWaveSolver.cpp:108-209.
 #include <hip/hip_runtime.h>
 
@@ -223,6 +226,118 @@ __global__ __launch_bounds__(LT) void lin_cell_dt_kernel(const double* __restric
   }
 }
 
+// Split Cauchy-Kowalevski predictor (fusedSPTVI_linear_split_ck_cpp.template;
+// "tpl" below).  One derivative level L is kept; with the acoustic hooks
+//   L_{k+1}[p] = lambda sum_d D_d u_d + s_k g,   L_{k+1}[u_d] = D_d p
+// (tpl:201-331: the material matrix multiplies the flux part only, the NCP
+// -D_d p enters unscaled), k = 0..B-1, and qh = L_0 + sum_k f_k L_{k+1} with
+// the template's factors f_0 = dt/2, f_{k+1} = f_k dt/(k+2) (tpl:196, :346-353).
+// Then per direction lFhi_d[p] = g sum_t w_t f_t / 3 - lambda D_d qh_{u_d},
+// lFhi_d[u_d] = -D_d qh_p (tpl:474-640), lduh = -W sum_d lFhi_d, and qh (with
+// the parameters) and lFhi_d extrapolated to the faces.
+// srow (nullable): {sigma0 = dt sum_t w_t f_t, s_0 .. s_{B-1}, sum_t w_t f_t},
+// f_t the source at the Gauss time nodes, s_k the k-th derivative at 0 of its
+// interpolant on the unit interval; sg[N]: spatial factor.
+template <int B>
+__global__ __launch_bounds__(LT) void lin_cell_split_kernel(const double* __restrict__ luh, double dx0, double dx1,
+                                                            double dx2, double dt, const double* __restrict__ sg,
+                                                            const double* __restrict__ srow,
+                                                            double* __restrict__ lduh, double* __restrict__ lQb,
+                                                            double* __restrict__ lFb,
+                                                            const DevTables* __restrict__ T) {
+  constexpr int N = B * B * B, F = B * B;
+  __shared__ double sL[4][N];
+  __shared__ double sQh[6][N];
+  __shared__ double sFh[3][4][N];
+  const int nd = threadIdx.x;
+  const bool on = nd < N;
+  const int x = nd % B, y = (nd / B) % B, z = nd / (B * B);
+  const int c3[3] = {x, y, z};
+  const double idx[3] = {1.0 / dx0, 1.0 / dx1, 1.0 / dx2};
+  double q[4] = {0, 0, 0, 0}, rho = 1.0, c = 1.0;
+  if (on) {
+    for (int v = 0; v < 4; v++) q[v] = luh[nd * 6 + v];
+    rho = luh[nd * 6 + 4];
+    c = luh[nd * 6 + 5];
+  }
+  const double lam = rho * c * c;
+  const double g = (srow && on) ? sg[nd] : 0.0;
+  if (srow) q[0] += srow[0] * g;  // tpl:181-188
+  double qh[4] = {q[0], q[1], q[2], q[3]};
+  double fac = 0.5 * dt;
+  // D_d of p and of u_d at this node from the level in sL
+  auto derivatives = [&](double (&Dp)[3], double (&Du)[3]) {
+    for (int d = 0; d < 3; d++) {
+      double sp = 0.0, su = 0.0;
+      for (int m = 0; m < B; m++) {
+        const double dd = idx[d] * T->dudx[c3[d] * B + m];
+        const int nn = line_node3<B>(d, x, y, z, m);
+        sp += sL[0][nn] * dd;
+        su += sL[1 + d][nn] * dd;
+      }
+      Dp[d] = sp;
+      Du[d] = su;
+    }
+  };
+#pragma unroll 1
+  for (int k = 0; k < B; k++) {
+    __syncthreads();
+    if (on)
+      for (int v = 0; v < 4; v++) sL[v][nd] = q[v];
+    __syncthreads();
+    if (on) {
+      double Dp[3], Du[3];
+      derivatives(Dp, Du);
+      q[0] = lam * (Du[0] + Du[1] + Du[2]);
+      for (int d = 0; d < 3; d++) q[1 + d] = Dp[d];
+      if (srow) q[0] += srow[1 + k] * g;
+      for (int v = 0; v < 4; v++) qh[v] += fac * q[v];
+      fac *= dt / (k + 2);
+    }
+  }
+  // lFhi_d from qh
+  __syncthreads();
+  if (on)
+    for (int v = 0; v < 4; v++) sL[v][nd] = qh[v];
+  __syncthreads();
+  if (on) {
+    double Dp[3], Du[3];
+    derivatives(Dp, Du);
+    const double ps = srow ? srow[B + 1] * g / 3 : 0.0;
+    const double W = T->w[z] * T->w[y] * T->w[x];
+    double fh[3][4];
+    for (int d = 0; d < 3; d++) {
+      for (int v = 0; v < 4; v++) fh[d][v] = 0.0;
+      fh[d][0] = ps - lam * Du[d];
+      fh[d][1 + d] = -Dp[d];
+    }
+    for (int v = 0; v < 4; v++) {
+      double r = 0.0;
+      for (int d = 0; d < 3; d++) r -= fh[d][v] * W;
+      lduh[nd * 4 + v] = r;
+    }
+    for (int v = 0; v < 4; v++) sQh[v][nd] = qh[v];
+    sQh[4][nd] = rho;
+    sQh[5][nd] = c;
+    for (int d = 0; d < 3; d++)
+      for (int v = 0; v < 4; v++) sFh[d][v][nd] = fh[d][v];
+  }
+  __syncthreads();
+  for (int task = nd; task < 6 * F; task += LT) {
+    const int f = task / F, fn = task - f * F, d = f >> 1, o = f & 1;
+    const int fa = fn / B, fb = fn - fa * B;
+    double sq[6] = {0, 0, 0, 0, 0, 0}, sf[4] = {0, 0, 0, 0};
+    for (int l = 0; l < B; l++) {
+      const int node = d == 0 ? (fa * B + fb) * B + l : (d == 1 ? (fa * B + l) * B + fb : (l * B + fa) * B + fb);
+      const double fc = o ? T->F1[l] : T->F0[l];
+      for (int k = 0; k < 6; k++) sq[k] += sQh[k][node] * fc;
+      for (int v = 0; v < 4; v++) sf[v] += sFh[d][v][node] * fc;
+    }
+    for (int k = 0; k < 6; k++) lQb[(f * F + fn) * 6 + k] = sq[k];
+    for (int v = 0; v < 4; v++) lFb[(f * F + fn) * 4 + v] = sf[v];
+  }
+}
+
 template <int B>
 int launch_b(const LinearCellArgs& a, hipStream_t s) {
   const DevTables* T = a.tab;
@@ -230,6 +345,10 @@ int launch_b(const LinearCellArgs& a, hipStream_t s) {
     case kLinPredictor:
       lin_cell_predictor_kernel<B><<<1, LT, 0, s>>>(a.luh, a.dx[0], a.dx[1], a.dx[2], a.dt, a.sg, a.srow, a.lduh,
                                                      a.lQb, a.lFb, T);
+      break;
+    case kLinPredictorSplit:
+      lin_cell_split_kernel<B><<<1, LT, 0, s>>>(a.luh, a.dx[0], a.dx[1], a.dx[2], a.dt, a.sg, a.srow, a.lduh, a.lQb,
+                                                 a.lFb, T);
       break;
     case kLinRiemann:
       lin_cell_riemann_kernel<B><<<1, 64, 0, s>>>(a.FL, a.FR, a.QL, a.QR, a.direction, T);

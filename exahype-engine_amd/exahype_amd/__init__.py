@@ -21,7 +21,7 @@ HEADER_PATH = os.path.join(REPO_DIR, "include", "exahype_amd.h")
 
 PDE_EULER, PDE_ACOUSTIC, PDE_ADVECTION, PDE_USER = 0, 1, 2, 3
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
-DIALECT_GENERIC, DIALECT_OPTIMISED, DIALECT_OPTIMISED_CERK = 0, 1, 2
+DIALECT_GENERIC, DIALECT_OPTIMISED, DIALECT_OPTIMISED_CERK, DIALECT_SPLIT_CK = 0, 1, 2, 3
 GUESS_TRIVIAL, GUESS_CERK = 0, 1
 CONVERTERS = ("Qhbnd", "Fhbnd", "QFace", "FFace", "Qhi", "Fhi", "PSi", "Lduh")
 EXA_OK, EXA_ERR_ARG, EXA_ERR_HIP, EXA_ERR_UNSUPPORTED, EXA_ERR_NOMEM = 0, 1, 2, 3, 4
@@ -243,7 +243,8 @@ class Context:
         _check(_lib.exa_ctx_set_predictor_variant(self.h, variant))
 
     def set_dialect(self, dialect, vect_size=1):
-        """Generic (0) or optimised-kernel (1; 2 with the CERK guess) Picard rule and padding width."""
+        """Generic (0) or optimised-kernel (1; 2 with the CERK guess) Picard rule and padding width
+        (nonlinear contexts); DIALECT_SPLIT_CK (3): the split Cauchy-Kowalevski predictor (linear)."""
         _check(_lib.exa_ctx_set_dialect(self.h, dialect, vect_size))
 
     def set_initial_guess(self, guess):

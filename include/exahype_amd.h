@@ -118,14 +118,29 @@ int exa_ctx_set_predictor_variant(exa_ctx* ctx, int variant);
  *   EXA_DIALECT_OPTIMISED_CERK the same with the Continuous Extension
  *                              Runge-Kutta initial guess and MaxIterations =
  *                              max(nDof - 3, 1)
+ *   EXA_DIALECT_SPLIT_CK       linear contexts only: the split Cauchy-Kowalevski
+ *                              predictor + volume integral of the optimised
+ *                              linear kernels (fusedSPTVI_linear_split_ck_cpp.template;
+ *                              space_time_predictor split_ck in a specification
+ *                              file, solverController.py:221-222) in
+ *                              exa_fusedSpaceTimePredictorVolumeIntegral and the
+ *                              acoustic mesh (x-line form): one derivative level
+ *                              kept, B levels in the time average with the
+ *                              template's factors (dt/2, then x dt/(t+2) from
+ *                              t = 0: dt^k/(2 k!), not the Taylor dt^k/(k+1)!),
+ *                              the material matrix on the flux part only, point
+ *                              sources at the Gauss time nodes.  Riemann, face
+ *                              integral and update stay generic; no adjoint.
  * vect_size (1, 2, 4 or 8 doubles: the generator's noarch, wsm, snb/hsw,
  * knl/skx widths) sets nVarPad and nDataPad of the exa_opt_* and
- * exa_converter_* entry points.  The optimised dialects run on the node-column
- * predictor kernel.  Meshes copy the context's settings when created. */
+ * exa_converter_* entry points.  The optimised Picard dialects run on the
+ * node-column predictor kernel.  Meshes copy the context's settings when
+ * created. */
 enum exa_dialect {
   EXA_DIALECT_GENERIC = 0,
   EXA_DIALECT_OPTIMISED = 1,
-  EXA_DIALECT_OPTIMISED_CERK = 2
+  EXA_DIALECT_OPTIMISED_CERK = 2,
+  EXA_DIALECT_SPLIT_CK = 3
 };
 int exa_ctx_set_dialect(exa_ctx* ctx, int dialect, int vect_size);
 

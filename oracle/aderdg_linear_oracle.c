@@ -186,6 +186,19 @@ void orc_lin_psi(const orc_solver* s, const double* centre, double h, double t, 
   }
 }
 
+/* Point-source moments of the split predictor (fusedSPTVI_linear_split_ck_cpp.template:112-155):
+ * PSi[t][B^3][V] at the Gauss time nodes t + dt node[t], t = 0..B-1, with the
+ * spatial factor of orc_lin_psi. */
+void orc_lin_psi_split(const orc_solver* s, const double* centre, double h, double t, double dt, const double* xs,
+                       double amp, double f0, double t0, double* PSi) {
+  const int B = s->b.order + 1, B3 = B * B * B, V = s->nvar;
+  double* full = (double*)malloc(sizeof(double) * (B + 1) * B3 * V);
+  /* slots 1..B of the generic moments are the Gauss time nodes */
+  orc_lin_psi(s, centre, h, t, dt, xs, amp, f0, t0, full);
+  memcpy(PSi, full + (size_t)B3 * V, sizeof(double) * B * B3 * V);
+  free(full);
+}
+
 /* Taylor averaging factors a_0 = 1, a_i = dt^i/(i+1)! (spaceTimePredictorLinear.cpph:409-421) */
 static void taylor_factors(int B, double dt, double* a) {
   a[0] = 1.0;
@@ -332,6 +345,170 @@ void orc_lin_stp(const orc_solver* s, int flags, const double* luh, const double
   free(Fn);
 #undef QI
 #undef FI
+}
+
+/* Split Cauchy-Kowalevski predictor of the optimised linear kernels
+ * (KernelGenerator/kernelgenerator/templates/aderdg/fusedSPTVI_linear_split_ck_cpp.template,
+ * "tpl" below) with its volume integral, dimension 2 or 3, the flux / NCP /
+ * material-matrix switches of orc_lin_stp.  Only the current time-derivative
+ * level is kept:
+ *   L_0 = luh + dt sum_t w_t PSi_t                                    tpl:179-199
+ *   L_{k+1} = sum_d [ MM(-D_d F_d(L_k)) - ncp_d(L_k, D_d L_k) ]
+ *             + sum_t derPS_k[t] PSi_t,   derPS_0 = FLCoeff,
+ *             derPS_{k+1}[j] = sum_i derPS_k[i] dudx[i][j]             tpl:201-384
+ *   lQhi = L_0 + sum_{k=0}^{B-1} dt^(k+1)/(k+2)! L_{k+1}               tpl:386-395
+ * (B derivative levels where the generic kernel sums B - 1; the material
+ * matrix multiplies the flux part only, the NCP enters unscaled; the source
+ * derivatives are those on the unit time interval).  Then lQbnd = lQhi and
+ * the parameters extrapolated (tpl:402-470) and per direction d
+ *   lFhi_d = sum_t w_t/D PSi_t + MM(D_d F_d(lQhi)) + ncp_d(lQhi, D_d lQhi),
+ * lFbnd from lFhi_d, lduh -= W lFhi_d (tpl:474-640).
+ * D_d f(node) = sum_m (1/dx_d) dudx[c_d][m] f(line node m).
+ *   luh[B^D][nData]; PSi [B][B^D][V] or NULL; out lduh[B^D][V],
+ *   lQhi_out[B^D][nData] (may be NULL), lQbnd[2D][B^(D-1)][nData], lFbnd[2D][B^(D-1)][V]. */
+void orc_lin_stp_split(const orc_solver* s, int flags, const double* luh, const double* dx, double dt,
+                       const double* PSi, double* lduh, double* lQhi_out, double* lQbnd, double* lFbnd) {
+  const int D = s->dim, B = s->b.order + 1, V = s->nvar, ND = s->nvar + s->npar;
+  const int BD = ipow(B, D), BF = ipow(B, D - 1);
+  const double* w = s->b.w;
+  const double* dudx = s->b.dudx;
+  const double* F0 = s->b.F0;
+  const double* F1 = s->b.F1;
+  double* Q = (double*)malloc(sizeof(double) * BD * ND);  /* lQi with the parameters */
+  double* Qh = (double*)malloc(sizeof(double) * BD * ND); /* lQhi */
+  double* Qn = (double*)malloc(sizeof(double) * BD * V);  /* lQi_next */
+  double* Fd = (double*)malloc(sizeof(double) * BD * V);  /* flux of one direction */
+  double* Fh = (double*)malloc(sizeof(double) * BD * V);  /* lFhi of one direction */
+  double* PS0 = (double*)calloc((size_t)BD * V, sizeof(double));
+  double derPS[ORC_MAXB], buf[ORC_MAXB];
+  memcpy(Q, luh, sizeof(double) * BD * ND);
+  if (PSi) /* tpl:181-188 */
+    for (int t = 0; t < B; t++)
+      for (int n = 0; n < BD; n++)
+        for (int v = 0; v < V; v++) Q[n * ND + v] += dt * w[t] * PSi[((size_t)t * BD + n) * V + v];
+  memcpy(Qh, Q, sizeof(double) * BD * ND);
+  for (int j = 0; j < B; j++) derPS[j] = F0[j];
+  double dtav = 0.5 * dt;
+  for (int t = 0; t < B; t++) {
+    memset(Qn, 0, sizeof(double) * BD * V);
+    for (int d = 0; d < D; d++) {
+      const double invDx = 1.0 / dx[d];
+      if (flags & LIN_FLUX) { /* tpl:204-223 (x), 237-257 (y), 271-289 (z) */
+        for (int n = 0; n < BD; n++) {
+          double F[3][MAXLV];
+          lin_flux(s->pde, &Q[n * ND], F);
+          for (int v = 0; v < V; v++) Fd[n * V + v] = F[d][v];
+        }
+        for (int n = 0; n < BD; n++) {
+          const int c = coord(n, d, B);
+          double R[3][MAXLV];
+          memset(R, 0, sizeof(R));
+          for (int m = 0; m < B; m++) {
+            const double cn = -(invDx * dudx[c * B + m]); /* negativeDudxT_by_dx */
+            const int nn = with_coord(n, d, m, B);
+            for (int v = 0; v < V; v++) R[d][v] += Fd[nn * V + v] * cn;
+          }
+          if (flags & LIN_MM) lin_mm(s->pde, &Q[n * ND], R);
+          for (int v = 0; v < V; v++) Qn[n * V + v] += R[d][v];
+        }
+      }
+      if (flags & LIN_NCP) /* tpl:225-235, 259-269, 291-301 */
+        for (int n = 0; n < BD; n++) {
+          const int c = coord(n, d, B);
+          double G[3][MAXLV], R[3][MAXLV];
+          memset(G, 0, sizeof(G));
+          for (int m = 0; m < B; m++) {
+            const double cp = invDx * dudx[c * B + m]; /* dudxT_by_dx */
+            const int nn = with_coord(n, d, m, B);
+            for (int v = 0; v < V; v++) G[d][v] += Q[nn * ND + v] * cp;
+          }
+          lin_ncp(s->pde, &Q[n * ND], (const double(*)[MAXLV])G, R);
+          for (int v = 0; v < V; v++) Qn[n * V + v] -= R[d][v];
+        }
+    }
+    if (PSi) { /* tpl:313-331 */
+      for (int k = 0; k < B; k++)
+        for (int n = 0; n < BD; n++)
+          for (int v = 0; v < V; v++) Qn[n * V + v] += derPS[k] * PSi[((size_t)k * BD + n) * V + v];
+      for (int j = 0; j < B; j++) buf[j] = 0.0;
+      for (int i = 0; i < B; i++)
+        for (int j = 0; j < B; j++) buf[j] += derPS[i] * dudx[i * B + j];
+      for (int j = 0; j < B; j++) derPS[j] = buf[j];
+    }
+    for (int n = 0; n < BD; n++) /* tpl:333-342 */
+      for (int v = 0; v < V; v++) {
+        Qh[n * ND + v] += dtav * Qn[n * V + v];
+        Q[n * ND + v] = Qn[n * V + v];
+      }
+    dtav *= dt / (t + 2);
+  }
+  /* extrapolation of lQhi with the parameters (tpl:350-470) */
+  memset(lQbnd, 0, sizeof(double) * 2 * D * BF * ND);
+  for (int n = 0; n < BD; n++)
+    for (int d = 0; d < D; d++) {
+      const int l = coord(n, d, B), fn = face_index(n, d, B, D);
+      for (int k = 0; k < ND; k++) {
+        lQbnd[((size_t)(2 * d) * BF + fn) * ND + k] += Qh[n * ND + k] * F0[l];
+        lQbnd[((size_t)(2 * d + 1) * BF + fn) * ND + k] += Qh[n * ND + k] * F1[l];
+      }
+    }
+  if (PSi) /* tpl:477-494: time integral of the moments, shared by the directions */
+    for (int t = 0; t < B; t++)
+      for (int k = 0; k < BD * V; k++) PS0[k] += w[t] / D * PSi[(size_t)t * BD * V + k];
+  memset(lduh, 0, sizeof(double) * BD * V);
+  memset(lFbnd, 0, sizeof(double) * 2 * D * BF * V);
+  for (int d = 0; d < D; d++) {
+    const double invDx = 1.0 / dx[d];
+    memcpy(Fh, PS0, sizeof(double) * BD * V);
+    if (flags & LIN_FLUX) {
+      for (int n = 0; n < BD; n++) {
+        double F[3][MAXLV];
+        lin_flux(s->pde, &Qh[n * ND], F);
+        for (int v = 0; v < V; v++) Fd[n * V + v] = F[d][v];
+      }
+      for (int n = 0; n < BD; n++) {
+        const int c = coord(n, d, B);
+        double R[3][MAXLV];
+        memset(R, 0, sizeof(R));
+        for (int m = 0; m < B; m++) {
+          const double cp = invDx * dudx[c * B + m];
+          const int nn = with_coord(n, d, m, B);
+          for (int v = 0; v < V; v++) R[d][v] += Fd[nn * V + v] * cp;
+        }
+        if (flags & LIN_MM) lin_mm(s->pde, &Qh[n * ND], R);
+        for (int v = 0; v < V; v++) Fh[n * V + v] += R[d][v];
+      }
+    }
+    if (flags & LIN_NCP)
+      for (int n = 0; n < BD; n++) {
+        const int c = coord(n, d, B);
+        double G[3][MAXLV], R[3][MAXLV];
+        memset(G, 0, sizeof(G));
+        for (int m = 0; m < B; m++) {
+          const double cp = invDx * dudx[c * B + m];
+          const int nn = with_coord(n, d, m, B);
+          for (int v = 0; v < V; v++) G[d][v] += Qh[nn * ND + v] * cp;
+        }
+        lin_ncp(s->pde, &Qh[n * ND], (const double(*)[MAXLV])G, R);
+        for (int v = 0; v < V; v++) Fh[n * V + v] += R[d][v];
+      }
+    for (int n = 0; n < BD; n++) {
+      const int x = coord(n, 0, B), y = coord(n, 1, B), l = coord(n, d, B), fn = face_index(n, d, B, D);
+      const double W = (D == 3) ? w[coord(n, 2, B)] * w[y] * w[x] : w[y] * w[x];
+      for (int v = 0; v < V; v++) {
+        lFbnd[((size_t)(2 * d) * BF + fn) * V + v] += Fh[n * V + v] * F0[l];
+        lFbnd[((size_t)(2 * d + 1) * BF + fn) * V + v] += Fh[n * V + v] * F1[l];
+        lduh[n * V + v] -= Fh[n * V + v] * W;
+      }
+    }
+  }
+  if (lQhi_out) memcpy(lQhi_out, Qh, sizeof(double) * BD * ND);
+  free(Q);
+  free(Qh);
+  free(Qn);
+  free(Fd);
+  free(Fh);
+  free(PS0);
 }
 
 /* volumeIntegralLinear (volumeIntegralLinear.cpph:21-70): lduh = -W sum_d lFhi[d] */
@@ -487,7 +664,18 @@ static void lin_predict_all(const orc_solver* s, const double* luh, int n, doubl
       const long ix = c % n, iy = (c / n) % n, iz = c / ((long)n * n);
       const double centre[3] = {(ix + 0.5) * h, (iy + 0.5) * h, (iz + 0.5) * h};
       PSi = (double*)malloc(sizeof(double) * (B + 1) * B3 * AV);
-      orc_lin_psi(s, centre, h, t, dt, src_x, amp, f0, t0, PSi);
+      if (s->split_ck)
+        orc_lin_psi_split(s, centre, h, t, dt, src_x, amp, f0, t0, PSi);
+      else
+        orc_lin_psi(s, centre, h, t, dt, src_x, amp, f0, t0, PSi);
+    }
+    if (s->split_ck) { /* lFb is overwritten by the Riemann fluctuations: scratch here */
+      double* lFs = (double*)malloc(sizeof(double) * 6 * B2 * AV);
+      orc_lin_stp_split(s, ACF, luh + c * B3 * AD, dxv, dt, PSi, lduh + c * B3 * AV, NULL, lQb + c * 6 * B2 * AD,
+                        lFb ? lFb + c * 6 * B2 * AV : lFs);
+      free(lFs);
+      free(PSi);
+      continue;
     }
     double* lFhi = (double*)malloc(sizeof(double) * 4 * B3 * AV);
     orc_lin_stp(s, ACF, luh + c * B3 * AD, dxv, dt, PSi, NULL, lFhi, lQb + c * 6 * B2 * AD,
@@ -740,6 +928,7 @@ double orc_lin_misfit_grad(const orc_solver* s, const double* u0, int n, double 
   const int B = s->b.order + 1, B3 = B * B * B;
   const long nc = (long)n * n * n;
   const size_t ns = (size_t)nc * B3 * AD;
+  if (s->split_ck) return NAN; /* the adjoint transposes the generic predictor only */
   double* states = (double*)malloc(sizeof(double) * ns * nsteps);
   double* u = (double*)malloc(sizeof(double) * ns);
   double* rec = (double*)malloc(sizeof(double) * nsteps * (nrec > 0 ? nrec : 1));

@@ -42,6 +42,14 @@
  *                   the Taylor terms of the time interpolant [test_oracle_linear];
  *                   the lFhi term sum_t w_t PSi[1+t] / D in every direction
  *                   [test_oracle_pins]
+ *   lin_stp_split   the split Cauchy-Kowalevski template (no reference test
+ *                   runs it): exact polynomial algebra of the levels, time
+ *                   average (the template's factors dt^k/(2 k!)), volume
+ *                   integral and faces for tensor-polynomial data; point-source
+ *                   levels = derivatives of the unit-interval interpolant;
+ *                   split - generic = the factor differences and the extra
+ *                   level; the mesh is second order in time, the generic one
+ *                   of design order [test_oracle_split_ck]
  *   linear adjoint  finite differences + dot-product (transpose) identity
  *                   [test_oracle_linear]
  *   amr projections constant preservation (the reference's
@@ -119,6 +127,9 @@ typedef struct orc_solver {
    * flux a Q (adv), ncp b . grad Q (ncp_b), source -kappa Q (kappa) */
   double ncp_b[3];
   double kappa;
+  /* linear path: 1 = the split Cauchy-Kowalevski predictor of the optimised
+   * kernels (fusedSPTVI_linear_split_ck_cpp.template) in orc_lin_run */
+  int split_ck;
 } orc_solver;
 
 /* ---- PDE hooks ---------------------------------------------------------- */
@@ -170,6 +181,13 @@ void orc_lin_surface(const orc_solver* s, double* lduh, const double* lFhbnd, co
 void orc_lin_riemann(const orc_solver* s, int flags, double* FL, double* FR, const double* QL,
                      const double* QR, int direction);
 double orc_ricker(double t, double f0, double t0);
+/* split Cauchy-Kowalevski predictor + volume integral of the optimised linear
+ * kernels (fusedSPTVI_linear_split_ck_cpp.template), its point-source moments
+ * PSi[B][B^D][V] at the Gauss time nodes (:112-155) */
+void orc_lin_stp_split(const orc_solver* s, int flags, const double* luh, const double* dx, double dt,
+                       const double* PSi, double* lduh, double* lQhi_out, double* lQbnd, double* lFbnd);
+void orc_lin_psi_split(const orc_solver* s, const double* centre, double h, double t, double dt,
+                       const double* xs, double amp, double f0, double t0, double* PSi);
 void orc_lin_psi(const orc_solver* s, const double* centre, double h, double t, double dt,
                  const double* xs, double amp, double f0, double t0, double* PSi);
 int orc_lin_run(const orc_solver* s, double* luh, int n, double h, double dt, int nsteps,

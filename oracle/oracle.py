@@ -55,6 +55,7 @@ class Solver(ctypes.Structure):
         ("opt_iterations", ctypes.c_int),
         ("ncp_b", ctypes.c_double * 3),
         ("kappa", ctypes.c_double),
+        ("split_ck", ctypes.c_int),
     ]
 
 
@@ -112,6 +113,8 @@ def _lib():
     lib.orc_ricker.argtypes = [D, D, D]
     lib.orc_ricker.restype = D
     lib.orc_lin_psi.argtypes = [sp, dp, D, D, D, dp, D, D, D, dp]
+    lib.orc_lin_psi_split.argtypes = [sp, dp, D, D, D, dp, D, D, D, dp]
+    lib.orc_lin_stp_split.argtypes = [sp, I, dp, dp, D, dp, dp, dp, dp, dp]
     lib.orc_lin_run.argtypes = [sp, dp, I, D, D, I, dp, D, D, D, I, dp, dp, dp, I]
     lib.orc_lin_run.restype = I
     lib.orc_lin_misfit_grad.argtypes = [sp, dp, I, D, D, I, dp, D, D, D, I, dp, dp, dp, I]
@@ -425,9 +428,31 @@ class LinearOracle(Oracle):
     """Linear (Cauchy-Kowalevski) kernels: acoustic 3D (4 variables, parameters
     rho, c) or the 2D elasticity test PDE (9 variables, lambda, mu, rho)."""
 
-    def __init__(self, dim=3, order=4, pde=PDE_ACOUSTIC, cfl=0.9):
+    def __init__(self, dim=3, order=4, pde=PDE_ACOUSTIC, cfl=0.9, split_ck=False):
         nvar, npar = (4, 2) if pde == PDE_ACOUSTIC else (9, 3)
         super().__init__(dim, order, nvar=nvar, npar=npar, pde=pde, cfl=cfl)
+        self.s.split_ck = 1 if split_ck else 0
+
+    def stp_split(self, luh, dx, dt, flags=LIN_FLUX | LIN_NCP | LIN_MM, PSi=None):
+        """Split Cauchy-Kowalevski predictor + volume integral of the optimised
+        linear kernels (fusedSPTVI_linear_split_ck_cpp.template); PSi [B][B^D][V]
+        at the Gauss time nodes (psi_split) or None."""
+        ND, V, D = self.nd, self.nvar, self.dim
+        luh = np.ascontiguousarray(luh, dtype=np.float64)
+        lduh = np.zeros(self.nspace * V)
+        lQhi = np.zeros(self.nspace * ND)
+        lQbnd = np.zeros(2 * D * self.nface * ND)
+        lFbnd = np.zeros(2 * D * self.nface * V)
+        ps = np.ascontiguousarray(PSi, dtype=np.float64) if PSi is not None else None
+        lib().orc_lin_stp_split(self.ps, flags, _p(luh), _p(np.array(dx, dtype=np.float64)), dt, _opt(ps),
+                                _p(lduh), _p(lQhi), _p(lQbnd), _p(lFbnd))
+        return dict(lduh=lduh, lQhi=lQhi, lQbnd=lQbnd, lFbnd=lFbnd)
+
+    def psi_split(self, centre, h, t, dt, xs, amp, f0, t0):
+        out = np.zeros(self.B * self.nspace * self.nvar)
+        lib().orc_lin_psi_split(self.ps, _p(np.array(centre, dtype=np.float64)), h, t, dt,
+                                _p(np.array(xs, dtype=np.float64)), amp, f0, t0, _p(out))
+        return out
 
     def stp(self, luh, dx, dt, flags=LIN_FLUX | LIN_NCP | LIN_MM, PSi=None, want_levels=False):
         B, D, V, ND = self.B, self.dim, self.nvar, self.nd
