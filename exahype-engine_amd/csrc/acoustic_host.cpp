@@ -28,6 +28,7 @@
 
 #include "../../include/exahype_amd.h"
 #include "engine.h"
+#include "trace.h"
 
 namespace exa {
 const KernelCfg& ctx_cfg(const exa_ctx* c);
@@ -230,6 +231,7 @@ exa::AcousticArgs args(exa_acoustic* ac, double* u) {
 // one forward step on u (step index `step`, or the device counter if < 0);
 // receivers recorded into rec[(step - rec_base) * nrec + r] when rec != nullptr
 int enqueue_forward(exa_acoustic* ac, double* u, int step, double* rec, hipEvent_t* ev = nullptr) {
+  exa::TraceRange tr("exa.acoustic.step");
   exa::AcousticArgs a = args(ac, u);
   a.step_override = step;
   const int o = ac->k.order;
@@ -596,6 +598,7 @@ int exa_acoustic_misfit_gradient(exa_acoustic* ac, int nsteps, int every, const 
   double* rec_base = ac->d_rec - long(s0) * R;
   // forward sweep with checkpoints
   if (!rc && hipEventRecord(ev[0], ac->stream) != hipSuccess) rc = fail(hipErrorUnknown, "event");
+  exa::TraceRange tr_fwd("exa.acoustic.adjoint.forward");
   for (int k = 0; !rc && k < K; k++) {
     if (k % every == 0) {
       const hipError_t e = hipMemcpyAsync(d_ck + su * (k / every), ac->d_u, sizeof(double) * su,
@@ -625,6 +628,7 @@ int exa_acoustic_misfit_gradient(exa_acoustic* ac, int nsteps, int every, const 
     if (e != hipSuccess) rc = fail(e, "adjoint init");
   }
   // backward sweep, segment by segment
+  exa::TraceRange tr_bwd("exa.acoustic.adjoint.backward");
   for (int j = nck - 1; !rc && j >= 0; j--) {
     const int k0 = j * every, k1 = (k0 + every < K) ? k0 + every : K;
     (void)hipEventRecord(ev[2 + 3 * j], ac->stream);

@@ -22,6 +22,7 @@
 
 #include "../../include/exahype_amd.h"
 #include "engine.h"
+#include "trace.h"
 
 namespace exa {
 const KernelCfg& ctx_cfg(const exa_ctx* c);
@@ -186,6 +187,7 @@ struct CellSet {
   }
 
   int enqueue(hipStream_t s) {
+    exa::TraceRange tr("exa.batch.step");
     double* dt = d_state + ninst;
     double* dtn = d_state + 2 * ninst;
     RC_OK(status(exa::launch_predictor(k, s, ncells, d_luh, d_h, dt, d_lduh, d_qb, d_fb, d_picard,

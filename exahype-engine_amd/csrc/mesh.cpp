@@ -21,6 +21,7 @@
 
 #include "../../include/exahype_amd.h"
 #include "engine.h"
+#include "trace.h"
 
 namespace exa {
 const KernelCfg& ctx_cfg(const exa_ctx* c);
@@ -110,6 +111,7 @@ exa::LaunchOpts ghost_opts(const exa_mesh* m) {
 // predictor, ev[1] after it, ev[2] after the Riemann solves (four-kernel form)
 // or before the fused corrector, ev[3] after the step.
 int enqueue_predict(exa_mesh* m, hipEvent_t* ev) {
+  exa::TraceRange tr("exa.mesh.predictor");
   double* st = m->d_state;
   if (ev) HIP_OK(hipEventRecord(ev[0], m->stream), "event");
   exa::LaunchOpts o;
@@ -125,6 +127,7 @@ int enqueue_predict(exa_mesh* m, hipEvent_t* ev) {
 }
 
 int enqueue_correct(exa_mesh* m, hipEvent_t* ev) {
+  exa::TraceRange tr("exa.mesh.corrector");
   double* st = m->d_state;
   int rc;
   if (m->fused) {
@@ -376,6 +379,7 @@ int exa_mesh_run(exa_mesh* m, int nsteps, int use_graph) {
     exa::set_error("exa_mesh_run: invalid argument");
     return EXA_ERR_ARG;
   }
+  exa::TraceRange tr(use_graph ? "exa.mesh.run(graph)" : "exa.mesh.run");
   if (m->timing) {
     // events are recorded on the launch stream; nothing synchronises here
     while (m->ev.size() < size_t(4) * nsteps) {
