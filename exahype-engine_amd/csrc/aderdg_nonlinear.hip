@@ -367,10 +367,16 @@ __device__ __forceinline__ void predictor_tile(
       for (int v = 0; v < ND; v++) asm volatile("" : "+v"(q[t][v]));
   }
 
-  R fh[D][V];  // time-averaged flux of the last executed iteration
+  // time-averaged flux of the last executed iteration: in registers, or (FH_LDS,
+  // FP64 3D order 5, trivial guess) in LDS [d][node][v], where its 30 registers
+  // would cost spills at two waves per SIMD (64.6 -> 60.0 ms at 81^3; the CERK
+  // form does not spill and is faster with registers); same operations
+  constexpr bool FH_LDS = sizeof(R) == 8 && D == 3 && B == 6 && CPB == 1 && !NCS && !CG;
+  __shared__ double fhs[FH_LDS ? D * NS * V : 1];
+  [[maybe_unused]] R fh[FH_LDS ? 1 : D][V];
   [[maybe_unused]] R sh[V];  // NCS: time-averaged source - ncp of the last executed iteration
 #pragma unroll
-  for (int d = 0; d < D; d++)
+  for (int d = 0; d < (FH_LDS ? 1 : D); d++)
 #pragma unroll
     for (int v = 0; v < V; v++) fh[d][v] = R(0);
   if constexpr (NCS) {
@@ -389,7 +395,12 @@ __device__ __forceinline__ void predictor_tile(
 #pragma unroll
       for (int d = 0; d < D; d++)
 #pragma unroll
-        for (int v = 0; v < V; v++) fh[d][v] = R(0);
+        for (int v = 0; v < V; v++) {
+          if constexpr (FH_LDS)
+            fhs[(d * NS + s) * V + v] = 0.0;
+          else
+            fh[d][v] = R(0);
+        }
       if constexpr (NCS) {
 #pragma unroll
         for (int v = 0; v < V; v++) sh[v] = R(0);
@@ -407,7 +418,10 @@ __device__ __forceinline__ void predictor_tile(
         for (int d = 0; d < D; d++)
 #pragma unroll
           for (int v = 0; v < V; v++) {
-            fh[d][v] += F[d][v] * wt;
+            if constexpr (FH_LDS)
+              fhs[(d * NS + s) * V + v] += F[d][v] * wt;  // own node: no other lane touches it
+            else
+              fh[d][v] += F[d][v] * wt;
             slab[(s * D + d) * V + v] = F[d][v];
           }
         if constexpr (NCS) {
@@ -507,7 +521,7 @@ __device__ __forceinline__ void predictor_tile(
 
   // ---- time averages -> LDS (FP64) ----
   double* Qh = reinterpret_cast<double*>(buf) + cslot * NS * (ND + D * V);
-  double* Fh = Qh + NS * ND;  // [d][node][v]
+  double* Fh = FH_LDS ? fhs : Qh + NS * ND;  // [d][node][v]
   __syncthreads();  // slabs no longer read
   if (valid) {
 #pragma unroll
@@ -517,10 +531,12 @@ __device__ __forceinline__ void predictor_tile(
       for (int t = 0; t < B; t++) a += static_cast<double>(q[t][v]) * w[t];
       Qh[s * ND + v] = a;
     }
+    if constexpr (!FH_LDS) {
 #pragma unroll
-    for (int d = 0; d < D; d++)
+      for (int d = 0; d < D; d++)
 #pragma unroll
-      for (int v = 0; v < V; v++) Fh[(d * NS + s) * V + v] = static_cast<double>(fh[d][v]);
+        for (int v = 0; v < V; v++) Fh[(d * NS + s) * V + v] = static_cast<double>(fh[d][v]);
+    }
   }
   __syncthreads();
   if (!valid) return;
