@@ -240,8 +240,9 @@ __device__ __forceinline__ void predictor_tile(
     const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect,
     int hdim) {
   using S = Shape<D, B, V, NP>;
-  constexpr int ND = S::ND, NS = S::NS, CPB = S::CPB, NSP = S::NSP;
+  constexpr int ND = S::ND, NS = S::NS, CPB = S::CPB, NSP = S::NSP, THREADS_ = S::THREADS;
   static_assert(!NCS || V == 5, "user PDE hooks: 5 variables");
+  static_assert(CPB > 1 || THREADS_ / 64 <= CPB * NSP, "wave sums fit in red");
   constexpr int SLAB = NS * D * V + (NCS ? NS * V : 0);  // one flux (+ state) slab, per cell
   constexpr int SLAB_BYTES = 2 * CPB * SLAB * int(sizeof(R));
   constexpr int END_BYTES = CPB * NS * (ND + D * V) * 8;  // time averages, FP64
@@ -495,18 +496,33 @@ __device__ __forceinline__ void predictor_tile(
         }
     }
     if (adaptive) {
-      // deterministic per-cell tree reduction of the squared update
-      if (inslot) {
-        red[cib * NSP + s] = sq;
-        if (s < NSP - NS) red[cib * NSP + NS + s] = 0.0;
-      }
-      __syncthreads();
+      double res;
+      if constexpr (CPB == 1) {
+        // one cell per workgroup: butterfly sum in each wave (every lane ends
+        // with the same bits: a + b == b + a), then the wave sums in order --
+        // deterministic, one barrier instead of the tree's log2(NSP) + 1
+        double v = sq;  // 0 on lanes past the cell
 #pragma unroll
-      for (int stride = NSP / 2; stride > 0; stride >>= 1) {
-        if (inslot && s < stride) red[cib * NSP + s] += red[cib * NSP + s + stride];
+        for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = v;
         __syncthreads();
+        res = 0.0;
+#pragma unroll
+        for (int i = 0; i < THREADS_ / 64; i++) res += red[i];
+      } else {
+        // deterministic per-cell tree reduction of the squared update
+        if (inslot) {
+          red[cib * NSP + s] = sq;
+          if (s < NSP - NS) red[cib * NSP + NS + s] = 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int stride = NSP / 2; stride > 0; stride >>= 1) {
+          if (inslot && s < stride) red[cib * NSP + s] += red[cib * NSP + s + stride];
+          __syncthreads();
+        }
+        res = inslot ? red[cib * NSP] : 0.0;
       }
-      const double res = inslot ? red[cib * NSP] : 0.0;
       const double tol = 1e-7;
       if (active && res < tol * tol) {
         active = false;
