@@ -241,7 +241,7 @@ __device__ __forceinline__ void predictor_tile(
     int hdim) {
   using S = Shape<D, B, V, NP>;
   constexpr int ND = S::ND, NS = S::NS, CPB = S::CPB, NSP = S::NSP, THREADS_ = S::THREADS;
-  static_assert(!NCS || V == 5, "user PDE hooks: 5 variables");
+  static_assert(!NCS || V == kUserNVar, "user PDE hooks: the user PDE's variables");
   static_assert(CPB > 1 || THREADS_ / 64 <= CPB * NSP, "wave sums fit in red");
   constexpr int SLAB = NS * D * V + (NCS ? NS * V : 0);  // one flux (+ state) slab, per cell
   constexpr int SLAB_BYTES = 2 * CPB * SLAB * int(sizeof(R));
@@ -453,7 +453,7 @@ __device__ __forceinline__ void predictor_tile(
         if constexpr (NCS) {
           // gradQ of time slice t (:113-176), then rhs += w_t W dt (S - ncp) (:323-367)
           const R* qs = slab + NS * D * V;
-          R gq[D][5];
+          R gq[D][V];
 #pragma unroll
           for (int d = 0; d < D; d++)
 #pragma unroll
@@ -465,7 +465,7 @@ __device__ __forceinline__ void predictor_tile(
                      tabD<R>(T, cc[d] * B + n);
               gq[d][v] = g;
             }
-          R Sv[5];
+          R Sv[V];
           pde_ncs<D, R>(P, q[t], gq, Sv);
           const R us = static_cast<R>(w[t] * W * dt);
           const R wt = static_cast<R>(w[t]);
@@ -644,25 +644,25 @@ __global__ __launch_bounds__(256) void riemann_kernel(long nfaces, const int4* _
     aL[k] = wave_sum(aL[k]);
     aR[k] = wave_sum(aR[k]);
   }
-  const double smax = fmax(pde_max_abs_eig<D>(P, aL, d), pde_max_abs_eig<D>(P, aR, d));
-  if constexpr (V == 5) {
+  const double smax = fmax(pde_max_abs_eig<D, V>(P, aL, d), pde_max_abs_eig<D, V>(P, aR, d));
+  if constexpr (V == kUserNVar) {
     if (pde_has_ncp(P)) {
       // fluctuations (riemannSolverNonlinear.cpph:95-120): the NCP of the node
       // average with gradQ[d] = QR - QL; FR = F - ncp/2, FL = F + ncp/2
       for (int j = lane; j < NF; j += 64) {
-        double gq[D][5], qa[5], ncp[5];
+        double gq[D][V], qa[V], ncp[V];
 #pragma unroll
         for (int e = 0; e < D; e++)
 #pragma unroll
-          for (int k = 0; k < 5; k++) gq[e][k] = 0.0;
+          for (int k = 0; k < V; k++) gq[e][k] = 0.0;
 #pragma unroll
-        for (int k = 0; k < 5; k++) {
+        for (int k = 0; k < V; k++) {
           gq[d][k] = QR[j * ND + k] - QL[j * ND + k];
           qa[k] = 0.5 * (QR[j * ND + k] + QL[j * ND + k]);
         }
         ExaUserPde::ncp<D, double>(P.u, qa, gq, ncp);
 #pragma unroll
-        for (int k = 0; k < 5; k++) {
+        for (int k = 0; k < V; k++) {
           const double fl = FL[j * V + k], fr = FR[j * V + k];
           const double ql = QL[j * ND + k], qr = QR[j * ND + k];
           const double flux = 0.5 * (fr + fl) - 0.5 * smax * (qr - ql);
@@ -700,22 +700,24 @@ template <int D, int B, int V, int NP>
 __device__ __forceinline__ double node_cfl_dt(const double* __restrict__ Q, const PdeParams& P,
                                               const double (&invDx)[D], double cflpnpm) {
   double den = 0.0;
-  if (P.id == kPdeEuler) {
-    constexpr double gamma = 1.4;
-    const double irho = 1. / Q[0];
-    const double j2 = (D == 3) ? Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3] : Q[1] * Q[1] + Q[2] * Q[2];
-    const double p = (gamma - 1) * (Q[4] - 0.5 * irho * j2);
-    const double c = sqrt(gamma * p * irho);
+  if constexpr (V == 5) {
+    if (P.id == kPdeEuler) {
+      constexpr double gamma = 1.4;
+      const double irho = 1. / Q[0];
+      const double j2 = (D == 3) ? Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3] : Q[1] * Q[1] + Q[2] * Q[2];
+      const double p = (gamma - 1) * (Q[4] - 0.5 * irho * j2);
+      const double c = sqrt(gamma * p * irho);
 #pragma unroll
-    for (int d = 0; d < D; d++) {
-      const double u_n = Q[d + 1] * irho;
-      const double l0 = fabs(u_n - c), l1 = fabs(u_n), l4 = fabs(u_n + c);
-      den += fmax(fmax(l0, l1), l4) * invDx[d];
+      for (int d = 0; d < D; d++) {
+        const double u_n = Q[d + 1] * irho;
+        const double l0 = fabs(u_n - c), l1 = fabs(u_n), l4 = fabs(u_n + c);
+        den += fmax(fmax(l0, l1), l4) * invDx[d];
+      }
+      return cflpnpm / den;
     }
-    return cflpnpm / den;
   }
 #pragma unroll
-  for (int d = 0; d < D; d++) den += pde_max_abs_eig<D>(P, Q, d) * invDx[d];
+  for (int d = 0; d < D; d++) den += pde_max_abs_eig<D, V>(P, Q, d) * invDx[d];
   return cflpnpm / den;
 }
 
@@ -1010,8 +1012,8 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
   }
   // Rusanov speed of each face from its two averages (riemannSolverNonlinear.cpph:51-72)
   if (valid && s < 2 * D)
-    sm[s] = fmax(pde_max_abs_eig<D>(P, avg + (s * 2 + 0) * ND, s >> 1),
-                 pde_max_abs_eig<D>(P, avg + (s * 2 + 1) * ND, s >> 1));
+    sm[s] = fmax(pde_max_abs_eig<D, V>(P, avg + (s * 2 + 0) * ND, s >> 1),
+                 pde_max_abs_eig<D, V>(P, avg + (s * 2 + 1) * ND, s >> 1));
   __syncthreads();
   // phase 3: fluxes into red[task][V] (red's partial products were consumed
   // in phase 2; phase 3 reads only sm and the face data)
@@ -1163,7 +1165,7 @@ struct Launch {
     const bool persistent = grid < tiles && !cerk;
     if (pde_has_ncs(k.pde)) {
       // source / non-conservative terms: FP64, trivial guess, one workgroup per tile, 5 variables
-      if constexpr (V == 5 && NP == 0) {
+      if constexpr (V == kUserNVar && NP == 0) {
         if (k.precision != 0 || cerk) return -1;
         hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, double, false, true>), dim3(tiles), dim3(S::THREADS), 0,
                            st, n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive,
@@ -1279,13 +1281,19 @@ struct Entry {
         &Launch<D, B, V, NP>::solution_update                                                       \
   }
 
-// Compiled shapes: Euler (5 variables) for orders 1..6 in 2D and 1..5 in 3D,
-// plus the reference unit-test shape (Order 3, one parameter).
+// Compiled shapes: 5 variables (Euler, advection, the default user PDE) for
+// orders 1..6 in 2D and 1..5 in 3D, plus the reference unit-test shape (Order
+// 3, one parameter); other variable counts for linear advection (1 and 3
+// variables) and for a user PDE that declares kNVar (orders 3 and 4; shapes
+// that coincide with the list above are found there first).
 const Entry kEntries[] = {
     EXA_ENTRY(2, 2, 5, 0), EXA_ENTRY(2, 3, 5, 0), EXA_ENTRY(2, 4, 5, 0), EXA_ENTRY(2, 5, 5, 0),
     EXA_ENTRY(2, 6, 5, 0), EXA_ENTRY(2, 7, 5, 0), EXA_ENTRY(3, 2, 5, 0), EXA_ENTRY(3, 3, 5, 0),
     EXA_ENTRY(3, 4, 5, 0), EXA_ENTRY(3, 5, 5, 0), EXA_ENTRY(3, 6, 5, 0), EXA_ENTRY(2, 4, 5, 1),
     EXA_ENTRY(3, 4, 5, 1),
+    EXA_ENTRY(2, 4, 1, 0), EXA_ENTRY(3, 4, 1, 0), EXA_ENTRY(3, 3, 3, 0),
+    EXA_ENTRY(2, 4, kUserNVar, 0), EXA_ENTRY(2, 5, kUserNVar, 0), EXA_ENTRY(3, 4, kUserNVar, 0),
+    EXA_ENTRY(3, 5, kUserNVar, 0),
 };
 
 const Entry* find_entry(const KernelCfg& k) {

@@ -286,6 +286,12 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
   } else if (!exa::shape_supported(dim, order, nvar, npar)) {
     set_error("exa_ctx_create: (dim, order, nvar, npar) not compiled into this library");
     return EXA_ERR_UNSUPPORTED;
+  } else if (pde == EXA_PDE_EULER && nvar != 5) {
+    set_error("exa_ctx_create: EXA_PDE_EULER has 5 variables");
+    return EXA_ERR_ARG;
+  } else if (pde == EXA_PDE_USER && nvar != exa::kUserNVar) {
+    set_error("exa_ctx_create: EXA_PDE_USER has the variable count of the compiled ExaUserPde (kNVar, default 5)");
+    return EXA_ERR_ARG;
   }
   exa_ctx* c = new exa_ctx;
   if (!exa::build_tables(order, &c->tab)) {

@@ -13,19 +13,22 @@
  * USER_PDE=path/to/my_pde.h) where that header defines struct ExaUserPde with
  * the members below; without it the test PDE defined here is compiled.
  *
- * Contract (5 conserved variables, no material parameters; R is double in
- * the FP64 form):
+ * Contract (N = kNVar conserved variables, 5 when the struct does not
+ * declare kNVar; no material parameters; R is double in the FP64 form):
+ *   static constexpr int kNVar;     // optional: the number of variables N
+ *                                   // (compiled for orders 3 and 4; 5 variables
+ *                                   // for every compiled order)
  *   static constexpr bool kNcp;     // nonConservativeProduct is used
  *   static constexpr bool kSource;  // algebraicSource is used
  *   template <int D, typename R> static __device__ void
- *       flux(const double* prm, const R* Q, R (&F)[D][5]);
+ *       flux(const double* prm, const R* Q, R (&F)[D][N]);
  *   template <int D> static __device__ double
  *       max_abs_eigenvalue(const double* prm, const double* Q, int d);
  *       // max_v |lambda_v(Q, d)| (the reference's eigenvalues followed by the
  *       // max of absolute values, riemannSolverNonlinear.cpph:61-70,
  *       // stableTimeStepSize.cpph:14-62)
  *   template <int D, typename R> static __device__ void
- *       ncp(const double* prm, const R* Q, const R (&gradQ)[D][5], R* BgradQ);
+ *       ncp(const double* prm, const R* Q, const R (&gradQ)[D][N], R* BgradQ);
  *   template <typename R> static __device__ void
  *       source(const double* prm, const R* Q, R* S);
  *       // algebraicSource without x and t (the device mesh keeps no cell
