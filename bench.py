@@ -119,7 +119,7 @@ def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, vari
     max_iter = 2 * (order + 1) if adaptive else picard
     iters = executed_iterations(pic, dim, max_iter, adaptive)
     # algorithmic predictor flops of one launch at the Picard counts of the last
-    # step (the survey's contraction model); the CERK guess's two flux
+    # step (the algorithmic contraction model of predictor_flops_per_cell); the CERK guess's two flux
     # divergences are reported apart
     flops = float(np.sum(predictor_flops_per_cell(dim, B, V, V, iters)))
     cerk_flops = float(np.sum(predictor_flops_per_cell(dim, B, V, V, iters, cerk=True))) - flops if guess else 0.0

@@ -1,7 +1,7 @@
 """GPU parity: the HIP library (through the C ABI) against the CPU oracle.
 
-FP64 bar: <= 1e-12 relative L2 per array (north-star parity bar for the FP64
-form) and identical per-cell Picard counts.  Inputs are seeded; sizes are
+FP64 bar: <= 1e-12 relative L2 per array (the parity bar of the FP64 form)
+and identical per-cell Picard counts.  Inputs are seeded; sizes are
 small enough for the oracle to finish in seconds.
 """
 import json
