@@ -45,6 +45,8 @@ struct exa_ctx {
   // Picard-guard list of the batched predictor: {listed, listed in total, list[guard_cap]}
   int* d_guard = nullptr;
   long guard_cap = 0;
+  // batched entry points: d_h holds one edge length per cell (0) or dx[D] per cell (dim)
+  int cell_dx = 0;
 };
 
 namespace {
@@ -381,6 +383,19 @@ int exa_ctx_set_advection(exa_ctx* c, const double* a) {
     return EXA_ERR_ARG;
   }
   for (int d = 0; d < c->k.dim; d++) c->k.pde.a[d] = a[d];
+  return EXA_OK;
+}
+
+int exa_ctx_set_cell_dx(exa_ctx* c, int per_direction) {
+  if (!c || (per_direction != 0 && per_direction != 1)) {
+    set_error("exa_ctx_set_cell_dx: invalid argument (0 or 1)");
+    return EXA_ERR_ARG;
+  }
+  if (is_linear(c)) {
+    set_error("exa_ctx_set_cell_dx: the batched entry points serve the nonlinear path");
+    return EXA_ERR_UNSUPPORTED;
+  }
+  c->cell_dx = per_direction ? c->k.dim : 0;
   return EXA_OK;
 }
 
@@ -1147,6 +1162,7 @@ int exa_batch_predictor(exa_ctx* c, void* stream, long ncells, const double* d_l
     return EXA_ERR_ARG;
   }
   exa::LaunchOpts o;
+  o.hdim = c->cell_dx;
   if (ncells > 0 && exa::predictor_uses_guard(c->k)) {
     const int rc = ensure_guard(c, ncells);
     if (rc) return rc;
@@ -1170,6 +1186,7 @@ int exa_batch_predictor_noalloc(exa_ctx* c, void* stream, long ncells, const dou
   }
   const hipStream_t st = static_cast<hipStream_t>(stream);
   exa::LaunchOpts o;
+  o.hdim = c->cell_dx;
   if (ncells > 0 && exa::predictor_uses_guard(c->k)) {
     int* g = nullptr;
     const int rc = scratch_for_stream(st, &g);
@@ -1199,8 +1216,10 @@ int exa_batch_corrector(exa_ctx* c, void* stream, long ncells, double* d_luh, do
     set_error("exa_batch_corrector: invalid argument");
     return EXA_ERR_ARG;
   }
+  exa::LaunchOpts o;
+  o.hdim = c->cell_dx;
   return launch_status(exa::launch_corrector(c->k, static_cast<hipStream_t>(stream), ncells, d_luh, d_lduh,
-                                             d_lFhbnd, d_h, d_dt, d_dt_next),
+                                             d_lFhbnd, d_h, d_dt, d_dt_next, o),
                        "exa_batch_corrector");
 }
 
@@ -1210,8 +1229,10 @@ int exa_batch_stable_dt(exa_ctx* c, void* stream, long ncells, const double* d_l
     set_error("exa_batch_stable_dt: invalid argument");
     return EXA_ERR_ARG;
   }
+  exa::LaunchOpts o;
+  o.hdim = c->cell_dx;
   return launch_status(exa::launch_stable_dt(c->k, static_cast<hipStream_t>(stream), ncells, d_luh, d_h,
-                                             d_dt_out),
+                                             d_dt_out, o),
                        "exa_batch_stable_dt");
 }
 

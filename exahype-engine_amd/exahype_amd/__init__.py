@@ -55,6 +55,7 @@ _sig("exa_ctx_create", [ctypes.POINTER(_cp), ctypes.c_int, ctypes.c_int, ctypes.
                         ctypes.c_int, ctypes.c_int, ctypes.c_double])
 _sig("exa_ctx_destroy", [_cp])
 _sig("exa_ctx_set_advection", [_cp, _dp])
+_sig("exa_ctx_set_cell_dx", [_cp, ctypes.c_int])
 _sig("exa_ctx_set_pde_params", [_cp, _dp, ctypes.c_int])
 _sig("exa_ctx_set_point_source", [_cp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double])
 _sig("exa_ctx_set_predictor_variant", [_cp, ctypes.c_int])
@@ -293,6 +294,10 @@ class Context:
 
     def set_advection(self, a):
         _check(_lib.exa_ctx_set_advection(self.h, _p(np.ascontiguousarray(a, dtype=np.float64))))
+
+    def set_cell_dx(self, per_direction):
+        """Batched entry points: h is [ncells] (False) or the dx vector per cell [ncells][dim] (True)."""
+        _check(_lib.exa_ctx_set_cell_dx(self.h, int(bool(per_direction))))
 
     # -- per-cell compatibility API ---------------------------------------
     def fusedSpaceTimePredictorVolumeIntegral(self, luh, dx, t, dt, addVolumeIntegralResultToUpdate=True,

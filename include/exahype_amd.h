@@ -314,6 +314,14 @@ int exa_stableTimeStepSize(exa_ctx* ctx, const double* luh, const double* dx, do
  * from the device scalar d_dt so that a captured step needs no host value.
  * ------------------------------------------------------------------------- */
 
+/* Cell extents of the batched entry points: per_direction = 0 (default): d_h
+ * holds one edge length per cell [ncells]; 1: d_h holds the reference's dx
+ * vector per cell [ncells][dim] (ADERDGSolver.h:2542-2553 passes dx[D]), used
+ * per direction by the predictor (node-column kernel), the face integrals and
+ * the stable dt.  The FP32 line and BF16 tile kernels take cubic cells: with 1
+ * an FP32 context runs the FP32 node-column kernel and BF16 is refused. */
+int exa_ctx_set_cell_dx(exa_ctx* ctx, int per_direction);
+
 /* Predictor + volume integral for ncells cells (reference row: a4 + a5).
  * d_picard (nullable): per-cell Picard count.  With the Picard guard of the
  * FP32 one-iteration CERK form (exa_ctx_set_picard_guard) the cell list of
