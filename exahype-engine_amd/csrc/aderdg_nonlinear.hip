@@ -359,7 +359,7 @@ __device__ __forceinline__ void predictor_tile(
       for (int v = 0; v < V; v++)
         q[t][v] = static_cast<R>(u[v]) - (rdt * xt * K1[v]) - (R(0.5) * rdt * xt * xt * (K2[v] - K1[v]));
     }
-  } else {
+  } else if constexpr (NCS) {
     // trivial guess: hide that the B slices start equal, or the compiler keeps
     // the copies folded and its schedule of the first iteration spills
 #pragma unroll
@@ -386,7 +386,122 @@ __device__ __forceinline__ void predictor_tile(
   }
   bool active = valid;
   int iters = maxIter;
-  for (int iter = 0; iter < maxIter; iter++) {
+
+  // Reduction of the squared update of iteration iter over the cell and the
+  // adaptive rule (stop below 1e-7); true when no cell of the block iterates further.
+  auto finished = [&](int iter, double sq) -> bool {
+    if (!adaptive) {
+      __syncthreads();
+      return false;
+    }
+    double res;
+    if constexpr (CPB == 1) {
+      // one cell per workgroup: butterfly sum in each wave (every lane ends
+      // with the same bits: a + b == b + a), then the wave sums in order --
+      // deterministic, one barrier instead of the tree's log2(NSP) + 1
+      double v = sq;  // 0 on lanes past the cell
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+      if ((tid & 63) == 0) red[tid >> 6] = v;
+      __syncthreads();
+      res = 0.0;
+#pragma unroll
+      for (int i = 0; i < THREADS_ / 64; i++) res += red[i];
+    } else {
+      // deterministic per-cell tree reduction of the squared update
+      if (inslot) {
+        red[cib * NSP + s] = sq;
+        if (s < NSP - NS) red[cib * NSP + NS + s] = 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int stride = NSP / 2; stride > 0; stride >>= 1) {
+        if (inslot && s < stride) red[cib * NSP + s] += red[cib * NSP + s + stride];
+        __syncthreads();
+      }
+      res = inslot ? red[cib * NSP] : 0.0;
+    }
+    const double tol = 1e-7;
+    if (active && res < tol * tol) {
+      active = false;
+      // reference return conventions; optimised kernels: min(iter + 1, MaxIterations)
+      iters = dialect ? (iter + 1 < maxIter ? iter + 1 : maxIter) : ((D == 3) ? iter : maxIter + 2);
+    }
+    return !__syncthreads_or(active);
+  };
+
+  int first = 0;
+  if constexpr (!CG && !NCS) {
+    // Trivial guess: the B time slices of the first iterate all equal luh, so
+    // iteration 0 forms their flux, its time average and the stiffness sum once:
+    // rhs[t] = W F0[t] luh - w[t] sum_d sum_n (ta tb dt / dx)_d Kxi[n][c_d] F_d(luh).
+    // Its squared update is taken against luh, so the B copies of luh are never held.
+    if (maxIter > 0) {
+      R acc[B][V];
+#pragma unroll
+      for (int t = 0; t < B; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) acc[t][v] = R(0);
+      R* slab = reinterpret_cast<R*>(buf) + cslot * SLAB;
+      if (active) {
+        R F[D][V];
+        pde_flux<D, R>(P, q[0], F);
+#pragma unroll
+        for (int d = 0; d < D; d++)
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            R a = R(0);
+#pragma unroll
+            for (int t = 0; t < B; t++) a += F[d][v] * static_cast<R>(w[t]);
+            if constexpr (FH_LDS)
+              fhs[(d * NS + s) * V + v] = a;
+            else
+              fh[d][v] = a;
+            slab[(s * D + d) * V + v] = F[d][v];
+          }
+      }
+      __syncthreads();
+      double sq = 0.0;
+      if (active) {
+        R g[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) g[v] = R(0);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          const R upd = static_cast<R>(ta[d] * tb[d] * dt * invDx[d]);
+#pragma unroll
+          for (int n = 0; n < B; n++) {
+            const R ukc = upd * kxs[n * B + cc[d]];
+#pragma unroll
+            for (int v = 0; v < V; v++) g[v] += ukc * slab[((base[d] + n * stride_of<D, B>(d)) * D + d) * V + v];
+          }
+        }
+        const R iw = static_cast<R>(iweight);
+#pragma unroll
+        for (int t = 0; t < B; t++) {
+          const R rw = static_cast<R>(W * T->F0[t]);
+          const R wt = static_cast<R>(w[t]);
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            const R r = rw * static_cast<R>(u[v]) - wt * g[v];
+#pragma unroll
+            for (int tp = 0; tp < B; tp++) acc[tp][v] += iw * r * tabI<R>(T, tp * B + t);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < B; t++)
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            const double dq = static_cast<double>(acc[t][v]) - static_cast<double>(static_cast<R>(u[v]));
+            sq += dq * dq;
+            q[t][v] = acc[t][v];
+          }
+      }
+      first = finished(0, sq) ? maxIter : 1;
+    }
+  }
+
+  for (int iter = first; iter < maxIter; iter++) {
     R acc[B][V];
 #pragma unroll
     for (int t = 0; t < B; t++)
@@ -442,11 +557,13 @@ __device__ __forceinline__ void predictor_tile(
           const R upd = static_cast<R>(w[t] * ta[d] * tb[d] * dt * invDx[d]);
 #pragma unroll
           for (int n = 0; n < B; n++) {
-            const R kc = kxs[n * B + cc[d]];  // Kxi[n][c_d]
+            // Kxi[n][c_d] times the update factor, once per (d, n): one FMA per
+            // flux value instead of a multiply and an FMA
+            const R ukc = upd * kxs[n * B + cc[d]];
 #pragma unroll
             for (int v = 0; v < V; v++) {
               const R f = slab[((base[d] + n * stride_of<D, B>(d)) * D + d) * V + v];
-              r[v] -= upd * f * kc;
+              r[v] -= ukc * f;
             }
           }
         }
@@ -495,44 +612,7 @@ __device__ __forceinline__ void predictor_tile(
           q[t][v] = acc[t][v];
         }
     }
-    if (adaptive) {
-      double res;
-      if constexpr (CPB == 1) {
-        // one cell per workgroup: butterfly sum in each wave (every lane ends
-        // with the same bits: a + b == b + a), then the wave sums in order --
-        // deterministic, one barrier instead of the tree's log2(NSP) + 1
-        double v = sq;  // 0 on lanes past the cell
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-        if ((tid & 63) == 0) red[tid >> 6] = v;
-        __syncthreads();
-        res = 0.0;
-#pragma unroll
-        for (int i = 0; i < THREADS_ / 64; i++) res += red[i];
-      } else {
-        // deterministic per-cell tree reduction of the squared update
-        if (inslot) {
-          red[cib * NSP + s] = sq;
-          if (s < NSP - NS) red[cib * NSP + NS + s] = 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int stride = NSP / 2; stride > 0; stride >>= 1) {
-          if (inslot && s < stride) red[cib * NSP + s] += red[cib * NSP + s + stride];
-          __syncthreads();
-        }
-        res = inslot ? red[cib * NSP] : 0.0;
-      }
-      const double tol = 1e-7;
-      if (active && res < tol * tol) {
-        active = false;
-        // reference return conventions; optimised kernels: min(iter + 1, MaxIterations)
-        iters = dialect ? (iter + 1 < maxIter ? iter + 1 : maxIter) : ((D == 3) ? iter : maxIter + 2);
-      }
-      if (!__syncthreads_or(active)) break;
-    } else {
-      __syncthreads();
-    }
+    if (finished(iter, sq)) break;
   }
 
   // ---- time averages -> LDS (FP64) ----
