@@ -190,35 +190,41 @@ __device__ __forceinline__ void euler_flux_dir<float>(const float* Q, float irho
 // At one point, from its time line q[t][v] (t ascending, FP32):
 // lFhi_d = sum_t w_t F_d(q[t]), stored at af[(d NS) V + pv + v] (pv = point
 // index * V), and fc_d = sum_t c_t F_d(q[t]) with c = T->cwf, returned.
+// Of the 15 Euler flux components 12 are distinct: the mass fluxes are the
+// momenta, the momentum fluxes are symmetric (F_x[2] = F_y[1] = q_x q_y / rho,
+// F_x[3] = F_z[1], F_y[3] = F_z[2]); each distinct one is formed and summed once.
 template <int B>
 __device__ __forceinline__ void time_line_averages(const float (&q)[B][LShape<B>::V], const DevTables* __restrict__ T,
                                                    float* af, int pv, float (&fc)[3][LShape<B>::V]) {
-  constexpr int V = 5, NS = B * B * B;
-  float fh[3][V];
+  constexpr int V = 5, NS = B * B * B, NDIST = 12;
+  // distinct components: q_x, q_y, q_z, xx, yy, zz, xy, xz, yz, ex, ey, ez
+  float sw[NDIST], sc[NDIST];
 #pragma unroll
-  for (int d = 0; d < 3; d++)
-#pragma unroll
-    for (int v = 0; v < V; v++) fh[d][v] = fc[d][v] = 0.f;
+  for (int i = 0; i < NDIST; i++) sw[i] = sc[i] = 0.f;
 #pragma unroll
   for (int t = 0; t < B; t++) {
     const float wt = T->wf[t], ct = T->cwf[t];
+    const float* Q = q[t];
     float ir, pr;
-    euler_prim<float>(q[t], ir, pr);
+    euler_prim<float>(Q, ir, pr);
+    const float ux = ir * Q[1], uy = ir * Q[2], uz = ir * Q[3], e = Q[4] + pr;
+    const float f[NDIST] = {Q[1],          Q[2],          Q[3],      Q[1] * ux + pr, Q[2] * uy + pr, Q[3] * uz + pr,
+                            Q[2] * ux,     Q[3] * ux,     Q[3] * uy, e * ux,         e * uy,         e * uz};
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-      float F[V];
-      euler_flux_dir<float>(q[t], ir, pr, d, F);
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        fh[d][v] += F[v] * wt;
-        fc[d][v] += F[v] * ct;
-      }
+    for (int i = 0; i < NDIST; i++) {
+      sw[i] += f[i] * wt;
+      sc[i] += f[i] * ct;
     }
   }
+  // F_d[v] -> distinct component
+  constexpr int map[3][V] = {{0, 3, 6, 7, 9}, {1, 6, 4, 8, 10}, {2, 7, 8, 5, 11}};
 #pragma unroll
   for (int d = 0; d < 3; d++)
 #pragma unroll
-    for (int v = 0; v < V; v++) af[d * NS * V + pv + v] = fh[d][v];
+    for (int v = 0; v < V; v++) {
+      af[d * NS * V + pv + v] = sw[map[d][v]];
+      fc[d][v] = sc[map[d][v]];
+    }
 }
 
 // Component v of the FP32 x flux, the same operations as euler_flux_dir<float>(.., 0, ..).
