@@ -219,6 +219,29 @@ __device__ __forceinline__ void node_volume_and_faces(
   if (s == 0 && picard) picard[cell] = iters;
 }
 
+// Node weight W (reference: w[z] w[y] w[x], 2D w[y] w[x]), the two transverse
+// weights per direction in the reference's multiplication order, the node's
+// coordinate along d and the first node of its line along d.
+template <int D, int B>
+__device__ __forceinline__ void node_geometry(int s, const double* w, double& W, double (&ta)[D], double (&tb)[D],
+                                              int (&cc)[D], int (&base)[D]) {
+  int z, y, x;
+  node_zyx<D, B>(s, z, y, x);
+  W = (D == 3) ? w[z] * w[y] * w[x] : w[y] * w[x];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    if (D == 3) {
+      ta[d] = d == 2 ? w[y] : w[z];
+      tb[d] = d == 0 ? w[y] : w[x];
+    } else {
+      ta[d] = d == 0 ? w[y] : w[x];
+      tb[d] = 1.0;
+    }
+    cc[d] = coord<D, B>(d, z, y, x);
+    base[d] = s - cc[d] * stride_of<D, B>(d);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Space-time predictor + volume integral
 // ---------------------------------------------------------------------------
@@ -283,22 +306,10 @@ __device__ __forceinline__ void predictor_tile(
 
   // node weight (reference: w[z]*w[y]*w[x], 2D w[y]*w[x]) and the two
   // transverse weights per direction in the reference's multiplication order
-  const double W = (D == 3) ? w[z] * w[y] * w[x] : w[y] * w[x];
-  const double iweight = 1.0 / W;
-  double ta[D], tb[D];
+  double W, ta[D], tb[D];
   int cc[D], base[D];
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    if (D == 3) {
-      ta[d] = d == 2 ? w[y] : w[z];
-      tb[d] = d == 0 ? w[y] : w[x];
-    } else {
-      ta[d] = d == 0 ? w[y] : w[x];
-      tb[d] = 1.0;
-    }
-    cc[d] = coord<D, B>(d, z, y, x);
-    base[d] = s - cc[d] * stride_of<D, B>(d);
-  }
+  node_geometry<D, B>(s, w, W, ta, tb, cc, base);
+  const double iweight = 1.0 / W;
   // Kxi in LDS: a node reads its stiffness column Kxi[n][c_d] per use instead
   // of holding D B entries in registers (they would cost the second wave per SIMD)
   __shared__ R kxs[B * B];
@@ -427,6 +438,10 @@ __device__ __forceinline__ void predictor_tile(
       // reference return conventions; optimised kernels: min(iter + 1, MaxIterations)
       iters = dialect ? (iter + 1 < maxIter ? iter + 1 : maxIter) : ((D == 3) ? iter : maxIter + 2);
     }
+    // one cell per workgroup: every thread holds the same res, so the decision
+    // is uniform without a second barrier (red is rewritten only after the
+    // next iteration's time-slice barriers)
+    if constexpr (CPB == 1) return !(blk < ncells && res >= tol * tol);  // the block's cell is blk
     return !__syncthreads_or(active);
   };
 
@@ -643,8 +658,19 @@ __device__ __forceinline__ void predictor_tile(
 #pragma unroll
     for (int v = 0; v < V; v++) shd[v] = static_cast<double>(sh[v]);
   }
-  node_volume_and_faces<D, B, V, NP>(Qh, Fh, s, cell, ta, tb, cc, base, invDx, W, dt, iters, NCS ? shd : nullptr, T,
-                                     lduh, lQhbnd, lFhbnd, picard, luh_upd);
+  // the per-node constants again, from a node index the optimiser cannot
+  // equate with s: the loop's copies end with the loop instead of being held
+  // (spilled at two waves per SIMD) across it
+  int s2 = s;
+  asm volatile("" : "+v"(s2));
+  double W2, ta2[D], tb2[D], invDx2[D];
+  int cc2[D], base2[D];
+  node_geometry<D, B>(s2, w, W2, ta2, tb2, cc2, base2);
+#pragma unroll
+  for (int d = 0; d < D; d++) invDx2[d] = 1.0 / hcell[hdim ? cell * hdim + d : cell];
+  const double dt2 = dtp[cinst ? cinst[cell] : 0];
+  node_volume_and_faces<D, B, V, NP>(Qh, Fh, s2, cell, ta2, tb2, cc2, base2, invDx2, W2, dt2, iters,
+                                     NCS ? shd : nullptr, T, lduh, lQhbnd, lFhbnd, picard, luh_upd);
 }
 
 // Predictor + volume integral, one workgroup per tile of CPB cells.  Two
