@@ -219,6 +219,15 @@ __device__ __forceinline__ void node_volume_and_faces(
   if (s == 0 && picard) picard[cell] = iters;
 }
 
+// The 3D Euler flux has 12 distinct components: the mass fluxes are the
+// momenta (slots 0-2), the momentum fluxes are symmetric (diagonal 3-5,
+// xy, xz, yz 6-8), the energy fluxes 9-11.  euler_slot(d, v) = slot of F_d[v];
+// the first occurrence (d smallest) of a slot is its canonical component.
+__host__ __device__ constexpr int euler_slot(int d, int v) {
+  return v == 0 ? d : (v == 4 ? 9 + d : (v == d + 1 ? 3 + d : 4 + d + v));
+}
+__host__ __device__ constexpr bool euler_canonical(int d, int v) { return !(v >= 1 && v <= 3 && v < d + 1); }
+
 // Node weight W (reference: w[z] w[y] w[x], 2D w[y] w[x]), the two transverse
 // weights per direction in the reference's multiplication order, the node's
 // coordinate along d and the first node of its line along d.
@@ -255,7 +264,7 @@ __device__ __forceinline__ void node_geometry(int s, const double* w, double& W,
 // gradQ = dudx lQi / dx, adds w_t W dt (S - B(Q) gradQ) to its rhs and
 // accumulates the time-averaged S, which the volume integral adds as W lShi
 // (volumeIntegralNonlinear.cpph:204-222).
-template <int D, int B, int V, int NP, typename R, bool CG, bool NCS = false>
+template <int D, int B, int V, int NP, typename R, bool CG, bool NCS = false, bool EUS = false>
 __device__ __forceinline__ void predictor_tile(
     long blk, long ncells, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
@@ -266,10 +275,22 @@ __device__ __forceinline__ void predictor_tile(
   constexpr int ND = S::ND, NS = S::NS, CPB = S::CPB, NSP = S::NSP, THREADS_ = S::THREADS;
   static_assert(!NCS || V == kUserNVar, "user PDE hooks: the user PDE's variables");
   static_assert(CPB > 1 || THREADS_ / 64 <= CPB * NSP, "wave sums fit in red");
-  constexpr int SLAB = NS * D * V + (NCS ? NS * V : 0);  // one flux (+ state) slab, per cell
+  // EUS (3D Euler, FP64): a slab holds the 12 distinct flux components per
+  // node, in slot order, and the time-averaged flux accumulates the canonical
+  // ones in registers (the reference's F_y[1] = rho^-1 q_y q_x, F_z[1], F_z[2]
+  // are taken from F_x[2], F_x[3], F_y[3]: the same products in another order)
+  static_assert(!EUS || (D == 3 && V == 5 && !NCS), "Euler flux slots: 3D, five variables");
+  // slab values per node (EUS: 12 slots padded to 13, an odd stride in doubles,
+  // so that the nodes a wave reads spread over the LDS banks)
+  constexpr int SW = EUS ? 13 : D * V;
+  constexpr int SLAB = NS * SW + (NCS ? NS * V : 0);  // one flux (+ state) slab, per cell
   constexpr int SLAB_BYTES = 2 * CPB * SLAB * int(sizeof(R));
   constexpr int END_BYTES = CPB * NS * (ND + D * V) * 8;  // time averages, FP64
   constexpr int BUF_BYTES = SLAB_BYTES > END_BYTES ? SLAB_BYTES : END_BYTES;
+  // slab position of F_d[v] of node n; `keep` is false for the flux components
+  // EUS stores once under another (d, v)
+  auto sidx = [](int n, int d, int v) { return EUS ? n * SW + euler_slot(d, v) : (n * D + d) * V + v; };
+  auto keep = [](int d, int v) { return !EUS || euler_canonical(d, v); };
   __shared__ __align__(16) unsigned char buf[BUF_BYTES];
   __shared__ double red[CPB * NSP];
 
@@ -332,7 +353,8 @@ __device__ __forceinline__ void predictor_tile(
 #pragma unroll
         for (int d = 0; d < D; d++)
 #pragma unroll
-          for (int v = 0; v < V; v++) slab[(s * D + d) * V + v] = F[d][v];
+          for (int v = 0; v < V; v++)
+            if (keep(d, v)) slab[sidx(s, d, v)] = F[d][v];
       }
       __syncthreads();
       R K[V];
@@ -346,7 +368,7 @@ __device__ __forceinline__ void predictor_tile(
             R a = R(0);
 #pragma unroll
             for (int n = 0; n < B; n++)
-              a += slab[((base[d] + n * stride_of<D, B>(d)) * D + d) * V + v] *
+              a += slab[sidx(base[d] + n * stride_of<D, B>(d), d, v)] *
                    (static_cast<R>(invDx[d]) * tabD<R>(T, cc[d] * B + n));
             K[v] += a;
           }
@@ -383,7 +405,7 @@ __device__ __forceinline__ void predictor_tile(
   // FP64 3D order 5, trivial guess) in LDS [d][node][v], where its 30 registers
   // would cost spills at two waves per SIMD (64.6 -> 60.0 ms at 81^3; the CERK
   // form does not spill and is faster with registers); same operations
-  constexpr bool FH_LDS = sizeof(R) == 8 && D == 3 && B == 6 && CPB == 1 && !NCS && !CG;
+  constexpr bool FH_LDS = sizeof(R) == 8 && D == 3 && B == 6 && CPB == 1 && !NCS && !CG && !EUS;
   __shared__ double fhs[FH_LDS ? D * NS * V : 1];
   [[maybe_unused]] R fh[FH_LDS ? 1 : D][V];
   [[maybe_unused]] R sh[V];  // NCS: time-averaged source - ncp of the last executed iteration
@@ -470,9 +492,9 @@ __device__ __forceinline__ void predictor_tile(
             for (int t = 0; t < B; t++) a += F[d][v] * static_cast<R>(w[t]);
             if constexpr (FH_LDS)
               fhs[(d * NS + s) * V + v] = a;
-            else
+            else if (keep(d, v))
               fh[d][v] = a;
-            slab[(s * D + d) * V + v] = F[d][v];
+            if (keep(d, v)) slab[sidx(s, d, v)] = F[d][v];
           }
       }
       __syncthreads();
@@ -488,7 +510,7 @@ __device__ __forceinline__ void predictor_tile(
           for (int n = 0; n < B; n++) {
             const R ukc = upd * kxs[n * B + cc[d]];
 #pragma unroll
-            for (int v = 0; v < V; v++) g[v] += ukc * slab[((base[d] + n * stride_of<D, B>(d)) * D + d) * V + v];
+            for (int v = 0; v < V; v++) g[v] += ukc * slab[sidx(base[d] + n * stride_of<D, B>(d), d, v)];
           }
         }
         const R iw = static_cast<R>(iweight);
@@ -551,13 +573,13 @@ __device__ __forceinline__ void predictor_tile(
           for (int v = 0; v < V; v++) {
             if constexpr (FH_LDS)
               fhs[(d * NS + s) * V + v] += F[d][v] * wt;  // own node: no other lane touches it
-            else
+            else if (keep(d, v))
               fh[d][v] += F[d][v] * wt;
-            slab[(s * D + d) * V + v] = F[d][v];
+            if (keep(d, v)) slab[sidx(s, d, v)] = F[d][v];
           }
         if constexpr (NCS) {
 #pragma unroll
-          for (int v = 0; v < V; v++) slab[NS * D * V + s * V + v] = q[t][v];
+          for (int v = 0; v < V; v++) slab[NS * SW + s * V + v] = q[t][v];
         }
       }
       __syncthreads();
@@ -577,14 +599,14 @@ __device__ __forceinline__ void predictor_tile(
             const R ukc = upd * kxs[n * B + cc[d]];
 #pragma unroll
             for (int v = 0; v < V; v++) {
-              const R f = slab[((base[d] + n * stride_of<D, B>(d)) * D + d) * V + v];
+              const R f = slab[sidx(base[d] + n * stride_of<D, B>(d), d, v)];
               r[v] -= ukc * f;
             }
           }
         }
         if constexpr (NCS) {
           // gradQ of time slice t (:113-176), then rhs += w_t W dt (S - ncp) (:323-367)
-          const R* qs = slab + NS * D * V;
+          const R* qs = slab + NS * SW;
           R gq[D][V];
 #pragma unroll
           for (int d = 0; d < D; d++)
@@ -646,7 +668,11 @@ __device__ __forceinline__ void predictor_tile(
 #pragma unroll
       for (int d = 0; d < D; d++)
 #pragma unroll
-        for (int v = 0; v < V; v++) Fh[(d * NS + s) * V + v] = static_cast<double>(fh[d][v]);
+        for (int v = 0; v < V; v++) {
+          // EUS: the component stored once is read from its canonical (d, v)
+          const int dd = keep(d, v) ? d : v - 1, vv = keep(d, v) ? v : d + 1;
+          Fh[(d * NS + s) * V + v] = static_cast<double>(fh[dd][vv]);
+        }
     }
   }
   __syncthreads();
@@ -677,15 +703,15 @@ __device__ __forceinline__ void predictor_tile(
 // waves per SIMD (at most 256 registers per lane) for the conservative forms;
 // the source / NCP form keeps the extra state of its gradient and source terms
 // and runs at one.
-template <int D, int B, int V, int NP, typename R, bool CG, bool NCS = false>
+template <int D, int B, int V, int NP, typename R, bool CG, bool NCS = false, bool EUS = false>
 __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS), NCS ? 1 : 2) void predictor_kernel(
     long ncells, long ntiles, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
     double* __restrict__ lQhbnd, double* __restrict__ lFhbnd, int* __restrict__ picard,
     const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect,
     int hdim) {
-  predictor_tile<D, B, V, NP, R, CG, NCS>(blockIdx.x, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard,
-                                          T, P, maxIter, adaptive, luh_upd, dialect, hdim);
+  predictor_tile<D, B, V, NP, R, CG, NCS, EUS>(blockIdx.x, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd,
+                                               picard, T, P, maxIter, adaptive, luh_upd, dialect, hdim);
 }
 
 // Persistent form: a grid smaller than ntiles loops over the tiles.
@@ -1294,6 +1320,20 @@ struct Launch {
                                     const double* luh, const double* h, const double* dt, double* lduh, double* qb,
                                     double* fb, int* pic, const KernelCfg& k, int maxIter, int adaptive,
                                     const LaunchOpts& o, int dl) {
+    // 3D Euler of order 5 in FP64: the slab of the 12 distinct flux components
+    if constexpr (D == 3 && B == 6 && V == 5 && NP == 0 && sizeof(R) == 8) {
+      if (k.pde.id == kPdeEuler && !persistent) {
+        if (cerk)
+          hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, R, true, false, true>), dim3(tiles), dim3(S::THREADS), 0,
+                             st, n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive,
+                             o.luh_update, dl, o.hdim);
+        else
+          hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, R, false, false, true>), dim3(tiles), dim3(S::THREADS), 0,
+                             st, n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive,
+                             o.luh_update, dl, o.hdim);
+        return;
+      }
+    }
     if (cerk)
       hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, R, true>), dim3(tiles), dim3(S::THREADS), 0, st, n, tiles,
                          luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, o.luh_update, dl, o.hdim);
