@@ -51,7 +51,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 FETCH_SIZE (x2, the factor a 1 GiB copy calibrates) + WRITE_SIZE (x1):
 # profiles/euler3d_n5_81cubed_kernels.md.
 MEASURED_PREDICTOR_TRAFFIC = {(81, 5, "fp32", 1): 14.69e9, (81, 5, "fp32", 0): 18.69e9,
-                              (81, 5, "fp64", 1): 18.46e9, (81, 5, "fp64", 0): 21.31e9}
+                              (81, 5, "fp64", 1): 18.45e9, (81, 5, "fp64", 0): 18.75e9}
 
 
 def log(*a):
