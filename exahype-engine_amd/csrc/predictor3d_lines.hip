@@ -187,6 +187,21 @@ __device__ __forceinline__ void euler_flux_dir<float>(const float* Q, float irho
   F[4] = (Q[4] + p) * ud;
 }
 
+// Point index of the FP32 forms' per-point LDS arrays (the CERK stage fluxes,
+// Fc_d, lQhi, lFhi_d): x slowest, then z, y fastest.  A time-line lane (t, z, y)
+// owns point (z, y, x = t) at index (t B + z) B + y = its lane index, and the
+// lines a wave reads in x, y and z start on distinct LDS banks (with the
+// natural x-fastest order the x-lines' starts, 6 apart, met on the same banks).
+// Strides along x, y, z: B^2, 1, B.
+template <int B>
+__host__ __device__ constexpr int lds_point(int z, int y, int x) {
+  return (x * B + z) * B + y;
+}
+template <int B>
+__host__ __device__ constexpr int lds_stride(int d) {
+  return d == 0 ? B * B : (d == 1 ? 1 : B);
+}
+
 // At one point, from its time line q[t][v] (t ascending, FP32):
 // lFhi_d = sum_t w_t F_d(q[t]), stored at af[(d NS) V + pv + v] (pv = point
 // index * V), and fc_d = sum_t c_t F_d(q[t]) with c = T->cwf, returned.
@@ -409,11 +424,12 @@ __device__ __forceinline__ void finish_cell_points(const float* P, const float* 
   float* aq = avg;             // [NS][V]
   float* af = avg + NS * V;    // [3][NS][V]
   if (l < NS) {
+    const int pl = lds_point<B>(l / (B * B), (l / B) % B, l % B);  // node l in the LDS point order
 #pragma unroll
     for (int v = 0; v < V; v++) {
-      aq[l * V + v] = qh[v];
+      aq[pl * V + v] = qh[v];
 #pragma unroll
-      for (int d = 0; d < 3; d++) af[(d * NS + l) * V + v] = fh[d][v];
+      for (int d = 0; d < 3; d++) af[(d * NS + pl) * V + v] = fh[d][v];
     }
   }
   __syncthreads();
@@ -433,8 +449,9 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
                                                      double* luh_upd, int nthreads, int f32faces) {
   constexpr int V = 5, NS = B * B * B, NF = B * B;
   if (l < NS) {
-    // volume integral at node s = l = (z, y, x)
-    const int x = l % B, y = (l / B) % B, z = l / (B * B);
+    // volume integral at the node of LDS point l = (x B + z) B + y
+    const int y = l % B, z = (l / B) % B, x = l / (B * B);
+    const int nl = (z * B + y) * B + x;  // its index in the cell's global node order
     const float invDx = __builtin_amdgcn_rcpf(static_cast<float>(h));  // 1 ulp: FP32 forms only
     float du[V];
 #pragma unroll
@@ -443,7 +460,7 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
     for (int d = 0; d < 3; d++) {
       const float tw = (d == 0) ? T->wf[z] * T->wf[y] : (d == 1 ? T->wf[z] * T->wf[x] : T->wf[y] * T->wf[x]);
       const int c = (d == 0) ? x : (d == 1 ? y : z);
-      const int stride = (d == 0) ? 1 : (d == 1 ? B : B * B);
+      const int stride = lds_stride<B>(d);
       const float* line = af + (d * NS + l - c * stride) * V;
       float part[V];
 #pragma unroll
@@ -460,10 +477,10 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
     if (luh_upd) {
       const double upd = dt * (T->iw[z] * T->iw[y] * T->iw[x]);
 #pragma unroll
-      for (int v = 0; v < V; v++) luh_upd[(cell * NS + l) * V + v] += static_cast<double>(du[v]) * upd;
+      for (int v = 0; v < V; v++) luh_upd[(cell * NS + nl) * V + v] += static_cast<double>(du[v]) * upd;
     } else {
 #pragma unroll
-      for (int v = 0; v < V; v++) lduh[(cell * NS + l) * V + v] = static_cast<double>(du[v]);
+      for (int v = 0; v < V; v++) lduh[(cell * NS + nl) * V + v] = static_cast<double>(du[v]);
     }
   }
   // extrapolation: one (direction, face node, variable) triple gives both faces.
@@ -476,9 +493,10 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
     const int fn = f32faces ? o % NF : (o / V) % NF;
     const int d = o / (V * NF);
     const int a = fn / B, b = fn % B;
-    // the line through face node (a, b) along d: node(m) = base + m * stride
-    const int base = (d == 0) ? (a * B + b) * B : (d == 1 ? a * B * B + b : a * B + b);
-    const int sv = ((d == 0) ? 1 : (d == 1 ? B : B * B)) * V;
+    // the line through face node (a, b) along d, (a, b) = (z, y), (z, x), (y, x):
+    // LDS point(m) = base + m * stride
+    const int base = (d == 0) ? lds_point<B>(a, b, 0) : (d == 1 ? lds_point<B>(a, 0, b) : lds_point<B>(0, a, b));
+    const int sv = lds_stride<B>(d) * V;
     const float* pq = aq + base * V + v;
     const float* pf = af + d * NS * V + base * V + v;
     float q0 = 0.f, q1 = 0.f, f0 = 0.f, f1 = 0.f;
@@ -611,7 +629,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
   const int ybase = (t * B + z) * B * LS + y * V;         // y-line (t, z, *, x = y), stride LS
   const int zbase = (t * B * B + y) * LS + z * V;         // z-line (t, *, y, x = z), stride B LS
   const int tbase = (z * B + y) * LS + t * V;             // time line of (z, y, x = t), stride B^2 LS
-  [[maybe_unused]] const int pv = ((z * B + y) * B + t) * V;  // that point in [node][v] arrays
+  [[maybe_unused]] const int pv = lds_point<B>(z, y, t) * V;  // that point in the [point][v] LDS arrays
 
   // FIX: the last Picard iteration is collapsed.  Its output lQi' = luh -
   // dt/h P G enters only lQhi = sum_t w_t lQi'(t), and by linearity
@@ -662,7 +680,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           const int c = d == 0 ? t : (d == 1 ? y : z);              // coordinate of p along d
-          const int stride = d == 0 ? 1 : (d == 1 ? B : B * B);       // point stride along d
+          const int stride = lds_stride<B>(d);                        // point stride along d
           const float* line = fl + d * V * NS + pt - c * stride;
 #pragma unroll
           for (int n = 0; n < B; n++) {
@@ -947,12 +965,12 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
     __syncthreads();
     if (lane_on) {
       // coordinates of p = (z, y, x = t), recomputed from pv (not kept live through the loop)
-      const int pp = pv / V, px = pp % B, py = (pp / B) % B, pz = pp / (B * B);
+      const int pp = pv / V, py = pp % B, pz = (pp / B) % B, px = pp / (B * B);
       float dv[V] = {0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         const int c = d == 0 ? px : (d == 1 ? py : pz);              // coordinate of p along d
-        const int stride = d == 0 ? V : (d == 1 ? B * V : B * B * V);  // [node][v] stride along d
+        const int stride = lds_stride<B>(d) * V;                      // [point][v] stride along d
         const float* line = fcb + d * NS * V + pv - c * stride;
 #pragma unroll
         for (int n = 0; n < B; n++) {
