@@ -715,7 +715,7 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS), NCS ? 1 : 2) void pr
 }
 
 // Persistent form: a grid smaller than ntiles loops over the tiles.
-template <int D, int B, int V, int NP, typename R, bool CG>
+template <int D, int B, int V, int NP, typename R, bool CG, bool EUS = false>
 __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS), 2) void predictor_persistent_kernel(
     long ncells, long ntiles, const double* luh, const double* __restrict__ hcell,
     const double* __restrict__ dtp, const int* __restrict__ cinst, double* __restrict__ lduh,
@@ -723,8 +723,8 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS), 2) void predictor_pe
     const DevTables* __restrict__ T, PdeParams P, int maxIter, int adaptive, double* luh_upd, int dialect,
     int hdim) {
   for (long blk = blockIdx.x; blk < ntiles; blk += gridDim.x) {
-    predictor_tile<D, B, V, NP, R, CG>(blk, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd, picard, T, P,
-                                   maxIter, adaptive, luh_upd, dialect, hdim);
+    predictor_tile<D, B, V, NP, R, CG, false, EUS>(blk, ncells, luh, hcell, dtp, cinst, lduh, lQhbnd, lFhbnd,
+                                                   picard, T, P, maxIter, adaptive, luh_upd, dialect, hdim);
     __syncthreads();  // LDS of this tile fully consumed before the next one
   }
 }
@@ -1320,10 +1320,14 @@ struct Launch {
                                     const double* luh, const double* h, const double* dt, double* lduh, double* qb,
                                     double* fb, int* pic, const KernelCfg& k, int maxIter, int adaptive,
                                     const LaunchOpts& o, int dl) {
-    // 3D Euler of order 5 in FP64: the slab of the 12 distinct flux components
-    if constexpr (D == 3 && B == 6 && V == 5 && NP == 0 && sizeof(R) == 8) {
-      if (k.pde.id == kPdeEuler && !persistent) {
-        if (cerk)
+    // 3D Euler of orders 4 and 5 in FP64: the slab of the 12 distinct flux components
+    if constexpr (D == 3 && (B == 5 || B == 6) && V == 5 && NP == 0 && sizeof(R) == 8) {
+      if (k.pde.id == kPdeEuler) {
+        if (persistent)
+          hipLaunchKernelGGL((predictor_persistent_kernel<D, B, V, NP, R, false, true>), dim3(grid),
+                             dim3(S::THREADS), 0, st, n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde,
+                             maxIter, adaptive, o.luh_update, dl, o.hdim);
+        else if (cerk)
           hipLaunchKernelGGL((predictor_kernel<D, B, V, NP, R, true, false, true>), dim3(tiles), dim3(S::THREADS), 0,
                              st, n, tiles, luh, h, dt, o.cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive,
                              o.luh_update, dl, o.hdim);
