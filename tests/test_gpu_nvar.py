@@ -7,8 +7,9 @@ and 3D at order 3 and 3 variables in 3D at order 2, and a user PDE for its
 declared kNVar.  Checked at the FP64 bar 1e-12 relative L2 with identical
 Picard counts: the per-cell fused predictor + volume integral, the per-face
 Rusanov solve, uniform periodic meshes in both step forms; the FP32 node-column
-form within 1e-6; and the host refuses Euler or the user PDE with another
-variable count.  The oracle's advection path is pinned to the exact space-time
+form within 1e-6; 3D order 5 with five advected variables (the node-column
+kernel's general flux slab, which Euler no longer uses), both initial guesses;
+and the host refuses Euler or the user PDE with another variable count.  The oracle's advection path is pinned to the exact space-time
 solution (tests/test_oracle_dialect.py, CERK exactness for quadratic data).
 """
 import numpy as np
@@ -108,6 +109,45 @@ def test_fp32_node_column_form():
     ref = luh.copy()
     oracle(D, order, V).run_uniform(ref, n, 1.0 / n, steps, 0.0)
     assert rel_l2(got, ref) <= 1e-6
+
+
+@pytest.mark.parametrize("guess", ["trivial", "cerk"])
+def test_order5_advection_five_variables_vs_oracle(guess):
+    """3D order 5 with five variables and a PDE other than Euler: the FP64
+    node-column kernel with its general flux slab (Euler runs the 12-slot
+    form), both initial guesses, per-cell and on a periodic mesh."""
+    D, order, V, n, steps = 3, 5, 5, 3, 2
+    print(f"[order5 advection] guess={guess}", flush=True)
+    dial = "cerk" if guess == "cerk" else None
+    c = ctx(D, order, V)
+    if guess == "cerk":
+        c.set_initial_guess(E.GUESS_CERK)
+    rng = np.random.default_rng(5)
+    B = order + 1
+    luh = 1.0 + 0.3 * rng.normal(size=B ** D * V)
+    h = [0.1, 0.12, 0.09]
+    o = O.Oracle(D, order, nvar=V, pde=O.PDE_ADVECTION, adv=A, dialect=dial)
+    dt = 0.3 * o.stable_dt(luh, h)
+    r = c.fusedSpaceTimePredictorVolumeIntegral(luh, h, 0.0, dt)
+    ref = o.fused(luh, h, dt)
+    for k in ("lduh", "lQhbnd", "lFhbnd"):
+        assert rel_l2(r[k], ref[k]) <= 1e-12, (k, rel_l2(r[k], ref[k]))
+    assert r["iters"] == ref["iters"]
+    u0 = smooth(D, n, order, V)
+    mesh = E.Mesh(c, n, 1.0 / n)
+    mesh.set_solution(u0)
+    mesh.init_dt(0.0)
+    mesh.run(steps)
+    mesh.sync()
+    got, pic, (_, dtn) = mesh.get_solution(), mesh.picard(), mesh.time()
+    mesh.close()
+    c.close()
+    refm = u0.copy()
+    dt_o, pic_o = O.Oracle(D, order, nvar=V, pde=O.PDE_ADVECTION, adv=A, dialect=dial).run_uniform(
+        refm, n, 1.0 / n, steps, 0.0, want_picard=True)
+    assert rel_l2(got, refm) <= 1e-12, rel_l2(got, refm)
+    np.testing.assert_array_equal(pic, pic_o)
+    assert abs(dtn / dt_o - 1) <= 1e-12
 
 
 def test_variable_counts_are_checked():
