@@ -483,46 +483,49 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
       for (int v = 0; v < V; v++) lduh[(cell * NS + nl) * V + v] = static_cast<double>(du[v]);
     }
   }
-  // extrapolation: one (direction, face node, variable) triple gives both faces.
+  // extrapolation: a lane takes one (variable, face node) pair and gives both
+  // faces of the three directions (d unrolled: compile-time line strides and
+  // face offsets; the pair is decoded once).
   // FP64 faces: the reference layout [face][node][var] (var fastest across
   // lanes); FP32 faces: [face][var][node] (node fastest), the mesh-internal
   // layout of the FP32 step whose corrector loads one variable row per step.
-  constexpr int NTRI = 3 * NF * V;
-  for (int o = l; o < NTRI; o += nthreads) {
-    const int v = f32faces ? (o / NF) % V : o % V;
-    const int fn = f32faces ? o % NF : (o / V) % NF;
-    const int d = o / (V * NF);
+  constexpr int NPAIR = NF * V;
+  for (int o = l; o < NPAIR; o += nthreads) {
+    const int v = f32faces ? o / NF : o % V;
+    const int fn = f32faces ? o % NF : o / V;
     const int a = fn / B, b = fn % B;
-    // the line through face node (a, b) along d, (a, b) = (z, y), (z, x), (y, x):
-    // LDS point(m) = base + m * stride
-    const int base = (d == 0) ? lds_point<B>(a, b, 0) : (d == 1 ? lds_point<B>(a, 0, b) : lds_point<B>(0, a, b));
-    const int sv = lds_stride<B>(d) * V;
-    const float* pq = aq + base * V + v;
-    const float* pf = af + d * NS * V + base * V + v;
-    float q0 = 0.f, q1 = 0.f, f0 = 0.f, f1 = 0.f;
+    const long fb = cell * 6 * NF * V + (f32faces ? v * NF + fn : fn * V + v);
 #pragma unroll
-    for (int m = 0; m < B; m++) {
-      const float qv = pq[m * sv], fv = pf[m * sv];
-      const float c0 = T->F0f[m], c1 = T->F1f[m];
-      q0 += qv * c0;
-      q1 += qv * c1;
-      f0 += fv * c0;
-      f1 += fv * c1;
-    }
-    if (f32faces) {  // the values are FP32 numbers: FP32 storage is exact
-      const long fb = cell * 6 * NF * V + v * NF + fn;
-      float* qf = reinterpret_cast<float*>(lQhbnd);
-      float* ff = reinterpret_cast<float*>(lFhbnd);
-      qf[fb + (2 * d) * NF * V] = q0;
-      qf[fb + (2 * d + 1) * NF * V] = q1;
-      ff[fb + (2 * d) * NF * V] = f0;
-      ff[fb + (2 * d + 1) * NF * V] = f1;
-    } else {
-      const long base = cell * 6 * NF * V + fn * V + v;
-      lQhbnd[base + (2 * d) * NF * V] = q0;
-      lQhbnd[base + (2 * d + 1) * NF * V] = q1;
-      lFhbnd[base + (2 * d) * NF * V] = f0;
-      lFhbnd[base + (2 * d + 1) * NF * V] = f1;
+    for (int d = 0; d < 3; d++) {
+      // the line through face node (a, b) along d, (a, b) = (z, y), (z, x), (y, x):
+      // LDS point(m) = base + m * stride
+      const int base = (d == 0) ? lds_point<B>(a, b, 0) : (d == 1 ? lds_point<B>(a, 0, b) : lds_point<B>(0, a, b));
+      const int sv = lds_stride<B>(d) * V;
+      const float* pq = aq + base * V + v;
+      const float* pf = af + d * NS * V + base * V + v;
+      float q0 = 0.f, q1 = 0.f, f0 = 0.f, f1 = 0.f;
+#pragma unroll
+      for (int m = 0; m < B; m++) {
+        const float qv = pq[m * sv], fv = pf[m * sv];
+        const float c0 = T->F0f[m], c1 = T->F1f[m];
+        q0 += qv * c0;
+        q1 += qv * c1;
+        f0 += fv * c0;
+        f1 += fv * c1;
+      }
+      if (f32faces) {  // the values are FP32 numbers: FP32 storage is exact
+        float* qf = reinterpret_cast<float*>(lQhbnd);
+        float* ff = reinterpret_cast<float*>(lFhbnd);
+        qf[fb + (2 * d) * NF * V] = q0;
+        qf[fb + (2 * d + 1) * NF * V] = q1;
+        ff[fb + (2 * d) * NF * V] = f0;
+        ff[fb + (2 * d + 1) * NF * V] = f1;
+      } else {
+        lQhbnd[fb + (2 * d) * NF * V] = q0;
+        lQhbnd[fb + (2 * d + 1) * NF * V] = q1;
+        lFhbnd[fb + (2 * d) * NF * V] = f0;
+        lFhbnd[fb + (2 * d + 1) * NF * V] = f1;
+      }
     }
   }
   if (l == 0 && picard) picard[cell] = iters;
