@@ -338,11 +338,16 @@ int exa_ctx_create(exa_ctx** out, int dim, int order, int nvar, int npar, int pd
     h.Pt[i] = c->tab.iK1[i] * c->tab.w[i % B];
     h.Ptf[i] = static_cast<float>(h.Pt[i]);
   }
+  double cm[3] = {0.0, 0.0, 0.0};
   for (int j = 0; j < B; j++) {
     double cj = 0.0;
     for (int i = 0; i < B; i++) cj += c->tab.w[i] * c->tab.iK1[i * B + j] * c->tab.w[j];
     h.cwf[j] = static_cast<float>(cj);
+    cm[0] += cj;
+    cm[1] += cj * c->tab.x[j];
+    cm[2] += cj * c->tab.x[j] * c->tab.x[j];
   }
+  for (int k = 0; k < 3; k++) h.cmom[k] = static_cast<float>(cm[k]);
   for (int i = 0; i < 3 * B * B; i++) h.fgp[i] = c->tab.fgp[i];
   hipError_t e = hipMalloc(&c->d_tab, sizeof(exa::DevTables));
   if (e == hipSuccess) e = hipMemcpy(c->d_tab, &h, sizeof(h), hipMemcpyHostToDevice);

@@ -31,6 +31,7 @@ struct DevTables {
   double Pt[kMaxBasis * kMaxBasis];  // Pt[i][j] = iK1[i][j] * w[j]
   float Ptf[kMaxBasis * kMaxBasis];
   float cwf[kMaxBasis];  // c[j] = sum_i w[i] Pt[i][j]: time average of one Picard update
+  float cmom[3];         // sum_j c[j] x[j]^k, k = 0, 1, 2 (c-weighted time moments)
   double fgp[3 * kMaxBasis * kMaxBasis];  // fineGridProjector[sub][n][m] = phi_n((sub + x_m) / 3), [(sub*B+n)*B+m]
   double iw[kMaxBasis];                   // 1 / w[i]
 };

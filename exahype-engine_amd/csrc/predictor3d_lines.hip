@@ -208,14 +208,18 @@ __host__ __device__ constexpr int lds_stride(int d) {
 // Of the 15 Euler flux components 12 are distinct: the mass fluxes are the
 // momenta, the momentum fluxes are symmetric (F_x[2] = F_y[1] = q_x q_y / rho,
 // F_x[3] = F_z[1], F_y[3] = F_z[2]); each distinct one is formed and summed once.
-template <int B>
+// MASS: the w- and c-weighted time sums of the momenta (the mass fluxes) are
+// given (mw, mc; closed forms for a polynomial time line) instead of summed.
+template <int B, bool MASS = false>
 __device__ __forceinline__ void time_line_averages(const float (&q)[B][LShape<B>::V], const DevTables* __restrict__ T,
-                                                   float* af, int pv, float (&fc)[3][LShape<B>::V]) {
+                                                   float* af, int pv, float (&fc)[3][LShape<B>::V],
+                                                   const float* mw = nullptr, const float* mc = nullptr) {
   constexpr int V = 5, NS = B * B * B, NDIST = 12;
   // distinct components: q_x, q_y, q_z, xx, yy, zz, xy, xz, yz, ex, ey, ez
   float sw[NDIST], sc[NDIST];
 #pragma unroll
   for (int i = 0; i < NDIST; i++) sw[i] = sc[i] = 0.f;
+  constexpr int I0 = MASS ? 3 : 0;  // first summed component
 #pragma unroll
   for (int t = 0; t < B; t++) {
     const float wt = T->wf[t], ct = T->cwf[t];
@@ -226,9 +230,16 @@ __device__ __forceinline__ void time_line_averages(const float (&q)[B][LShape<B>
     const float f[NDIST] = {Q[1],          Q[2],          Q[3],      Q[1] * ux + pr, Q[2] * uy + pr, Q[3] * uz + pr,
                             Q[2] * ux,     Q[3] * ux,     Q[3] * uy, e * ux,         e * uy,         e * uz};
 #pragma unroll
-    for (int i = 0; i < NDIST; i++) {
+    for (int i = I0; i < NDIST; i++) {
       sw[i] += f[i] * wt;
       sc[i] += f[i] * ct;
+    }
+  }
+  if constexpr (MASS) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      sw[i] = mw[i];
+      sc[i] = mc[i];
     }
   }
   // F_d[v] -> distinct component
@@ -726,8 +737,19 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
 #pragma unroll
           for (int v = 0; v < V; v++) tl[tbase + i * (B * B * LS) + v] = qg[i][v];
       } else {  // the collapsed iteration evaluates fluxes of the guess (stage reads are done)
+        // the guess is quadratic in time, q = luh - cdt x K1 - cdt x^2 (K2 - K1) / 2, so the
+        // time sums of the momenta (the mass fluxes) are closed forms: w-weighted
+        // luh - cdt g0, c-weighted luh m0 - cdt K1 m1 - cdt (K2 - K1) m2 / 2 (m_k = sum_t c_t x_t^k)
+        float mw[3], mc[3];
+        const float m0 = T->cmom[0], m1 = T->cmom[1], m2 = T->cmom[2];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          const int v = i + 1;
+          mw[i] = u[v] - cdt * (K1[v] * (1.f / 3.f) + K2[v] * (1.f / 6.f));
+          mc[i] = u[v] * m0 - cdt * K1[v] * m1 - (0.5f * cdt) * (K2[v] - K1[v]) * m2;
+        }
         float fc[3][V];
-        time_line_averages<B>(qg, T, af, pv, fc);
+        time_line_averages<B, true>(qg, T, af, pv, fc, mw, mc);
         store_fc(fc);
       }
     }
