@@ -825,12 +825,13 @@ __device__ __forceinline__ void atomic_min_pos_double(double* addr, double v) {
   atomicMin(reinterpret_cast<unsigned long long*>(addr), static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
-// CFL*PNPM / sum_d max_v |lambda_v(Q, d)| / dx (stableTimeStepSize.cpph:14-62).
-// For Euler the direction-independent 1/rho, p and c are computed once; the
-// per-direction maximum keeps pde_max_abs_eig's expressions (same bits).
+// A node's stable dt is CFL*PNPM / den with den = sum_d max_v |lambda_v(Q, d)| / dx
+// (stableTimeStepSize.cpph:14-62); node_cfl_den returns den.  For Euler the
+// direction-independent 1/rho, p and c are computed once; the per-direction
+// maximum keeps pde_max_abs_eig's expressions (same bits).
 template <int D, int B, int V, int NP>
-__device__ __forceinline__ double node_cfl_dt(const double* __restrict__ Q, const PdeParams& P,
-                                              const double (&invDx)[D], double cflpnpm) {
+__device__ __forceinline__ double node_cfl_den(const double* __restrict__ Q, const PdeParams& P,
+                                               const double (&invDx)[D]) {
   double den = 0.0;
   if constexpr (V == 5) {
     if (P.id == kPdeEuler) {
@@ -845,12 +846,32 @@ __device__ __forceinline__ double node_cfl_dt(const double* __restrict__ Q, cons
         const double l0 = fabs(u_n - c), l1 = fabs(u_n), l4 = fabs(u_n + c);
         den += fmax(fmax(l0, l1), l4) * invDx[d];
       }
-      return cflpnpm / den;
+      return den;
     }
   }
 #pragma unroll
   for (int d = 0; d < D; d++) den += pde_max_abs_eig<D, V>(P, Q, d) * invDx[d];
-  return cflpnpm / den;
+  return den;
+}
+
+template <int D, int B, int V, int NP>
+__device__ __forceinline__ double node_cfl_dt(const double* __restrict__ Q, const PdeParams& P,
+                                              const double (&invDx)[D], double cflpnpm) {
+  return cflpnpm / node_cfl_den<D, B, V, NP>(Q, P, invDx);
+}
+
+// Workgroup maximum (fmax: NaN lanes are ignored, as fmin ignores them in block_min).
+template <int THREADS>
+__device__ __forceinline__ double block_max(double v, double* sred) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sred[wid] = v;
+  __syncthreads();
+  double r = sred[0];
+#pragma unroll
+  for (int i = 1; i < THREADS / 64; i++) r = fmax(r, sred[i]);
+  return r;
 }
 
 template <int THREADS>
@@ -985,31 +1006,39 @@ struct FusedShape {
   static constexpr bool ok = LDS_BYTES <= 60 * 1024;
 };
 
+// Integer coordinates of a cell of a uniform n^D mesh (x fastest).
+struct CellXYZ {
+  long ix, iy, iz;
+};
 template <int D>
-__device__ __forceinline__ long periodic_neighbour(long cell, int n, int d, int dir) {
-  long ix = cell % n, iy = (cell / n) % n, iz = (D == 3) ? cell / (long(n) * n) : 0;
-  if (d == 0) ix = (ix + dir + n) % n;
-  if (d == 1) iy = (iy + dir + n) % n;
-  if (d == 2) iz = (iz + dir + n) % n;
+__device__ __forceinline__ CellXYZ cell_xyz(long cell, int n) {
+  return {cell % n, (cell / n) % n, (D == 3) ? cell / (long(n) * n) : 0};
+}
+
+// Periodic neighbour of the cell at c in direction d (dir = +-1): the
+// coordinate steps with a wrap at the domain ends, no division.
+template <int D>
+__device__ __forceinline__ long periodic_neighbour(const CellXYZ& c, int n, int d, int dir) {
+  auto step = [&](long i) { return (i + dir == n) ? 0 : (i + dir < 0 ? n - 1 : i + dir); };
+  const long ix = d == 0 ? step(c.ix) : c.ix, iy = d == 1 ? step(c.iy) : c.iy, iz = d == 2 ? step(c.iz) : c.iz;
   return (D == 3) ? (iz * n + iy) * n + ix : iy * n + ix;
 }
 
-// Ghost face of face f = 2d + o of a cell when that face lies on a
+// Ghost face of face f = 2d + o of the cell at c when that face lies on a
 // non-periodic domain face (bit f of bcmask), else -1.  Index f * n^(D-1) +
 // transverse cell index: (iz, iy) for d = 0, (iz, ix) for d = 1, (iy, ix) for
 // d = 2; in 2D iy for d = 0, ix for d = 1.
 template <int D>
-__device__ __forceinline__ long ghost_face(long cell, int n, int f, unsigned bcmask) {
+__device__ __forceinline__ long ghost_face(const CellXYZ& c, int n, int f, unsigned bcmask) {
   if (!((bcmask >> f) & 1u)) return -1;
-  const long ix = cell % n, iy = (cell / n) % n, iz = (D == 3) ? cell / (long(n) * n) : 0;
   const int d = f >> 1;
-  const long id = (d == 0) ? ix : (d == 1 ? iy : iz);
+  const long id = (d == 0) ? c.ix : (d == 1 ? c.iy : c.iz);
   if (id != ((f & 1) ? n - 1 : 0)) return -1;
   long tr;
   if (D == 3)
-    tr = (d == 0) ? iz * n + iy : (d == 1 ? iz * n + ix : iy * n + ix);
+    tr = (d == 0) ? c.iz * n + c.iy : (d == 1 ? c.iz * n + c.ix : c.iy * n + c.ix);
   else
-    tr = (d == 0) ? iy : ix;
+    tr = (d == 0) ? c.iy : c.ix;
   return long(f) * ((D == 3) ? long(n) * n : long(n)) + tr;
 }
 
@@ -1029,17 +1058,26 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
     long ncells, int n, double* __restrict__ luh, const TF* __restrict__ lQhbnd,
     const TF* __restrict__ lFhbnd, const double* __restrict__ hcell, const double* __restrict__ dtp,
     double* __restrict__ dt_next, const DevTables* __restrict__ T, PdeParams P, double cflpnpm,
-    const TF* __restrict__ gq, const TF* __restrict__ gf, unsigned bcmask, double* __restrict__ slots) {
+    const TF* __restrict__ gq, const TF* __restrict__ gf, unsigned bcmask, double* __restrict__ slots,
+    long cell0) {
   using S = Shape<D, B, V, NP>;
   using FS = FusedShape<D, B, V, NP>;
   constexpr int ND = S::ND, NS = S::NS, NF = S::NF, CPB = S::CPB, THREADS = S::THREADS;
   constexpr int NTASK = FS::NTASK;
+  // GRID3 (3D, one cell per workgroup): the grid is (n, n, planes) and block
+  // (x, y, z) is the cell (x, y, cell0 + z) -- cell0 is then the first z plane;
+  // otherwise the launch covers cells [cell0, ncells), CPB per workgroup
+  constexpr bool GRID3 = D == 3 && CPB == 1;
   __shared__ double sred[THREADS / 64];
   __shared__ double sbuf[CPB * (FS::RED + FS::AVG + FS::PART + FS::SMAX)];
   const int tid = threadIdx.x;
   const int cib = tid / NS;
   const int s = tid - cib * NS;
-  const long cell = long(blockIdx.x) * CPB + cib;
+  // the cell's coordinates: GRID3 from the block index (no division)
+  const long cg = cell0 + long(blockIdx.x) * CPB + cib;
+  const CellXYZ cxyz = GRID3 ? CellXYZ{long(blockIdx.x), long(blockIdx.y), cell0 + long(blockIdx.z)}
+                             : cell_xyz<D>(cg, n);
+  const long cell = GRID3 ? (cxyz.iz * n + cxyz.iy) * n + cxyz.ix : cg;
   const bool valid = (cib < CPB) && (cell < ncells);
   double* red = sbuf + (valid ? cib : 0) * (FS::RED + FS::AVG + FS::PART + FS::SMAX);  // [task][side][ND]; later [task][V]
   double* avg = red + FS::RED;                                                         // [face][side][ND]
@@ -1050,7 +1088,7 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
   // loads both sides' states and fluxes once and keeps them for phase 3.
   constexpr bool ONE = NTASK <= NS;
   // kept in the face storage type (FP32 faces: half the registers), widened at use
-  TF rQL[ONE ? ND : 1], rQR[ONE ? ND : 1], rFL[ONE ? V : 1], rFR[ONE ? V : 1];
+  TF rQo[ONE ? ND : 1], rQn[ONE ? ND : 1], rFo[ONE ? V : 1], rFn[ONE ? V : 1];
   // the node's solution, loaded up front so that its latency overlaps the face phases
   double Q[ND];
   if (valid) {
@@ -1062,39 +1100,41 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
     if (valid && s < NTASK) {
       const int task = s;
       const int f = task / NF, fn = task - f * NF, o = f & 1;
-      const long g = ghost_face<D>(cell, n, f, bcmask);
-      const long nb = g >= 0 ? 0 : periodic_neighbour<D>(cell, n, f >> 1, o ? 1 : -1);
+      const long g = ghost_face<D>(cxyz, n, f, bcmask);
+      const long nb = g >= 0 ? 0 : periodic_neighbour<D>(cxyz, n, f >> 1, o ? 1 : -1);
       // face blocks (NF nodes each) of this cell and of the neighbour or ghost
       const TF* qo = lQhbnd + (cell * 2 * D + f) * NF * ND;
       const TF* qn = g >= 0 ? gq + g * NF * ND : lQhbnd + (nb * 2 * D + (f ^ 1)) * NF * ND;
       const TF* fo = lFhbnd + (cell * 2 * D + f) * NF * V;
       const TF* fnb = g >= 0 ? gf + g * NF * V : lFhbnd + (nb * 2 * D + (f ^ 1)) * NF * V;
+      // this cell's side and the other side (the left state is the other
+      // side's for o = 0, this cell's for o = 1)
 #pragma unroll
       for (int k = 0; k < ND; k++) {
-        const TF a = static_cast<TF>(face_val(qo, fn, k, NF, ND)), b = static_cast<TF>(face_val(qn, fn, k, NF, ND));
-        rQL[k] = o ? a : b;
-        rQR[k] = o ? b : a;
+        rQo[k] = static_cast<TF>(face_val(qo, fn, k, NF, ND));
+        rQn[k] = static_cast<TF>(face_val(qn, fn, k, NF, ND));
       }
 #pragma unroll
       for (int k = 0; k < V; k++) {
-        const TF a = static_cast<TF>(face_val(fo, fn, k, NF, V)), b = static_cast<TF>(face_val(fnb, fn, k, NF, V));
-        rFL[k] = o ? a : b;
-        rFR[k] = o ? b : a;
+        rFo[k] = static_cast<TF>(face_val(fo, fn, k, NF, V));
+        rFn[k] = static_cast<TF>(face_val(fnb, fn, k, NF, V));
       }
+      // the face averages' partial products per side; the Rusanov speed is the
+      // larger of the two sides' (fmax: the side order does not matter)
       const double wj = (D == 3) ? w[fn / B] * w[fn % B] : w[fn];
 #pragma unroll
       for (int k = 0; k < ND; k++) {
-        red[(task * 2 + 0) * ND + k] = wj * static_cast<double>(rQL[k]);
-        red[(task * 2 + 1) * ND + k] = wj * static_cast<double>(rQR[k]);
+        red[(task * 2 + 0) * ND + k] = wj * static_cast<double>(rQo[k]);
+        red[(task * 2 + 1) * ND + k] = wj * static_cast<double>(rQn[k]);
       }
     }
   } else if (valid) {
     for (int task = s; task < NTASK; task += NS) {
       const int f = task / NF, fn = task - f * NF, o = f & 1;
-      const long g = ghost_face<D>(cell, n, f, bcmask);
+      const long g = ghost_face<D>(cxyz, n, f, bcmask);
       const TF* qo = lQhbnd + (cell * 2 * D + f) * NF * ND;
       const TF* qn = g >= 0 ? gq + g * NF * ND
-                            : lQhbnd + (periodic_neighbour<D>(cell, n, f >> 1, o ? 1 : -1) * 2 * D + (f ^ 1)) * NF * ND;
+                            : lQhbnd + (periodic_neighbour<D>(cxyz, n, f >> 1, o ? 1 : -1) * 2 * D + (f ^ 1)) * NF * ND;
       const TF* QL = o ? qo : qn;
       const TF* QR = o ? qn : qo;
       const double wj = (D == 3) ? w[fn / B] * w[fn % B] : w[fn];
@@ -1152,18 +1192,23 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
   if constexpr (ONE) {
     if (valid && s < NTASK) {
       const int task = s;
-      const double smax = sm[task / NF];
+      const int f = task / NF;
+      // 0.5 (FR + FL) - 0.5 smax (QR - QL) with (L, R) = (other, own) for o = 0 and
+      // (own, other) for o = 1: FR + FL is the sum of both sides either way and
+      // QR - QL = +-(Qother - Qown) exactly, so the sign moves to the factor and
+      // both cells of a face form the same bits
+      const double hs = (f & 1) ? 0.5 * sm[f] : -(0.5 * sm[f]);
 #pragma unroll
       for (int k = 0; k < V; k++) {
-        const double fr = rFR[k], fl = rFL[k], qr = rQR[k], ql = rQL[k];
-        red[task * V + k] = 0.5 * (fr + fl) - 0.5 * smax * (qr - ql);
+        const double fw = rFo[k], fx = rFn[k], qw = rQo[k], qx = rQn[k];
+        red[task * V + k] = 0.5 * (fw + fx) - hs * (qx - qw);
       }
     }
   } else if (valid) {
     for (int task = s; task < NTASK; task += NS) {
       const int f = task / NF, fn = task - f * NF, d = f >> 1, o = f & 1;
-      const long g = ghost_face<D>(cell, n, f, bcmask);
-      const long nb = g >= 0 ? 0 : periodic_neighbour<D>(cell, n, d, o ? 1 : -1);
+      const long g = ghost_face<D>(cxyz, n, f, bcmask);
+      const long nb = g >= 0 ? 0 : periodic_neighbour<D>(cxyz, n, d, o ? 1 : -1);
       const TF* qo = lQhbnd + (cell * 2 * D + f) * NF * ND;
       const TF* qn = g >= 0 ? gq + g * NF * ND : lQhbnd + (nb * 2 * D + (f ^ 1)) * NF * ND;
       const TF* fo = lFhbnd + (cell * 2 * D + f) * NF * V;
@@ -1183,7 +1228,7 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
   }
   __syncthreads();
   // phase 4
-  double mydt = DBL_MAX;
+  double myden = 0.0;  // lanes past the cell: no constraint
   if (valid) {
     int z, y, x;
     node_zyx<D, B>(s, z, y, x);
@@ -1217,11 +1262,17 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
     double invDx[D];
 #pragma unroll
     for (int d = 0; d < D; d++) invDx[d] = 1.0 / h;
-    mydt = node_cfl_dt<D, B, V, NP>(Q, P, invDx, cflpnpm);
+    myden = node_cfl_den<D, B, V, NP>(Q, P, invDx);
   }
-  const double bmin = block_min<THREADS>(mydt, sred);
-  double* dst = slots ? slots + (blockIdx.x % kDtSlots) * kDtSlotStride : dt_next;
-  if (tid == 0 && bmin < DBL_MAX) atomic_min_pos_double(dst, bmin);
+  // the cell's stable dt: min over nodes of cflpnpm / den = cflpnpm / max den
+  // (correctly rounded division is monotone in den > 0: same bits, one division)
+  const double bmax = block_max<THREADS>(myden, sred);
+  const long lin = long(blockIdx.x) + long(gridDim.x) * (long(blockIdx.y) + long(gridDim.y) * blockIdx.z);
+  double* dst = slots ? slots + (lin % kDtSlots) * kDtSlotStride : dt_next;
+  if (tid == 0 && bmax > 0.0) {
+    const double bmin = cflpnpm / bmax;
+    if (bmin < DBL_MAX) atomic_min_pos_double(dst, bmin);
+  }
 }
 
 // Single-cell helpers used by the per-cell compatibility entry points.
@@ -1376,21 +1427,30 @@ struct Launch {
     if constexpr (FusedShape<D, B, V, NP>::ok) {
       if (pde_has_ncp(k.pde)) return -1;  // the fused corrector solves conservative Rusanov problems only
       if (n <= 0) return 0;
-      const long blocks = (n + S::CPB - 1) / S::CPB;
+      // cells [c0, c1) of the n cells; one cell per workgroup in 3D: whole z planes,
+      // grid (nside, nside, planes)
+      const long c0 = o.cell0, c1 = o.cell_end < 0 ? n : o.cell_end;
+      if (c0 < 0 || c0 >= c1 || c1 > n) return -1;
+      constexpr bool GRID3 = D == 3 && S::CPB == 1;
+      const long plane = long(nside) * nside;
+      if (GRID3 && (c0 % plane || c1 % plane)) return -1;
+      const dim3 grid = GRID3 ? dim3(nside, nside, unsigned((c1 - c0) / plane))
+                              : dim3(unsigned((c1 - c0 + S::CPB - 1) / S::CPB));
+      const long first = GRID3 ? c0 / plane : c0;
       if (o.faces_f32) {
         if constexpr (D == 3 && B == 6 && V == 5 && NP == 0) {  // the FP32 line predictor's shape
-          hipLaunchKernelGGL((corrector_riemann_kernel<D, B, V, NP, float>), dim3(blocks), dim3(S::THREADS), 0, st, n,
-                             nside, luh, reinterpret_cast<const float*>(qb), reinterpret_cast<const float*>(fb), h, dt,
+          hipLaunchKernelGGL((corrector_riemann_kernel<D, B, V, NP, float>), grid, dim3(S::THREADS), 0, st, c1, nside,
+                             luh, reinterpret_cast<const float*>(qb), reinterpret_cast<const float*>(fb), h, dt,
                              dt_next, k.tab, k.pde, k.cfl * k.pnpm, reinterpret_cast<const float*>(o.ghost_q),
-                             reinterpret_cast<const float*>(o.ghost_f), o.bcmask, o.dt_slots);
+                             reinterpret_cast<const float*>(o.ghost_f), o.bcmask, o.dt_slots, first);
           return static_cast<int>(hipGetLastError());
         } else {
           return -1;
         }
       }
-      hipLaunchKernelGGL((corrector_riemann_kernel<D, B, V, NP, double>), dim3(blocks), dim3(S::THREADS), 0, st, n,
-                         nside, luh, qb, fb, h, dt, dt_next, k.tab, k.pde, k.cfl * k.pnpm, o.ghost_q, o.ghost_f,
-                         o.bcmask, o.dt_slots);
+      hipLaunchKernelGGL((corrector_riemann_kernel<D, B, V, NP, double>), grid, dim3(S::THREADS), 0, st, c1, nside,
+                         luh, qb, fb, h, dt, dt_next, k.tab, k.pde, k.cfl * k.pnpm, o.ghost_q, o.ghost_f, o.bcmask,
+                         o.dt_slots, first);
       return static_cast<int>(hipGetLastError());
     } else {
       return -1;
@@ -1465,6 +1525,8 @@ int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* lu
   const Entry* e = find_entry(k);
   if (!e) return -1;
   if (n <= 0) return 0;
+  // a part of the cells: the one-iteration FP32 CERK line kernel only
+  if ((o.cell0 != 0 || o.cell_end >= 0) && (!predictor_supports_range(k) || o.hdim)) return -1;
   // BF16 operands: the tile kernel (3D Euler order 5, trivial guess) only
   if (k.precision == 2) {
     if (!line_kernels_serve(k) || k.guess || o.hdim) return -1;
@@ -1477,7 +1539,7 @@ int launch_predictor(const KernelCfg& k, hipStream_t s, long n, const double* lu
     const bool fp32 = k.precision != 0;
     if ((v == 0 && fp32) || v == 3)
       return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, fp32, o.cinst, o.luh_update,
-                                      o.faces_f32, o.guard);
+                                      o.faces_f32, o.guard, o.cell0, o.cell_end);
     if (v == 2)
       return launch_predictor3d_lines(k, s, n, luh, h, dt, lduh, qb, fb, pic, false, o.cinst, o.luh_update,
                                       o.faces_f32);
