@@ -97,6 +97,11 @@ struct LaunchOpts {
   // node-column predictor and corrector: 0 = one edge length per cell (h[cell]),
   // D = one per direction (h[cell][d]); the other kernels take cubic cells only
   int hdim = 0;
+  // the one-iteration FP32 CERK line predictor and the fused corrector: the
+  // launch covers cells [cell0, cell_end) of the n cells (cell_end < 0: n);
+  // the other kernels cover all cells and return -1 for a part
+  long cell0 = 0;
+  long cell_end = -1;
 };
 
 // Slots of the spread dt reduction: one device-scope atomic per workgroup on a
@@ -146,7 +151,7 @@ bool line_kernels_serve(const KernelCfg& k);
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
                              const double* dt, double* lduh, double* lQhbnd, double* lFhbnd, int* picard,
                              bool mfma, const int* cinst = nullptr, double* luh_update = nullptr,
-                             bool f32faces = false, int* guard = nullptr);
+                             bool f32faces = false, int* guard = nullptr, long cell0 = 0, long cell_end = -1);
 // FP32 tile form of the same predictor (3D Euler, order 5): space-time iterate
 // in LDS, contractions on 16x16x4 matrix-core steps; -1 for FP64.
 int launch_predictor3d_tile(const KernelCfg& k, hipStream_t s, long ncells, const double* luh, const double* h,
@@ -157,6 +162,9 @@ bool predictor_f32_faces_supported(const KernelCfg& k);
 // The predictor selected by k runs the Picard guard when given LaunchOpts::guard
 // (the one-iteration FP32 CERK line kernel with guard_tol > 0).
 bool predictor_uses_guard(const KernelCfg& k);
+// The predictor selected by k covers a part of the cells (LaunchOpts::cell0,
+// cell_end): the one-iteration FP32 CERK line kernel.
+bool predictor_supports_range(const KernelCfg& k);
 // Time bookkeeping of the device step per instance, state = t[ninst], dt[ninst],
 // dt_next[ninst]: t += dt; dt = dt_next; dt_next = DBL_MAX.
 int launch_advance_time(hipStream_t s, double* state, int ninst = 1);

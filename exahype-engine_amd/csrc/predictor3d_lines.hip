@@ -561,7 +561,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
     long ncells, const double* luh, const double* __restrict__ hcell, const double* __restrict__ dtp,
     const int* __restrict__ cinst, double* __restrict__ lduh, double* __restrict__ lQhbnd,
     double* __restrict__ lFhbnd, int* __restrict__ picard, const DevTables* __restrict__ T, PdeParams P,
-    int maxIter, int adaptive, double* luh_upd, int f32faces, GuardArgs G) {
+    int maxIter, int adaptive, double* luh_upd, int f32faces, GuardArgs G, long cell0 = 0) {
   using S = LShape<B>;
   constexpr int V = S::V, NS = S::NS;
   // x-line stride in the tile; the fixed-count form pads each line by one
@@ -1069,7 +1069,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
       __syncthreads();  // the next cell rewrites the LDS buffers
     }
   } else {
-    cell_work(blockIdx.x);
+    cell_work(cell0 + blockIdx.x);  // the launch covers cells [cell0, cell0 + gridDim.x)
   }
 }
 
@@ -1388,21 +1388,28 @@ __global__ void guard_reset_kernel(int* guard) {
 
 int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const double* luh, const double* h,
                              const double* dt, double* lduh, double* qb, double* fb, int* pic, bool mfma,
-                             const int* cinst, double* luh_upd, bool f32faces, int* guard) {
+                             const int* cinst, double* luh_upd, bool f32faces, int* guard, long cell0,
+                             long cell_end) {
   constexpr int B = 6;
   using S = LShape<B>;
-  const long blocks = (n + S::CPB - 1) / S::CPB;
+  static_assert(S::CPB == 1, "one cell per workgroup");
+  // cells [cell0, cell_end): the one-iteration CERK form only; the others cover all n cells
+  const bool range = cell0 != 0 || (cell_end >= 0 && cell_end != n);
+  if (cell_end < 0) cell_end = n;
+  if (cell0 < 0 || cell0 >= cell_end || cell_end > n) return -1;
+  const long blocks = range ? cell_end - cell0 : (n + S::CPB - 1) / S::CPB;
   const int adaptive = k.picard_max <= 0;
   const int maxIter = adaptive ? 2 * (k.order + 1) : k.picard_max;
   const int f32 = (f32faces && k.precision != 0) ? 1 : 0;  // FP32 faces exist only in the FP32 forms
   const dim3 grid(blocks), block(S::THREADS);
   const GuardArgs none;
   if (k.precision == 0) {
-    if (k.guess) return -1;  // the FP64 line kernel has no CERK guess (the node-column kernel has)
+    if (range || k.guess) return -1;  // the FP64 line kernel has no CERK guess (the node-column kernel has)
     hipLaunchKernelGGL((predictor3d_lines_kernel<B, double, false, false, false>), grid, block, 0, st, n, luh, h, dt,
                        cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, 0, none);
   } else {
     if (k.guess && !(mfma && !adaptive)) return -1;  // CERK: fixed-count matrix-core form only
+    if (range && !(mfma && !adaptive && maxIter == 1 && k.guess)) return -1;
     if (mfma && !adaptive && maxIter == 1 && k.guess) {  // one collapsed iteration: no sweep code, smaller LDS
       // Picard guard: cells whose proxy residual reaches the tolerance are
       // recomputed with the fixed count k.guard_recount (listed cells only)
@@ -1412,10 +1419,12 @@ int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const d
         g.tol2 = k.guard_tol * k.guard_tol;
       }
       hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true, true>), grid, block, 0, st, n, luh, h,
-                         dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32, g);
+                         dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32, g, cell0);
       if (g.guard) {
-        // persistent grid over the list: four workgroups per CU of the fixed-count kernel
-        const long lgrid = n < 1024 ? n : 1024;
+        // persistent grid over the list: four workgroups per CU of the fixed-count
+        // kernel (a part of the mesh: one per CU, the list is usually empty)
+        const long lmax = range ? 256 : 1024;
+        const long lgrid = blocks < lmax ? blocks : lmax;
         const int rc = k.guard_recount > 0 ? k.guard_recount : (B - 3 > 1 ? B - 3 : 1);
         hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true, false, true>), dim3(lgrid), block, 0,
                            st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, rc, adaptive, luh_upd, f32, g);
@@ -1445,10 +1454,12 @@ bool line_kernels_serve(const KernelCfg& k) {
   return !k.guess || (k.precision != 0 && k.picard_max > 0 && (k.pred_variant == 0 || k.pred_variant == 3));
 }
 
-bool predictor_uses_guard(const KernelCfg& k) {
+bool predictor_supports_range(const KernelCfg& k) {
   return line_kernels_serve(k) && k.precision == 1 && k.picard_max == 1 && k.guess &&
-         (k.pred_variant == 0 || k.pred_variant == 3) && k.guard_tol > 0;
+         (k.pred_variant == 0 || k.pred_variant == 3);
 }
+
+bool predictor_uses_guard(const KernelCfg& k) { return predictor_supports_range(k) && k.guard_tol > 0; }
 
 bool predictor_f32_faces_supported(const KernelCfg& k) {
   return line_kernels_serve(k) && k.precision != 0 && k.pred_variant != 1;

@@ -120,6 +120,7 @@ _sig("exa_mesh_picard_guard", [_cp, ctypes.POINTER(ctypes.c_long)])
 _sig("exa_mesh_stream", [_cp], ctypes.c_void_p)
 _sig("exa_mesh_set_timing", [_cp, ctypes.c_int])
 _sig("exa_mesh_set_fused", [_cp, ctypes.c_int])
+_sig("exa_mesh_set_overlap", [_cp, ctypes.c_int])
 _sig("exa_mesh_phase_times", [_cp, _dp, _dp, _dp, _ip])
 _I, _D = ctypes.c_int, ctypes.c_double
 _u8p = ctypes.POINTER(ctypes.c_ubyte)
@@ -457,6 +458,12 @@ class Mesh:
     def set_fused(self, enabled):
         """Two-kernel fused step (default where compiled) or the four-kernel form."""
         _check(_lib.exa_mesh_set_fused(self.h, int(bool(enabled))))
+
+    def set_overlap(self, slabs):
+        """Slab-pipelined fused step: `slabs` (>= 3) slabs of z planes, the corrector
+        of a slab beside the predictors of the later ones; 0 switches it off.
+        3D fused periodic meshes with the one-iteration FP32 CERK predictor."""
+        _check(_lib.exa_mesh_set_overlap(self.h, int(slabs)))
 
     def set_boundary(self, kinds):
         """Boundary kind per domain face [2*d + o] (BC_*)."""

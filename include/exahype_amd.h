@@ -405,6 +405,17 @@ int exa_mesh_picard_guard(exa_mesh* mesh, long* recomputed);
  * phases (predictor into lduh, Riemann per face, face integrals + update, dt)
  * as separate kernels.  Results agree to rounding. */
 int exa_mesh_set_fused(exa_mesh* mesh, int enabled);
+/* Slab-pipelined fused step (no counterpart in the reference: a schedule of
+ * this library's two step kernels).  slabs >= 3 splits the mesh into slabs of
+ * z planes; the predictors run slab by slab on the mesh stream and the fused
+ * corrector of a slab runs on a second stream once the predictors of its
+ * neighbour slabs are done, beside the predictors of the later slabs.  The
+ * results equal the one-launch step's bit for bit.  Applies to 3D fused steps
+ * with all faces periodic and the one-iteration FP32 CERK predictor (else
+ * EXA_ERR_UNSUPPORTED), 3 <= slabs <= n; 0 switches it off (default).  Runs
+ * with timing enabled (exa_mesh_set_timing) and the split-step entry points
+ * use the one-launch step. */
+int exa_mesh_set_overlap(exa_mesh* mesh, int slabs);
 /* Boundary kind per domain face kinds[2*d + o] (o = 0 at x_d = 0, o = 1 at
  * x_d = n h), EXA_BC_*; default all periodic.  A direction is periodic on both
  * faces or on neither.  Non-periodic faces follow the reference's
