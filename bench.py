@@ -14,6 +14,12 @@ Two forms run in the same process on the same initial state:
              corrector, solution update and CFL reduction in FP64.
 `value` is the FP32 form's DoF-updates/s; the FP64 form, the per-variable
 relative L2 error of FP32 vs FP64 and the speed-up are reported beside it.
+With --slabs S the FP32 form's timed steps run slab-pipelined (S slabs of z
+planes: the corrector of a slab on a second stream beside the predictors of
+the later slabs, bitwise the one-launch step's results); its per-kernel times
+and the roofline then come from the same steps re-run with one launch per
+phase.  Default 0: one launch per phase (the pipelined step measured within
+0-2 % of it).
 
 `--workload batched` measures the batched uneven workload: a static
 two-level 3D Euler mesh of order 4 (27^3 coarse cells, those within 0.25 of
@@ -99,12 +105,14 @@ def executed_iterations(counts, dim, max_iter, adaptive):
     return np.where(counts == max_iter + 2, counts, max_iter)  # 2D convention loses the index
 
 
-def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, variant=0, guess=0):
+def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, variant=0, guess=0, slabs=0):
     dim, V = 3, 5
     ctx = E.Context(dim, order, V, 0, precision=precision, picard_max=picard)
     ctx.set_predictor_variant(variant)
     ctx.set_initial_guess(guess)
     mesh = E.Mesh(ctx, n, 1.0 / n)
+    if slabs:
+        mesh.set_overlap(slabs)  # slab-pipelined step (runs with timing events use the one-launch step)
     prm = P.density_wave_params(dim, seed, modes)
     # warm-up steps on the same state, then restart from the initial condition
     mesh.adjust_solution(prm)
@@ -114,17 +122,31 @@ def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, vari
         mesh.sync()
     mesh.adjust_solution(prm)
     mesh.init_dt(0.0)
-    mesh.set_timing(True)  # HIP events around each kernel on the launch stream, no syncs
+    if not slabs:
+        mesh.set_timing(True)  # HIP events around each kernel on the launch stream, no syncs
     mesh.sync()
     t0 = time.perf_counter()
     mesh.run(steps)
     mesh.sync()
     wall = time.perf_counter() - t0
-    ph = mesh.phase_times()
     pic = mesh.picard()
     recomputed = mesh.picard_guard_recomputed()
     t_end, dt_next = mesh.time()
     sol = mesh.get_solution()
+    one_launch = None
+    if slabs:
+        # per-kernel times: the same steps from the same state in the one-launch step with HIP
+        # events around each kernel (the kernels' own rates, not shared with the other stream)
+        mesh.adjust_solution(prm)
+        mesh.init_dt(0.0)
+        mesh.set_timing(True)
+        mesh.sync()
+        t1 = time.perf_counter()
+        mesh.run(steps)
+        mesh.sync()
+        one_launch = dict(ms_per_step=(time.perf_counter() - t1) * 1e3 / steps,
+                          bitwise_equal=bool(np.array_equal(mesh.get_solution(), sol)))
+    ph = mesh.phase_times()
     mesh.close()
     ctx.close()
     B = order + 1
@@ -168,7 +190,7 @@ def run_form(E, P, n, order, steps, warmup, precision, picard, seed, modes, vari
                       achieved_executed=executed / (pred_ms * 1e-3) / 1e12,
                       kernel_ms=pred_ms,
                       effective_with_cerk=(flops + cerk_flops) / (pred_ms * 1e-3) / 1e12),
-        t_end=t_end, dt_next=dt_next, solution=sol)
+        t_end=t_end, dt_next=dt_next, solution=sol, slabs=slabs, one_launch_step=one_launch)
 
 
 def cpu_threads():
@@ -444,6 +466,9 @@ def main():
     ap.add_argument("--acoustic-predictor", type=int, default=0, choices=(0, 1),
                     help="acoustic forward predictor: 0 x-line kernel, 1 node kernel")
     ap.add_argument("--modes", type=int, default=8)
+    ap.add_argument("--slabs", type=int, default=0,
+                    help="FP32 one-iteration CERK form: z-plane slabs of the pipelined step (the corrector of a "
+                         "slab beside the predictors of the later ones on a second stream); 0: one launch per phase")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-n", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=None)
@@ -527,8 +552,13 @@ def main():
     # FP32 fixed count: one iteration with the CERK guess (cells the Picard guard
     # flags are recomputed with three), else the FP64 form's largest executed count
     k32 = args.picard_fp32 if args.picard_fp32 > 0 else (1 if cerk else r64["picard_max_executed"])
+    # slab-pipelined step where it applies (one-iteration CERK form)
+    slabs = args.slabs if (cerk and k32 == 1 and args.slabs >= 3) else 0
     r32 = run_form(E, P, args.n, args.order, args.steps, args.warmup, E.PREC_FP32, k32, args.seed, args.modes,
-                   guess=1 if cerk else 0)
+                   guess=1 if cerk else 0, slabs=slabs)
+    if r32["one_launch_step"]:
+        log(f"[bench] fp32 one launch per phase: {r32['one_launch_step']['ms_per_step']:.2f} ms/step, "
+            f"bitwise equal to the {r32['slabs']}-slab step: {r32['one_launch_step']['bitwise_equal']}")
     log(f"[bench] fp32: {r32['value']:.4e} DoF/s, {r32['ms_per_step']:.2f} ms/step "
         f"(pred {r32['predictor_ms']:.2f} riem {r32['riemann_ms']:.2f} corr {r32['corrector_ms']:.2f}), "
         f"guess {args.guess}, picard {k32}, guard recomputed {r32['guard_recomputed_cells']} cells, "
@@ -597,7 +627,7 @@ def main():
                         + ", FP64 Riemann/corrector/update/CFL; errors vs the FP64 generic kernels (trivial guess, "
                           "adaptive Picard)",
             "cells": args.n ** 3, "order": args.order, "picard_fp32": k32, "guess_fp32": args.guess,
-            "seed": args.seed,
+            "seed": args.seed, "slabs": r32["slabs"],
         },
         "roofline": r32["roofline"],
         "cpu_baseline": cpu,
@@ -606,7 +636,12 @@ def main():
         "fp32_phases_ms": {k: r32[k] for k in ("predictor_ms", "riemann_ms", "corrector_ms")},
         "step_form": "two kernels per step: predictor (+ volume part of the update), then one corrector "
                      "kernel (Riemann problems of every cell's faces + face integrals + update + CFL); "
-                     "riemann_ms is the boundary-condition slot between them (periodic: empty)",
+                     "riemann_ms is the boundary-condition slot between them (periodic: empty)"
+                     + (f"; timed in {r32['slabs']} slabs of z planes, the corrector of a slab on a second stream "
+                        f"beside the predictors of the later slabs (fp32_phases_ms and the roofline: the same steps "
+                        f"re-run with one launch per phase and HIP events, fp32_one_launch_step)"
+                        if r32["slabs"] else ""),
+        "fp32_one_launch_step": r32["one_launch_step"],
         "fp32_guard_recomputed_cells": r32["guard_recomputed_cells"],
         "speedup_fp32_vs_fp64": r32["value"] / r64["value"],
         "rel_l2_err_fp32_vs_fp64": err,

@@ -554,7 +554,7 @@ struct GuardArgs {
   double tol2 = 0.0;
 };
 
-template <int B, typename R, bool MF, bool FIX, bool CG, bool ONE = false, bool LIST = false>
+template <int B, typename R, bool MF, bool FIX, bool CG, bool ONE = false, bool LIST = false, bool PART = false>
 // Fixed-count form: four workgroups per CU (LDS 39 KB each; the register
 // budget of 128 costs a few spills, measured faster than three workgroups)
 __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) void predictor3d_lines_kernel(
@@ -586,6 +586,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
   // buffer holds the CERK stage fluxes / Fc_d [3][NS][V] and lQhi [NS][V], then af
   static_assert(!ONE || FIX, "one collapsed iteration: fixed-count form");
   static_assert(!LIST || (FIX && CG && !ONE), "cell list: the fixed-count CERK form");
+  static_assert(!PART || (ONE && CG && !LIST), "a part of the mesh: the one-iteration CERK form");
   constexpr int ONE_BYTES = 4 * NS * V * 4;
   constexpr int NEED = ONE ? ONE_BYTES + AF_BYTES
                           : (FIX ? TILE_BYTES + AF_BYTES : (sizeof(R) == 4 ? 2 * TILE_BYTES : TILE_BYTES));
@@ -1069,7 +1070,9 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
       __syncthreads();  // the next cell rewrites the LDS buffers
     }
   } else {
-    cell_work(cell0 + blockIdx.x);  // the launch covers cells [cell0, cell0 + gridDim.x)
+    // PART: the launch covers cells [cell0, cell0 + gridDim.x) (its own instantiation,
+    // so that profiles tell the launches over a part of the mesh from whole-mesh ones)
+    cell_work(PART ? cell0 + blockIdx.x : blockIdx.x);
   }
 }
 
@@ -1418,8 +1421,13 @@ int launch_predictor3d_lines(const KernelCfg& k, hipStream_t st, long n, const d
         g.guard = guard;
         g.tol2 = k.guard_tol * k.guard_tol;
       }
-      hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true, true>), grid, block, 0, st, n, luh, h,
-                         dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32, g, cell0);
+      if (range)
+        hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true, true, false, true>), grid, block, 0,
+                           st, n, luh, h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32,
+                           g, cell0);
+      else
+        hipLaunchKernelGGL((predictor3d_lines_kernel<B, float, true, true, true, true>), grid, block, 0, st, n, luh,
+                           h, dt, cinst, lduh, qb, fb, pic, k.tab, k.pde, maxIter, adaptive, luh_upd, f32, g);
       if (g.guard) {
         // persistent grid over the list: four workgroups per CU of the fixed-count
         // kernel (a part of the mesh: one per CU, the list is usually empty)
