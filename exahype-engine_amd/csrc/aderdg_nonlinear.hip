@@ -1099,14 +1099,30 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
   if constexpr (ONE) {
     if (valid && s < NTASK) {
       const int task = s;
-      const int f = task / NF, fn = task - f * NF, o = f & 1;
-      const long g = ghost_face<D>(cxyz, n, f, bcmask);
-      const long nb = g >= 0 ? 0 : periodic_neighbour<D>(cxyz, n, f >> 1, o ? 1 : -1);
+      const int f = task / NF, fn = task - f * NF;
+      // the other side of face f: face block nb * 2D + (f ^ 1) of the neighbour
+      // (ob >= 0) or ghost block g (ob = -1 - g); GRID3: the cell is the
+      // workgroup's, so the six are uniform and a lane selects its face's
+      auto other = [&](int ff) -> long {
+        const long g = ghost_face<D>(cxyz, n, ff, bcmask);
+        return g >= 0 ? -1 - g : periodic_neighbour<D>(cxyz, n, ff >> 1, (ff & 1) ? 1 : -1) * 2 * D + (ff ^ 1);
+      };
+      long ob;
+      if constexpr (GRID3) {
+        ob = other(0);
+#pragma unroll
+        for (int ff = 1; ff < 2 * D; ff++) {
+          const long b = other(ff);
+          ob = f == ff ? b : ob;
+        }
+      } else {
+        ob = other(f);
+      }
       // face blocks (NF nodes each) of this cell and of the neighbour or ghost
       const TF* qo = lQhbnd + (cell * 2 * D + f) * NF * ND;
-      const TF* qn = g >= 0 ? gq + g * NF * ND : lQhbnd + (nb * 2 * D + (f ^ 1)) * NF * ND;
+      const TF* qn = ob < 0 ? gq + (-1 - ob) * NF * ND : lQhbnd + ob * NF * ND;
       const TF* fo = lFhbnd + (cell * 2 * D + f) * NF * V;
-      const TF* fnb = g >= 0 ? gf + g * NF * V : lFhbnd + (nb * 2 * D + (f ^ 1)) * NF * V;
+      const TF* fnb = ob < 0 ? gf + (-1 - ob) * NF * V : lFhbnd + ob * NF * V;
       // this cell's side and the other side (the left state is the other
       // side's for o = 0, this cell's for o = 1)
 #pragma unroll
@@ -1182,10 +1198,19 @@ __global__ __launch_bounds__((Shape<D, B, V, NP>::THREADS)) void corrector_riema
   }
   __syncthreads();
   }
-  // Rusanov speed of each face from its two averages (riemannSolverNonlinear.cpph:51-72)
-  if (valid && s < 2 * D)
+  // Rusanov speed of each face from its two averages (riemannSolverNonlinear.cpph:51-72).
+  // Even NS: lane s < 4D takes side s & 1 of face s >> 1 and the pair's maximum
+  // comes from a lane swap (tid and tid ^ 1 are the cell's lanes s and s ^ 1)
+  if constexpr (NS % 2 == 0) {
+    if (valid && s < 4 * D) {
+      const double e = pde_max_abs_eig<D, V>(P, avg + s * ND, s >> 2);
+      const double m = fmax(e, __shfl_xor(e, 1, 64));
+      if (!(s & 1)) sm[s >> 1] = m;
+    }
+  } else if (valid && s < 2 * D) {
     sm[s] = fmax(pde_max_abs_eig<D, V>(P, avg + (s * 2 + 0) * ND, s >> 1),
                  pde_max_abs_eig<D, V>(P, avg + (s * 2 + 1) * ND, s >> 1));
+  }
   __syncthreads();
   // phase 3: fluxes into red[task][V] (red's partial products were consumed
   // in phase 2; phase 3 reads only sm and the face data)
