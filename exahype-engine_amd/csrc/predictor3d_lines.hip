@@ -59,6 +59,17 @@ __device__ __forceinline__ const X* opaque(const X* p) {
   return reinterpret_cast<const X*>((static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
+// Index of a line's first value in an LDS array, held in a register of its
+// own: the line's reads then differ from it by immediate offsets, and the
+// compiler pairs them into ds_read2_b32 off this one register.  A base it can
+// see through gets folded into the offsets, which overflow ds_read2's 8-bit
+// field, and each pair then costs a vector add for a fresh base.  x lines (B^2
+// points per step) take one per two points.
+__device__ __forceinline__ int line_base(int i) {
+  asm volatile("" : "+v"(i));
+  return i;
+}
+
 // Makes register values opaque to the optimiser at a phase boundary.  Without
 // it the flux products irho*Q[k] of the x, y, z and averaging phases are
 // common subexpressions and stay live across the whole Picard iteration
@@ -471,16 +482,19 @@ __device__ __forceinline__ void finish_from_averages(const float* aq, const floa
     for (int d = 0; d < 3; d++) {
       const float tw = (d == 0) ? T->wf[z] * T->wf[y] : (d == 1 ? T->wf[z] * T->wf[x] : T->wf[y] * T->wf[x]);
       const int c = (d == 0) ? x : (d == 1 ? y : z);
-      const int stride = lds_stride<B>(d);
-      const float* line = af + (d * NS + l - c * stride) * V;
+      const int stride = lds_stride<B>(d) * V;  // [point][v] stride along d
+      const int lb = line_base((d * NS + l) * V - c * stride);
+      int pb = lb;
       float part[V];
 #pragma unroll
       for (int v = 0; v < V; v++) part[v] = 0.f;
 #pragma unroll
       for (int m = 0; m < B; m++) {
         const float k = T->Kxif[c * B + m];
+        if (d == 0 && m > 0 && !(m & 1)) pb = line_base(lb + m * stride);  // x: a base per two points
+        const int nb = d == 0 ? pb + (m & 1) * stride : lb + m * stride;
 #pragma unroll
-        for (int v = 0; v < V; v++) part[v] += k * line[m * stride * V + v];
+        for (int v = 0; v < V; v++) part[v] += k * af[nb + v];
       }
 #pragma unroll
       for (int v = 0; v < V; v++) du[v] += part[v] * (tw * invDx);
@@ -670,8 +684,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
     // lQi(t) = luh - dt xi_t K1 - dt xi_t^2 (K2 - K1) / 2.  Lane l works on its
     // time-line point p = (z, y, x = t) (u = luh there); the stage fluxes go
     // through the tile, which the Picard loop has not used yet.
-    float* fl = tl;  // [3][V][NS]: variable-major, a line read at fixed v is a point-strided row
-    const int pt = pv / V;
+    float* fl = tl;  // [3][NS][V], points in the LDS point order
     float K1[V], K2[V], lw[V];
 #pragma unroll
     for (int v = 0; v < V; v++) lw[v] = u[v];
@@ -685,7 +698,7 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
           float F[V];
           euler_flux_dir<float>(lw, ir, pr, d, F);
 #pragma unroll
-          for (int v = 0; v < V; v++) fl[(d * V + v) * NS + pt] = F[v];
+          for (int v = 0; v < V; v++) fl[d * NS * V + pv + v] = F[v];
         }
       }
       __syncthreads();
@@ -695,14 +708,19 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           const int c = d == 0 ? t : (d == 1 ? y : z);              // coordinate of p along d
-          const int stride = lds_stride<B>(d);                        // point stride along d
-          const float* line = fl + d * V * NS + pt - c * stride;
+          const int stride = lds_stride<B>(d) * V;                    // [point][v] stride along d
+          // dudx[c][n] from the operator columns in LDS (opl[0][r][hh B + n] = dudx[4 hh + r][n])
+          float dr[B];
+#pragma unroll
+          for (int n = 0; n < B; n++) dr[n] = opl[0][c & 3][(c >> 2) * B + n];
+          const int lb = line_base(d * NS * V + pv - c * stride);
+          int pb = lb;
 #pragma unroll
           for (int n = 0; n < B; n++) {
-            // dudx[c][n] from the operator columns in LDS (opl[0][r][hh B + n] = dudx[4 hh + r][n])
-            const float dn = opl[0][c & 3][(c >> 2) * B + n];
+            if (d == 0 && n > 0 && !(n & 1)) pb = line_base(lb + n * stride);  // x: a base per two points
+            const int nb = d == 0 ? pb + (n & 1) * stride : lb + n * stride;
 #pragma unroll
-            for (int v = 0; v < V; v++) K[v] += line[v * NS + n * stride] * dn;
+            for (int v = 0; v < V; v++) K[v] += fl[nb + v] * dr[n];
           }
         }
       }
@@ -997,12 +1015,17 @@ __global__ __launch_bounds__((LShape<B>::THREADS), (ONE ? 5 : (FIX ? 4 : 1))) vo
       for (int d = 0; d < 3; d++) {
         const int c = d == 0 ? px : (d == 1 ? py : pz);              // coordinate of p along d
         const int stride = lds_stride<B>(d) * V;                      // [point][v] stride along d
-        const float* line = fcb + d * NS * V + pv - c * stride;
+        float dr[B];                                                  // dudx[c][n]
+#pragma unroll
+        for (int n = 0; n < B; n++) dr[n] = opl[0][c & 3][(c >> 2) * B + n];
+        const int lb = line_base(d * NS * V + pv - c * stride);
+        int pb = lb;
 #pragma unroll
         for (int n = 0; n < B; n++) {
-          const float dn = opl[0][c & 3][(c >> 2) * B + n];           // dudx[c][n]
+          if (d == 0 && n > 0 && !(n & 1)) pb = line_base(lb + n * stride);  // x: a base per two points
+          const int nb = d == 0 ? pb + (n & 1) * stride : lb + n * stride;
 #pragma unroll
-          for (int v = 0; v < V; v++) dv[v] += dn * line[n * stride + v];
+          for (int v = 0; v < V; v++) dv[v] += dr[n] * fcb[nb + v];
         }
       }
 #pragma unroll
