@@ -156,7 +156,41 @@ __device__ __forceinline__ void node_volume_and_faces(
   }
 
   // ---- extrapolation to the 2D faces (faces zeroed, then FCoeff sums) ----
-  {
+  if constexpr (D == 3) {
+    // a lane takes one (face node, value) pair and gives both faces of the
+    // three directions: the pair is decoded once, the line strides are
+    // compile-time constants; outputs in the reference layout [face][node][value]
+    constexpr int NPAIR = NF * ND;
+    for (int pr = s; pr < NPAIR; pr += NS) {
+      const int v = pr % ND, fn = pr / ND;
+      const int a = fn / B, b = fn % B;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        // the line through face node (a, b) along d: node n0 + l stride, (a, b) = (z, y), (z, x), (y, x)
+        const int n0 = d == 0 ? (a * B + b) * B : (d == 1 ? a * B * B + b : a * B + b);
+        constexpr int st[3] = {1, B, B * B};
+        double q0 = 0.0, q1 = 0.0, f0 = 0.0, f1 = 0.0;
+#pragma unroll
+        for (int l = 0; l < B; l++) {
+          const int node = n0 + l * st[d];
+          const double qv = Qh[node * ND + v];
+          q0 += qv * T->F0[l];
+          q1 += qv * T->F1[l];
+          if (v < V) {
+            const double fv = Fh[(d * NS + node) * V + v];
+            f0 += fv * T->F0[l];
+            f1 += fv * T->F1[l];
+          }
+        }
+        lQhbnd[cell * (2 * D * NF * ND) + ((2 * d) * NF + fn) * ND + v] = q0;
+        lQhbnd[cell * (2 * D * NF * ND) + ((2 * d + 1) * NF + fn) * ND + v] = q1;
+        if (v < V) {
+          lFhbnd[cell * (2 * D * NF * V) + ((2 * d) * NF + fn) * V + v] = f0;
+          lFhbnd[cell * (2 * D * NF * V) + ((2 * d + 1) * NF + fn) * V + v] = f1;
+        }
+      }
+    }
+  } else {
     constexpr int NQ = 2 * D * NF * ND;
     for (int o = s; o < NQ; o += NS) {
       const int v = o % ND;
