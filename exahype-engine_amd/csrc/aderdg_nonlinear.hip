@@ -135,10 +135,12 @@ __device__ __forceinline__ void node_volume_and_faces(
       const double upd = (D == 3 ? ta[d] * tb[d] : ta[d]) * invDx[d];
       const int c = cc[d];
 #pragma unroll
-      for (int v = 0; v < V; v++)
+      for (int m = 0; m < B; m++) {
+        // Kxi[c][m] times the weight factor once per (d, m): one FMA per flux value
+        const double k = T->Kxi[c * B + m] * upd;
 #pragma unroll
-        for (int m = 0; m < B; m++)
-          du[v] += T->Kxi[c * B + m] * Fh[(d * NS + base[d] + m * stride_of<D, B>(d)) * V + v] * upd;
+        for (int v = 0; v < V; v++) du[v] += k * Fh[(d * NS + base[d] + m * stride_of<D, B>(d)) * V + v];
+      }
     }
     if (du_add) {  // source / NCP: lduh += W lShi (volumeIntegralNonlinear.cpph:204-222)
 #pragma unroll
@@ -501,6 +503,13 @@ __device__ __forceinline__ void predictor_tile(
     return !__syncthreads_or(active);
   };
 
+  // The time solve lQi_new[t'] = sum_t iK1[t'][t] rhs[t] / W: the 1/W is folded
+  // into the right-hand side's factors -- the stiffness factor of the node per
+  // direction, (ta tb dt / dx)_d / W (a time slice scales it by w_t), and the
+  // F0 term's W F0[t] / W
+  R cd[D];
+#pragma unroll
+  for (int d = 0; d < D; d++) cd[d] = static_cast<R>(ta[d] * tb[d] * dt * invDx[d] * iweight);
   int first = 0;
   if constexpr (!CG && !NCS) {
     // Trivial guess: the B time slices of the first iterate all equal luh, so
@@ -539,7 +548,7 @@ __device__ __forceinline__ void predictor_tile(
         for (int v = 0; v < V; v++) g[v] = R(0);
 #pragma unroll
         for (int d = 0; d < D; d++) {
-          const R upd = static_cast<R>(ta[d] * tb[d] * dt * invDx[d]);
+          const R upd = cd[d];
 #pragma unroll
           for (int n = 0; n < B; n++) {
             const R ukc = upd * kxs[n * B + cc[d]];
@@ -547,16 +556,15 @@ __device__ __forceinline__ void predictor_tile(
             for (int v = 0; v < V; v++) g[v] += ukc * slab[sidx(base[d] + n * stride_of<D, B>(d), d, v)];
           }
         }
-        const R iw = static_cast<R>(iweight);
 #pragma unroll
         for (int t = 0; t < B; t++) {
-          const R rw = static_cast<R>(W * T->F0[t]);
+          const R rw = static_cast<R>(W * T->F0[t] * iweight);
           const R wt = static_cast<R>(w[t]);
 #pragma unroll
           for (int v = 0; v < V; v++) {
             const R r = rw * static_cast<R>(u[v]) - wt * g[v];
 #pragma unroll
-            for (int tp = 0; tp < B; tp++) acc[tp][v] += iw * r * tabI<R>(T, tp * B + t);
+            for (int tp = 0; tp < B; tp++) acc[tp][v] += r * tabI<R>(T, tp * B + t);
           }
         }
 #pragma unroll
@@ -618,14 +626,14 @@ __device__ __forceinline__ void predictor_tile(
       }
       __syncthreads();
       if (active) {
-        // rhs0 = weight * F0[t] * luh, then minus the x, y, z stiffness terms
-        const R rw = static_cast<R>(W * T->F0[t]);
+        // rhs0 / W = F0[t] luh (as W F0[t] / W), then minus the x, y, z stiffness terms
+        const R rw = static_cast<R>(W * T->F0[t] * iweight);
         R r[V];
 #pragma unroll
         for (int v = 0; v < V; v++) r[v] = rw * static_cast<R>(u[v]);
 #pragma unroll
         for (int d = 0; d < D; d++) {
-          const R upd = static_cast<R>(w[t] * ta[d] * tb[d] * dt * invDx[d]);
+          const R upd = static_cast<R>(w[t]) * cd[d];
 #pragma unroll
           for (int n = 0; n < B; n++) {
             // Kxi[n][c_d] times the update factor, once per (d, n): one FMA per
@@ -655,7 +663,7 @@ __device__ __forceinline__ void predictor_tile(
             }
           R Sv[V];
           pde_ncs<D, R>(P, q[t], gq, Sv);
-          const R us = static_cast<R>(w[t] * W * dt);
+          const R us = static_cast<R>(w[t] * W * dt * iweight);
           const R wt = static_cast<R>(w[t]);
 #pragma unroll
           for (int v = 0; v < V; v++) {
@@ -663,12 +671,11 @@ __device__ __forceinline__ void predictor_tile(
             sh[v] += Sv[v] * wt;
           }
         }
-        // (K1)^-1 in time: lQi_new[t'] += iweight * rhs[t] * iK1[t'][t]
-        const R iw = static_cast<R>(iweight);
+        // (K1)^-1 in time: lQi_new[t'] += rhs[t] / W * iK1[t'][t] (r holds rhs / W)
 #pragma unroll
         for (int tp = 0; tp < B; tp++)
 #pragma unroll
-          for (int v = 0; v < V; v++) acc[tp][v] += iw * r[v] * tabI<R>(T, tp * B + t);
+          for (int v = 0; v < V; v++) acc[tp][v] += r[v] * tabI<R>(T, tp * B + t);
       }
     }
     // residual and update of lQi (variables only; parameters are carried)
