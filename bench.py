@@ -56,8 +56,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per predictor launch at 81^3, order 5 (n, order, precision, CERK guess), measured with
 # rocprofv3 FETCH_SIZE (x2, the factor a 1 GiB copy calibrates) + WRITE_SIZE (x1):
 # profiles/euler3d_n5_81cubed_kernels.md.
-MEASURED_PREDICTOR_TRAFFIC = {(81, 5, "fp32", 1): 14.72e9, (81, 5, "fp32", 0): 17.59e9,
-                              (81, 5, "fp64", 1): 18.46e9, (81, 5, "fp64", 0): 18.68e9}
+MEASURED_PREDICTOR_TRAFFIC = {(81, 5, "fp32", 1): 14.68e9, (81, 5, "fp32", 0): 17.85e9,
+                              (81, 5, "fp64", 1): 18.46e9, (81, 5, "fp64", 0): 18.73e9}
 
 
 def log(*a):
