@@ -517,11 +517,7 @@ __device__ __forceinline__ void predictor_tile(
     // rhs[t] = W F0[t] luh - w[t] sum_d sum_n (ta tb dt / dx)_d Kxi[n][c_d] F_d(luh).
     // Its squared update is taken against luh, so the B copies of luh are never held.
     if (maxIter > 0) {
-      R acc[B][V];
-#pragma unroll
-      for (int t = 0; t < B; t++)
-#pragma unroll
-        for (int v = 0; v < V; v++) acc[t][v] = R(0);
+      R acc[B][V];  // time slice 0 assigns, the others accumulate (active lanes only)
       R* slab = reinterpret_cast<R*>(buf) + cslot * SLAB;
       if (active) {
         R F[D][V];
@@ -564,7 +560,12 @@ __device__ __forceinline__ void predictor_tile(
           for (int v = 0; v < V; v++) {
             const R r = rw * static_cast<R>(u[v]) - wt * g[v];
 #pragma unroll
-            for (int tp = 0; tp < B; tp++) acc[tp][v] += r * tabI<R>(T, tp * B + t);
+            for (int tp = 0; tp < B; tp++) {
+              if (t == 0)
+                acc[tp][v] = r * tabI<R>(T, tp * B + t);
+              else
+                acc[tp][v] += r * tabI<R>(T, tp * B + t);
+            }
           }
         }
 #pragma unroll
@@ -581,22 +582,11 @@ __device__ __forceinline__ void predictor_tile(
   }
 
   for (int iter = first; iter < maxIter; iter++) {
+    // the new iterate and the time-averaged flux of this iteration: time slice 0
+    // assigns, the others accumulate (active lanes only)
     R acc[B][V];
-#pragma unroll
-    for (int t = 0; t < B; t++)
-#pragma unroll
-      for (int v = 0; v < V; v++) acc[t][v] = R(0);
-    if (active) {
-#pragma unroll
-      for (int d = 0; d < D; d++)
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-          if constexpr (FH_LDS)
-            fhs[(d * NS + s) * V + v] = 0.0;
-          else
-            fh[d][v] = R(0);
-        }
-      if constexpr (NCS) {
+    if constexpr (NCS) {
+      if (active) {
 #pragma unroll
         for (int v = 0; v < V; v++) sh[v] = R(0);
       }
@@ -613,10 +603,17 @@ __device__ __forceinline__ void predictor_tile(
         for (int d = 0; d < D; d++)
 #pragma unroll
           for (int v = 0; v < V; v++) {
-            if constexpr (FH_LDS)
-              fhs[(d * NS + s) * V + v] += F[d][v] * wt;  // own node: no other lane touches it
-            else if (keep(d, v))
-              fh[d][v] += F[d][v] * wt;
+            if constexpr (FH_LDS) {
+              if (t == 0)
+                fhs[(d * NS + s) * V + v] = F[d][v] * wt;  // own node: no other lane touches it
+              else
+                fhs[(d * NS + s) * V + v] += F[d][v] * wt;
+            } else if (keep(d, v)) {
+              if (t == 0)
+                fh[d][v] = F[d][v] * wt;
+              else
+                fh[d][v] += F[d][v] * wt;
+            }
             if (keep(d, v)) slab[sidx(s, d, v)] = F[d][v];
           }
         if constexpr (NCS) {
@@ -675,7 +672,12 @@ __device__ __forceinline__ void predictor_tile(
 #pragma unroll
         for (int tp = 0; tp < B; tp++)
 #pragma unroll
-          for (int v = 0; v < V; v++) acc[tp][v] += r[v] * tabI<R>(T, tp * B + t);
+          for (int v = 0; v < V; v++) {
+            if (t == 0)
+              acc[tp][v] = r[v] * tabI<R>(T, tp * B + t);
+            else
+              acc[tp][v] += r[v] * tabI<R>(T, tp * B + t);
+          }
       }
     }
     // residual and update of lQi (variables only; parameters are carried)
