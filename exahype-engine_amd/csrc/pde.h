@@ -66,7 +66,18 @@ __device__ __forceinline__ void pde_flux(const PdeParams& P, const R* __restrict
   }
   if constexpr (V == 5) {
     constexpr R gamma = static_cast<R>(1.4);
-    const R irho = R(1) / Q[0];
+    R irho;
+    if constexpr (sizeof(R) == 8) {
+      // 1/rho: hardware reciprocal and two Newton steps (error <= 1 ulp; the
+      // IEEE division's scaling and fix-up steps cost twice the instructions)
+      irho = __builtin_amdgcn_rcp(Q[0]);
+      R e = __builtin_fma(-Q[0], irho, R(1));
+      irho = __builtin_fma(irho, e, irho);
+      e = __builtin_fma(-Q[0], irho, R(1));
+      irho = __builtin_fma(irho, e, irho);
+    } else {
+      irho = R(1) / Q[0];
+    }
     const R j2 = (D == 3) ? Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3] : Q[1] * Q[1] + Q[2] * Q[2];
     const R p = (gamma - R(1)) * (Q[4] - R(0.5) * irho * j2);
     F[0][0] = Q[1];
